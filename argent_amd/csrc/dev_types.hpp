@@ -1,0 +1,138 @@
+// Types shared by the HIP kernels and the host runtime that launches them.
+// Everything here is plain data laid out for the device: scalars of a scene
+// live in one DScene block in device memory (read with scalar loads), the
+// spectral tables in one float blob, and path state in structure-of-arrays
+// buffers sized for one wave of paths.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/argent.h"
+
+namespace ag {
+
+constexpr int MAX_KEYS = 2 + AG_MAX_MATERIALS;  // miss, panel, one per material
+constexpr int TRAV_BLOCK = 256;                  // threads per traversal block
+constexpr int STACK_LDS = 16;                    // stack entries per lane in LDS
+constexpr int STACK_SPILL = 64;                  // further entries per thread in HBM
+constexpr int MAX_SEGMENTS = 4096;               // work segments per wave
+
+// Hit codes written by the extension kernel.
+constexpr uint32_t HIT_MISS = 0xFFFFFFFFu;
+constexpr uint32_t HIT_PANEL = 0x80000000u;  // | panel index
+constexpr uint32_t HIT_FLOOR = 0xC0000000u;
+
+struct DMat {
+    int32_t kind;
+    float alpha, eta;
+    const float* tn;       // metal n grid (95)
+    const float* tk;       // metal k grid (95)
+    int32_t has_patina;
+    float p_strength, p_lo, p_hi, p_alpha, p_eta;
+    const float* tpatina;  // 1 nm tables
+    const float* ta;
+    const float* tb;
+    float opal_cell, opal_strength;
+};
+
+struct DPanel {
+    float c[3], hu[3], hv[3], n[3];
+    float area, hu2, hv2;  // area, |hu|^2, |hv|^2
+    const float* emit;     // 1 nm table, null = flag
+    int32_t camera_visible;
+    float feather;
+};
+
+struct DScene {
+    const float4* nodes;
+    const float4* tris;
+    const float* pos;       // n_vertices * 3
+    const float* nrm;       // n_vertices * 3 or null
+    const uint32_t* idx;    // n_triangles * 3
+    const uint16_t* tmat;   // n_triangles
+    const float* cavity;    // n_vertices or null
+    uint32_t n_tris;
+    int32_t n_mat;
+    DMat mat[AG_MAX_MATERIALS];
+    int32_t n_panels;
+    DPanel panel[AG_MAX_PANELS];
+    int32_t n_lights;
+    int32_t lights[AG_MAX_PANELS];
+    float light_cdf[AG_MAX_PANELS];
+    float env_pick;
+    int32_t env_kind;
+    float env_power;
+    const float* env_top;
+    const float* env_hor;
+    const float* env_bot;
+    int32_t env_w, env_h;
+    const float* env_weights;
+    const float* env_marg;
+    const float* env_cond;
+    float env_rs, env_rc, env_rsn, env_rcn;  // sin/cos of +rotation and of -rotation (radians)
+    const float* env_basis;  // 3 x 471
+    int32_t floor_present;
+    float floor_y;
+    int32_t floor_mat;
+    float floor_f0, floor_f1;
+    const float* energy;     // 32 x 32
+    const float* cmf;        // 471 x 3
+};
+
+// One view of a scene: camera, turntable, slot-0 material, film buffers.
+struct DView {
+    float pos[3], fwd[3], right[3], up[3];
+    float half_w, half_h, lens_radius, focus_dist, shift0, shift1;
+    float turn_deg;      // 0 = identity (no rotation, as the reference skips it)
+    float ts, tc;        // sin, cos of turn_deg in radians (host libm)
+    float tsn, tcn;      // sin, cos of -turn_deg
+    int32_t mat0;        // material override index in the view-material block, -1 none
+    uint32_t width, height;
+    uint32_t seed_h;     // hash32(seed ^ 0xa511e9b3)
+    float aspect_sx, aspect_sy;
+    float cam_m[9];
+    float* acc;          // pass accumulators: colour, albedo, normal, each w*h*3
+};
+
+// A contiguous range of path ids: `ns` samples starting at `s0` for pixels
+// [p0, p0 + npix) of view `view`.  Path id = begin + (s - s0) * npix + (p - p0).
+struct DSeg {
+    uint32_t begin;
+    uint32_t view;
+    uint32_t p0, npix;
+    uint32_t s0, ns;
+};
+
+struct DPaths {
+    float4* beta;
+    float4* L;
+    float4* lam;
+    float4* pdf;
+    float* prev_pdf;
+    uint32_t* flags;      // bit0 specular, bit1 rough_seen, bits 8..15 medium material (255 none)
+    uint64_t* rng;        // next_random state
+    uint32_t* pix_seed;
+    uint32_t* si;
+    uint32_t* view;
+    float4* galb;         // guide albedo
+    float4* gnrm;         // guide normal
+};
+
+struct DQueues {
+    float4* ray_o[2];     // (o.xyz, path id bits)
+    float4* ray_d[2];     // (d.xyz, depth bits)
+    float4* hit;          // (t, u, v, code bits)
+    uint32_t* bucket;     // MAX_KEYS-strided buckets of ray indices, capacity cap each
+    float4* sh_o;         // (origin.xyz, path id bits)
+    float4* sh_d;         // panel: (light point.xyz, ignore panel bits); sky: (dir.xyz, -1 bits)
+    float4* sh_c;         // contribution spectrum
+    float4* sh_x;         // (a, p_light, clamp or -1, kind bits)
+    uint32_t* counters;   // per depth: [ray count, shadow count, key counts...]
+    uint32_t cap;
+    int32_t n_keys;
+};
+
+constexpr int CNT_STRIDE = 2 + MAX_KEYS + 2;  // counters per depth slot
+
+}  // namespace ag

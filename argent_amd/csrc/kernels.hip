@@ -1,0 +1,1057 @@
+// Wavefront path tracer and ray queries for gfx950.
+//
+// One wave of paths (up to `cap` path slots) runs as a fixed sequence of
+// kernels per bounce: extend (closest hit over the BVH + panels + floor),
+// shade (material-sorted: emission, BSDF, next-event sample, continuation),
+// shadow (any-hit for the next-event rays).  Every kernel is a persistent
+// grid-stride loop whose trip count it reads from a device counter, so the
+// whole sequence launches without host round trips.  Queues are appended
+// with one atomic per wavefront (ballot + mbcnt prefix).
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "dev_types.hpp"
+#include "kernels.hpp"
+
+namespace ag {
+
+// ------------------------------------------------------------------ wave helpers
+AGD uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Append for every lane with pred set; returns the slot (valid where pred).
+// Must be reached by all lanes of the wavefront.
+AGD uint32_t wave_append(uint32_t* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    uint32_t prefix = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    uint32_t cnt = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (cnt) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        if ((int)lane_id() == leader) base = atomicAdd(counter, cnt);
+        base = __shfl(base, leader);
+    }
+    return base + prefix;
+}
+
+// Append into one of several counters by key; lanes with key < 0 skip.
+AGD uint32_t wave_append_keyed(uint32_t* counters, int key) {
+    uint32_t slot = 0;
+    uint64_t todo = __ballot(key >= 0);
+    while (todo) {
+        int leader = __ffsll((unsigned long long)todo) - 1;
+        int k = __shfl(key, leader);
+        bool mine = key == k && ((todo >> lane_id()) & 1ull);
+        uint64_t mask = __ballot(mine);
+        uint32_t prefix = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        uint32_t base = 0;
+        if ((int)lane_id() == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+        if (mine) slot = base + prefix;
+        todo &= ~mask;
+    }
+    return slot;
+}
+
+// ------------------------------------------------------------------ BVH traversal
+// Bvh::intersect / Bvh::occluded (bvh.rs:159-245) over the 4-wide device BVH.
+// Per-lane stack: STACK_LDS entries in LDS, the rest in a per-thread HBM spill
+// area.  Box tests are widened by a relative 2^-21 on both ends so rounding
+// never culls a box that holds the true closest hit; the triangle test is
+// Moller-Trumbore evaluated exactly as the reference does (no contraction),
+// with ties on t resolved to the lower original triangle index.
+struct TravStack {
+    uint32_t* node;  // LDS [STACK_LDS][blockDim]
+    float* tn;       // LDS [STACK_LDS][blockDim]
+    uint2* spill;    // HBM [STACK_SPILL] for this thread
+    uint32_t tid, nthreads;
+};
+
+AGD void st_push(const TravStack& s, int& sp, uint32_t e, float t) {
+    if (sp < STACK_LDS) {
+        s.node[sp * s.nthreads + s.tid] = e;
+        s.tn[sp * s.nthreads + s.tid] = t;
+    } else if (sp < STACK_LDS + STACK_SPILL) {
+        s.spill[sp - STACK_LDS] = make_uint2(e, __float_as_uint(t));
+    }
+    sp++;
+}
+AGD void st_pop(const TravStack& s, int& sp, uint32_t& e, float& t) {
+    sp--;
+    if (sp < STACK_LDS) {
+        e = s.node[sp * s.nthreads + s.tid];
+        t = s.tn[sp * s.nthreads + s.tid];
+    } else {
+        uint2 v = s.spill[sp - STACK_LDS];
+        e = v.x;
+        t = __uint_as_float(v.y);
+    }
+}
+
+constexpr float BOX_LO = 1.0f - 4.76837158203125e-7f;  // 1 - 2^-21
+constexpr float BOX_HI = 1.0f + 4.76837158203125e-7f;
+
+AGD void slab4(const float4* nd, f3 o, f3 inv, float tmax, float tmin_out[4], uint32_t ch[4]) {
+    float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+    uint4 c = *reinterpret_cast<const uint4*>(&nd[6]);
+    ch[0] = c.x; ch[1] = c.y; ch[2] = c.z; ch[3] = c.w;
+    float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
+    float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float tx0 = (lxa[k] - o.x) * inv.x, tx1 = (hxa[k] - o.x) * inv.x;
+        float ty0 = (lya[k] - o.y) * inv.y, ty1 = (hya[k] - o.y) * inv.y;
+        float tz0 = (lza[k] - o.z) * inv.z, tz1 = (hza[k] - o.z) * inv.z;
+        float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        float tmx = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        tmin *= BOX_LO;
+        tmx = fminf(tmx * BOX_HI, tmax);
+        bool hit = ch[k] != 0xFFFFFFFFu && tmin <= tmx;
+        tmin_out[k] = hit ? tmin : INF_F;
+    }
+}
+
+// bvh.rs:391-409, returning t (or -1) and u, v
+AGD float tri_test(const float4* tp, f3 o, f3 d, float* uo, float* vo, uint32_t* prim) {
+    float4 a = tp[0], b = tp[1], c = tp[2];
+    f3 v0{a.x, a.y, a.z}, e1{a.w, b.x, b.y}, e2{b.z, b.w, c.x};
+    *prim = __float_as_uint(c.y);
+    f3 p = cross(d, e2);
+    float det = dot(e1, p);
+    if (fabsf(det) < 1e-14f) return -1.0f;
+    float inv_det = 1.0f / det;
+    f3 s = o - v0;
+    float u = dot(s, p) * inv_det;
+    if (!(u >= 0.0f && u <= 1.0f)) return -1.0f;
+    f3 q = cross(s, e1);
+    float v = dot(d, q) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return -1.0f;
+    float t = dot(e2, q) * inv_det;
+    *uo = u;
+    *vo = v;
+    return t;
+}
+
+struct TriHit {
+    float t, u, v;
+    uint32_t prim;
+};
+
+AGD TriHit bvh_closest(const float4* __restrict__ nodes, const float4* __restrict__ tris, f3 o, f3 d, float tmax,
+                       const TravStack& st) {
+    TriHit best{tmax, 0.0f, 0.0f, 0xFFFFFFFFu};
+    f3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    int sp = 0;
+    uint32_t cur = 0;  // root (always an inner node)
+    for (;;) {
+        if (!(cur & 0x80000000u)) {
+            float tm[4];
+            uint32_t ch[4];
+            slab4(nodes + (size_t)cur * 8, o, inv, best.t, tm, ch);
+            // sort ascending (network on 4)
+#define CSWAP(i, j)                                       \
+    if (tm[j] < tm[i]) {                                  \
+        float tt = tm[i]; tm[i] = tm[j]; tm[j] = tt;      \
+        uint32_t cc = ch[i]; ch[i] = ch[j]; ch[j] = cc;   \
+    }
+            CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
+#undef CSWAP
+            if (tm[0] != INF_F) {
+                if (tm[3] != INF_F) st_push(st, sp, ch[3], tm[3]);
+                if (tm[2] != INF_F) st_push(st, sp, ch[2], tm[2]);
+                if (tm[1] != INF_F) st_push(st, sp, ch[1], tm[1]);
+                cur = ch[0];
+                continue;
+            }
+        } else {
+            uint32_t first = cur & 0x07FFFFFFu;
+            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
+            for (uint32_t k = 0; k < cnt; k++) {
+                float u, v;
+                uint32_t prim;
+                float t = tri_test(tris + (size_t)(first + k) * 3, o, d, &u, &v, &prim);
+                if (t > 1e-7f && (t < best.t || (t == best.t && prim < best.prim))) {
+                    best.t = t;
+                    best.u = u;
+                    best.v = v;
+                    best.prim = prim;
+                }
+            }
+        }
+        // pop the next entry not beyond the current best
+        bool found = false;
+        while (sp > 0) {
+            float tn;
+            st_pop(st, sp, cur, tn);
+            if (!(tn > best.t)) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return best;
+}
+
+AGD bool bvh_occluded(const float4* __restrict__ nodes, const float4* __restrict__ tris, f3 o, f3 d, float tmax,
+                      const TravStack& st) {
+    f3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    int sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+        if (!(cur & 0x80000000u)) {
+            float tm[4];
+            uint32_t ch[4];
+            slab4(nodes + (size_t)cur * 8, o, inv, tmax, tm, ch);
+            int first = -1;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (tm[k] != INF_F) {
+                    if (first < 0)
+                        first = k;
+                    else
+                        st_push(st, sp, ch[k], tm[k]);
+                }
+            }
+            if (first >= 0) {
+                cur = ch[first];
+                continue;
+            }
+        } else {
+            uint32_t firstt = cur & 0x07FFFFFFu;
+            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
+            for (uint32_t k = 0; k < cnt; k++) {
+                float u, v;
+                uint32_t prim;
+                float t = tri_test(tris + (size_t)(firstt + k) * 3, o, d, &u, &v, &prim);
+                if (t > 1e-7f && t < tmax) return true;
+            }
+        }
+        if (sp == 0) break;
+        float tn;
+        st_pop(st, sp, cur, tn);
+    }
+    return false;
+}
+
+#define TRAV_STACK_DECL                                                     \
+    __shared__ uint32_t s_node[STACK_LDS * TRAV_BLOCK];                     \
+    __shared__ float s_tn[STACK_LDS * TRAV_BLOCK];                          \
+    TravStack st;                                                           \
+    st.node = s_node;                                                       \
+    st.tn = s_tn;                                                           \
+    st.tid = threadIdx.x;                                                   \
+    st.nthreads = blockDim.x;                                               \
+    st.spill = spill + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * STACK_SPILL;
+
+// ------------------------------------------------------------------ ray-batch kernels
+__global__ __launch_bounds__(TRAV_BLOCK) void k_rays_intersect(const float4* __restrict__ nodes,
+                                                               const float4* __restrict__ tris, RaysSoA rays,
+                                                               HitsSoA hits, uint32_t n, uint2* spill) {
+    TRAV_STACK_DECL
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        f3 o{rays.ox[i], rays.oy[i], rays.oz[i]}, d{rays.dx[i], rays.dy[i], rays.dz[i]};
+        float tmax = rays.tmax ? rays.tmax[i] : INF_F;
+        TriHit h = bvh_closest(nodes, tris, o, d, tmax, st);
+        bool hit = h.prim != 0xFFFFFFFFu;
+        hits.t[i] = hit ? h.t : INF_F;
+        hits.u[i] = hit ? h.u : 0.0f;
+        hits.v[i] = hit ? h.v : 0.0f;
+        hits.prim[i] = h.prim;
+    }
+}
+
+__global__ __launch_bounds__(TRAV_BLOCK) void k_rays_occluded(const float4* __restrict__ nodes,
+                                                              const float4* __restrict__ tris, RaysSoA rays,
+                                                              uint8_t* out, uint32_t n, uint2* spill) {
+    TRAV_STACK_DECL
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        f3 o{rays.ox[i], rays.oy[i], rays.oz[i]}, d{rays.dx[i], rays.dy[i], rays.dz[i]};
+        float tmax = rays.tmax ? rays.tmax[i] : INF_F;
+        out[i] = bvh_occluded(nodes, tris, o, d, tmax, st) ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------------ scene pieces
+// light.rs:40-54
+AGD bool panel_intersect(const DPanel& P, f3 o, f3 d, float tmax, float* tout) {
+    f3 n = ld3(P.n);
+    float denom = dot(d, n);
+    if (fabsf(denom) < 1e-9f) return false;
+    float t = dot(ld3(P.c) - o, n) / denom;
+    if (t <= 1e-6f || t >= tmax) return false;
+    f3 p = (o + d * t) - ld3(P.c);
+    float a = dot(p, ld3(P.hu)) / P.hu2;
+    float b = dot(p, ld3(P.hv)) / P.hv2;
+    if (fabsf(a) <= 1.0f && fabsf(b) <= 1.0f) {
+        *tout = t;
+        return true;
+    }
+    return false;
+}
+// light.rs:57-87
+AGD float panel_profile(const DPanel& P, f3 p) {
+    if (P.feather <= 0.0f) return 1.0f;
+    f3 q = p - ld3(P.c);
+    float a = fminf(fabsf(dot(q, ld3(P.hu)) / P.hu2), 1.0f);
+    float b = fminf(fabsf(dot(q, ld3(P.hv)) / P.hv2), 1.0f);
+    float start = 1.0f - P.feather;
+    float fa = 1.0f, fb = 1.0f;
+    if (a > start) {
+        float t = (a - start) / P.feather;
+        fa = 1.0f - 0.75f * t * t * (3.0f - 2.0f * t);
+    }
+    if (b > start) {
+        float t = (b - start) / P.feather;
+        fb = 1.0f - 0.75f * t * t * (3.0f - 2.0f * t);
+    }
+    return fa * fb;
+}
+AGD Spec panel_le(const DPanel& P, f3 p, f3 dir, const Spec& lam) {
+    if (P.emit && dot(dir, ld3(P.n)) < 0.0f) return tab_sample(P.emit, lam) * panel_profile(P, p);
+    return splat(0.0f);
+}
+// light.rs:133-162
+AGD bool floor_intersect(const DScene& S, f3 o, f3 d, float tmax, float* tout) {
+    if (fabsf(d.y) < 1e-9f) return false;
+    float t = (S.floor_y - o.y) / d.y;
+    if (t > 1e-5f && t < tmax) {
+        *tout = t;
+        return true;
+    }
+    return false;
+}
+AGD float floor_presence(const DScene& S, f3 p) {
+    float r = sqrtf(p.x * p.x + p.z * p.z);
+    if (r <= S.floor_f0) return 1.0f;
+    if (r >= S.floor_f1) return 0.0f;
+    float t = (r - S.floor_f0) / (S.floor_f1 - S.floor_f0);
+    return 1.0f - t * t * (3.0f - 2.0f * t);
+}
+
+// envmap.rs:77-88 LinearRgb::eval
+AGD float linrgb_eval(const float* basis, const float* w, float l) {
+    float x = fminf(fmaxf(l - 360.0f, 0.0f), 469.999f);
+    int i = (int)x;
+    float f = x - (float)i;
+    float v = 0.0f;
+    for (int c = 0; c < 3; c++) {
+        const float* b = basis + c * AG_SPEC_N;
+        v += w[c] * (b[i] * (1.0f - f) + b[i + 1] * f);
+    }
+    return fmaxf(v, 0.0f);
+}
+// envmap.rs:182-204
+AGD void env_uv(const DScene& S, f3 dir, float* u, float* v) {
+    f3 d = rot_y(dir, S.env_rsn, S.env_rcn);
+    float uu = 0.5f + atan2f(d.x, -d.z) / (2.0f * PI_F);
+    *v = acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) / PI_F;
+    float r = fmodf(uu, 1.0f);
+    *u = r < 0.0f ? r + 1.0f : r;
+}
+AGD f3 env_dir(const DScene& S, float u, float v) {
+    float phi = (u - 0.5f) * 2.0f * PI_F;
+    float theta = v * PI_F;
+    float st = sinf(theta), ct = cosf(theta);
+    f3 d{st * sinf(phi), ct, -st * cosf(phi)};
+    return rot_y(d, S.env_rs, S.env_rc);
+}
+AGD float env_map_pdf(const DScene& S, f3 dir) {
+    float u, v;
+    env_uv(S, dir, &u, &v);
+    float st = sinf(PI_F * v);
+    if (st <= 1e-6f) return 0.0f;
+    int x = min((int)(u * (float)S.env_w), S.env_w - 1);
+    int y = min((int)(v * (float)S.env_h), S.env_h - 1);
+    const float* row = S.env_cond + (size_t)y * (S.env_w + 1);
+    float pc = (row[x + 1] - row[x]) * (float)S.env_w;
+    float pm = (S.env_marg[y + 1] - S.env_marg[y]) * (float)S.env_h;
+    return pc * pm / (2.0f * PI_F * PI_F * st);
+}
+// number of entries <= u in a sorted array (partition_point(|c| c <= u))
+AGD int count_le(const float* a, int n, float u) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= u)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+AGD f3 env_map_sample(const DScene& S, float u1, float u2, float* pdf) {
+    int h = S.env_h, w = S.env_w;
+    int y = min(max(count_le(S.env_marg, h + 1, u1), 1), h) - 1;
+    const float* cond = S.env_cond + (size_t)y * (w + 1);
+    int x = min(max(count_le(cond, w + 1, u2), 1), w) - 1;
+    float fy = (u1 - S.env_marg[y]) / fmaxf(S.env_marg[y + 1] - S.env_marg[y], 1e-12f);
+    float fx = (u2 - cond[x]) / fmaxf(cond[x + 1] - cond[x], 1e-12f);
+    float u = ((float)x + fminf(fmaxf(fx, 0.0f), 0.999f)) / (float)w;
+    float v = ((float)y + fminf(fmaxf(fy, 0.0f), 0.999f)) / (float)h;
+    f3 d = env_dir(S, u, v);
+    *pdf = env_map_pdf(S, d);
+    return d;
+}
+// light.rs:110-127 (Env::eval)
+AGD Spec env_eval(const DScene& S, f3 dir, const Spec& lam) {
+    if (S.env_kind == AG_ENV_MAP) {
+        float u, v;
+        env_uv(S, dir, &u, &v);
+        int x = min((int)(u * (float)S.env_w), S.env_w - 1);
+        int y = min((int)(v * (float)S.env_h), S.env_h - 1);
+        const float* w = S.env_weights + ((size_t)y * S.env_w + x) * 3;
+        Spec r;
+        for (int i = 0; i < 4; i++) r.v[i] = linrgb_eval(S.env_basis, w, lam.v[i]);
+        return r;
+    }
+    float y = fminf(fmaxf(dir.y, -1.0f), 1.0f);
+    if (y >= 0.0f) {
+        float t = powf(y, S.env_power);
+        return lerp(t, tab_sample(S.env_hor, lam), tab_sample(S.env_top, lam));
+    }
+    float t = powf(-y, 0.5f);
+    return lerp(t, tab_sample(S.env_hor, lam), tab_sample(S.env_bot, lam));
+}
+
+AGD float light_pick_pdf(const DScene& S, int panel) {
+    for (int k = 0; k < S.n_lights; k++)
+        if (S.lights[k] == panel) return S.light_cdf[k] - (k == 0 ? 0.0f : S.light_cdf[k - 1]);
+    return 0.0f;
+}
+
+AGD const DMat& material_of(const DScene& S, const DMat* vmats, const DView& V, int m) {
+    if (m == 0 && V.mat0 >= 0) return vmats[V.mat0];
+    return S.mat[m];
+}
+
+// material.rs:96-138
+AGD Bsdf material_bsdf(const DMat& M, const Spec& lam, float mask, float u, f3 p_obj, f3 n_world, const Frame& fr,
+                       const DView& V) {
+    Bsdf b;
+    b.alpha = M.alpha;
+    b.eta1 = M.eta;
+    b.w = 0.0f;
+    switch (M.kind) {
+        case AG_MAT_METAL: {
+            if (M.has_patina) {
+                float t = fminf(fmaxf((mask - M.p_lo) / fmaxf(M.p_hi - M.p_lo, 1e-4f), 0.0f), 1.0f);
+                float w = M.p_strength * t * t * (3.0f - 2.0f * t);
+                if (u < w) {
+                    b.kind = B_COATED;
+                    b.alpha = M.p_alpha;
+                    b.eta1 = M.p_eta;
+                    b.a = tab_sample(M.tpatina, lam);
+                    return b;
+                }
+            }
+            b.kind = B_CONDUCTOR;
+            for (int i = 0; i < 4; i++) {
+                b.a.v[i] = grid_eval(M.tn, lam.v[i]);
+                b.b.v[i] = grid_eval(M.tk, lam.v[i]);
+            }
+            return b;
+        }
+        case AG_MAT_GEM:
+            b.kind = B_DIELECTRIC;
+            b.a = tab_sample(M.ta, lam);
+            return b;
+        case AG_MAT_GEM_OPAQUE:
+        case AG_MAT_PLASTIC:
+            b.kind = B_COATED;
+            b.a = tab_sample(M.ta, lam);
+            return b;
+        case AG_MAT_DIFFUSE:
+            b.kind = B_DIFFUSE;
+            b.a = tab_sample(M.ta, lam);
+            return b;
+        case AG_MAT_OPAL: {
+            f3 dir;
+            float spacing, brightness;
+            opal_domain(p_obj / M.opal_cell, &dir, &spacing, &brightness);
+            f3 nw = normalize(n_world * 1.2f + rot_y(dir, V.ts, V.tc));
+            f3 n = to_local(fr, nw);
+            if (n.z < 0.0f) n = -n;
+            b.kind = B_OPAL;
+            b.on = n;
+            b.alpha = 0.22f * 0.22f;
+            b.spacing = spacing;
+            b.strength = M.opal_strength * brightness;
+            b.b = lam;
+            b.a = tab_sample(M.ta, lam);
+            b.w = 0.6f;
+            return b;
+        }
+        default: b.kind = B_BLACK; return b;
+    }
+}
+
+// material.rs:157-177
+AGD void guide_rgb(const DMat& M, float cavity, float out[3]) {
+    const float L[3] = {610.0f, 550.0f, 465.0f};
+    switch (M.kind) {
+        case AG_MAT_METAL:
+            for (int i = 0; i < 3; i++) {
+                float n = grid_eval(M.tn, L[i]), k = grid_eval(M.tk, L[i]);
+                out[i] = ((n - 1.0f) * (n - 1.0f) + k * k) / ((n + 1.0f) * (n + 1.0f) + k * k);
+            }
+            if (M.has_patina) {
+                float t = fminf(fmaxf((cavity - M.p_lo) / fmaxf(M.p_hi - M.p_lo, 1e-4f), 0.0f), 1.0f);
+                float w = M.p_strength * t * t * (3.0f - 2.0f * t);
+                for (int i = 0; i < 3; i++) out[i] = out[i] * (1.0f - w) + tab_eval(M.tpatina, L[i]) * w;
+            }
+            return;
+        case AG_MAT_GEM:
+        case AG_MAT_GEM_OPAQUE: out[0] = out[1] = out[2] = 1.0f; return;
+        case AG_MAT_OPAL:
+        case AG_MAT_PLASTIC:
+        case AG_MAT_DIFFUSE:
+            for (int i = 0; i < 3; i++) out[i] = tab_eval(M.ta, L[i]);
+            return;
+        default: out[0] = out[1] = out[2] = 0.0f; return;
+    }
+}
+
+// ------------------------------------------------------------------ wavefront kernels
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+AGD int find_segment(const DSeg* segs, int nseg, uint32_t path) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].begin <= path)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// render.rs:67-95 (per sample: pixel seed, jitter, lens, wavelengths, camera ray)
+__global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, int nseg, uint32_t npaths,
+                                                const DView* __restrict__ views, DPaths P, DQueues Q) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[0] = npaths;
+    for (uint32_t path = blockIdx.x * blockDim.x + threadIdx.x; path < npaths; path += gridDim.x * blockDim.x) {
+        const DSeg& sg = segs[find_segment(segs, nseg, path)];
+        uint32_t local = path - sg.begin;
+        uint32_t pix = sg.p0 + local % sg.npix;
+        uint32_t si = sg.s0 + local / sg.npix;
+        const DView& V = views[sg.view];
+        uint32_t x = pix % V.width, y = pix / V.width;
+        uint32_t ps = hash32((y * V.width + x) ^ V.seed_h);
+        float jx = tent(rng_dim(ps, si, 0), 1.0f);
+        float jy = tent(rng_dim(ps, si, 1), 1.0f);
+        float lu = rng_dim(ps, si, 2), lv = rng_dim(ps, si, 3);
+        Spec lam, pdf;
+        sample_wavelengths(rng_dim(ps, si, 4), lam, pdf);
+        float sx = (((float)x + 0.5f + jx) * V.aspect_sx) - 1.0f;
+        float sy = 1.0f - (((float)y + 0.5f + jy) * V.aspect_sy);
+        // camera.rs:57-70
+        float cx = (sx + V.shift0) * V.half_w;
+        float cy = (sy + V.shift1) * V.half_h;
+        f3 dir = normalize(ld3(V.fwd) + ld3(V.right) * cx + ld3(V.up) * cy);
+        f3 org = ld3(V.pos);
+        if (V.lens_radius > 0.0f) {
+            float ft = V.focus_dist / dot(dir, ld3(V.fwd));
+            f3 pf = org + dir * ft;
+            float dx, dy;
+            disk(lu, lv, &dx, &dy);
+            org = ld3(V.pos) + ld3(V.right) * (dx * V.lens_radius) + ld3(V.up) * (dy * V.lens_radius);
+            dir = normalize(pf - org);
+        }
+        P.beta[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        P.L[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.lam[path] = to4(lam);
+        P.pdf[path] = to4(pdf);
+        P.prev_pdf[path] = 1.0f;
+        P.flags[path] = 1u | (255u << 8);
+        P.rng[path] = rng_init(ps, si);
+        P.pix_seed[path] = ps;
+        P.si[path] = si;
+        P.view[path] = sg.view;
+        P.galb[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.gnrm[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        Q.ray_o[0][path] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
+        Q.ray_d[0][path] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
+    }
+}
+
+// scene.rs:169-218 (closest hit over the piece, the panels and the floor)
+__global__ __launch_bounds__(TRAV_BLOCK) void k_extend(const DScene* __restrict__ Sp, const DView* __restrict__ views,
+                                                       DPaths P, DQueues Q, int depth, uint2* spill) {
+    TRAV_STACK_DECL
+    const DScene& S = *Sp;
+    uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
+    const uint32_t n = cnt[0];
+    const float4* ro = Q.ray_o[depth & 1];
+    const float4* rd = Q.ray_d[depth & 1];
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        int key = -1;
+        if (i < n) {
+            float4 a = ro[i], b = rd[i];
+            uint32_t path = __float_as_uint(a.w);
+            f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+            const DView& V = views[P.view[path]];
+            f3 oo = o, od = d;
+            if (V.turn_deg != 0.0f) {
+                oo = rot_y(o, V.tsn, V.tcn);
+                od = rot_y(d, V.tsn, V.tcn);
+            }
+            TriHit h = bvh_closest(S.nodes, S.tris, oo, od, INF_F, st);
+            float tmax = INF_F;
+            uint32_t code = HIT_MISS;
+            if (h.prim != 0xFFFFFFFFu) {
+                tmax = h.t;
+                code = h.prim;
+            }
+            for (int k = 0; k < S.n_panels; k++) {
+                if (depth == 0 && !S.panel[k].camera_visible) continue;
+                float t;
+                if (panel_intersect(S.panel[k], o, d, tmax, &t)) {
+                    tmax = t;
+                    code = HIT_PANEL | (uint32_t)k;
+                }
+            }
+            if (S.floor_present) {
+                float t;
+                if (floor_intersect(S, o, d, tmax, &t)) {
+                    tmax = t;
+                    code = HIT_FLOOR;
+                }
+            }
+            Q.hit[i] = make_float4(tmax, h.u, h.v, __uint_as_float(code));
+            if (code == HIT_MISS)
+                key = 0;
+            else if (code == HIT_FLOOR)
+                key = 2 + S.floor_mat;
+            else if (code & HIT_PANEL)
+                key = 1;
+            else
+                key = 2 + (int)S.tmat[code];
+        }
+        uint32_t slot = wave_append_keyed(cnt + 2, key);
+        if (key >= 0) Q.bucket[(size_t)key * Q.cap + slot] = i;
+    }
+}
+
+struct ShadeArgs {
+    const DScene* S;
+    const DView* views;
+    const DMat* vmats;
+    uint32_t max_depth, rr_depth;
+    float clamp_indirect;
+};
+
+// integrator.rs:50-221, one bounce
+__global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q, int depth) {
+    const DScene& S = *A.S;
+    uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
+    uint32_t* cnt_next = Q.counters + (depth + 1) * CNT_STRIDE;
+    __shared__ uint32_t pre[MAX_KEYS + 1];
+    if (threadIdx.x == 0) {
+        pre[0] = 0;
+        for (int k = 0; k < Q.n_keys; k++) pre[k + 1] = pre[k] + cnt[2 + k];
+    }
+    __syncthreads();
+    const uint32_t total = pre[Q.n_keys];
+    const float4* ro = Q.ray_o[depth & 1];
+    const float4* rd = Q.ray_d[depth & 1];
+    float4* no = Q.ray_o[(depth + 1) & 1];
+    float4* nd = Q.ray_d[(depth + 1) & 1];
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        uint32_t j = base + threadIdx.x;
+        bool emit_ray = false, emit_shadow = false;
+        f3 nro, nrd;
+        uint32_t path = 0;
+        float4 sh_o, sh_d, sh_c, sh_x;
+        if (j < total) {
+            int key = 0;
+            while (key + 1 < Q.n_keys && pre[key + 1] <= j) key++;
+            uint32_t r = Q.bucket[(size_t)key * Q.cap + (j - pre[key])];
+            float4 a = ro[r], b = rd[r], hh = Q.hit[r];
+            path = __float_as_uint(a.w);
+            f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+            float t = hh.x;
+            uint32_t code = __float_as_uint(hh.w);
+            Spec beta = from4(P.beta[path]), L = from4(P.L[path]), lam = from4(P.lam[path]), pdf = from4(P.pdf[path]);
+            float prev_pdf = P.prev_pdf[path];
+            uint32_t flags = P.flags[path];
+            uint64_t rng = P.rng[path];
+            const uint32_t ps = P.pix_seed[path], si = P.si[path];
+            const DView& V = A.views[P.view[path]];
+            bool specular = flags & 1u, rough = (flags & 2u) != 0;
+            int medium = (int)((flags >> 8) & 255u);
+            const bool clamp = rough;
+            const float cm = A.clamp_indirect;
+            bool alive = true;
+            if (code != HIT_MISS && medium != 255) {
+                Spec sig = tab_sample(S.mat[medium].tb, lam);
+                for (int i = 0; i < 4; i++) beta.v[i] = beta.v[i] * expf(-sig.v[i] * t);
+            }
+            auto env_weight = [&](f3 dir) -> float {
+                if (S.env_kind == AG_ENV_MAP && !specular) {
+                    float pl = S.env_pick * env_map_pdf(S, dir);
+                    return prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl);
+                }
+                return 1.0f;
+            };
+            f3 p, p_obj, ng, ns;
+            int mat_id = 0;
+            float cavity = 0.0f;
+            if (code == HIT_MISS) {
+                add_clamped(L, beta * env_eval(S, d, lam) * env_weight(d), clamp, cm);
+                alive = false;
+            } else if (code == HIT_FLOOR) {
+                p = o + d * t;
+                if (next_random(rng) >= floor_presence(S, p)) {
+                    add_clamped(L, beta * env_eval(S, d, lam) * env_weight(d), clamp, cm);
+                    alive = false;
+                } else {
+                    f3 n = d.y < 0.0f ? f3{0.0f, 1.0f, 0.0f} : f3{-0.0f, -1.0f, -0.0f};
+                    p_obj = p;
+                    ng = n;
+                    ns = n;
+                    mat_id = S.floor_mat;
+                }
+            } else if (code & HIT_PANEL) {
+                const DPanel& Pn = S.panel[code & 0xFFFFu];
+                Spec le = panel_le(Pn, o + d * t, d, lam);
+                if (!is_black(le)) {
+                    if (specular) {
+                        add_clamped(L, beta * le, clamp, cm);
+                    } else {
+                        float cos_l = fmaxf(fabsf(dot(-d, ld3(Pn.n))), 1e-8f);
+                        float p_light =
+                            (1.0f - S.env_pick) * light_pick_pdf(S, (int)(code & 0xFFFFu)) * t * t / (cos_l * Pn.area);
+                        float w = prev_pdf * prev_pdf / (prev_pdf * prev_pdf + p_light * p_light);
+                        add_clamped(L, beta * le * w, clamp, cm);
+                    }
+                }
+                alive = false;
+            } else {
+                // scene.rs:173-201: attributes in object space, then to world
+                uint32_t prim = code;
+                uint32_t i0 = S.idx[3 * prim], i1 = S.idx[3 * prim + 1], i2 = S.idx[3 * prim + 2];
+                f3 pa = ld3(S.pos + 3 * i0), pb = ld3(S.pos + 3 * i1), pc = ld3(S.pos + 3 * i2);
+                f3 ngo = normalize(cross(pb - pa, pc - pa));
+                float hu = hh.y, hv = hh.z;
+                float w = 1.0f - hu - hv;
+                mat_id = S.tmat[prim];
+                const DMat& Mh = material_of(S, A.vmats, V, mat_id);
+                f3 nso = ngo;
+                if (Mh.kind != AG_MAT_GEM && S.nrm) {
+                    f3 na = ld3(S.nrm + 3 * i0), nb = ld3(S.nrm + 3 * i1), nc = ld3(S.nrm + 3 * i2);
+                    f3 nn = na * w + nb * hu + nc * hv;
+                    nso = length_sq(nn) > 1e-12f ? normalize(nn) : ngo;
+                }
+                if (S.cavity) cavity = S.cavity[i0] * w + S.cavity[i1] * hu + S.cavity[i2] * hv;
+                p_obj = pa * w + pb * hu + pc * hv;
+                if (V.turn_deg != 0.0f) {
+                    p = rot_y(p_obj, V.ts, V.tc);
+                    ng = rot_y(ngo, V.ts, V.tc);
+                    ns = rot_y(nso, V.ts, V.tc);
+                } else {
+                    p = p_obj;
+                    ng = ngo;
+                    ns = nso;
+                }
+            }
+            if (alive) {
+                const DMat& M = material_of(S, A.vmats, V, mat_id);
+                f3 wo_w = -d;
+                bool transmissive = M.kind == AG_MAT_GEM;
+                f3 nsu = ns, ngu = ng;
+                if (!transmissive && dot(ng, wo_w) < 0.0f) {
+                    nsu = -ns;
+                    ngu = -ng;
+                }
+                if (!transmissive && dot(nsu, wo_w) <= 0.0f) nsu = ngu;
+                Frame fr = make_frame(nsu);
+                f3 wo = to_local(fr, wo_w);
+                const uint32_t dim = 8u + 8u * (uint32_t)depth;
+                Bsdf bs = material_bsdf(M, lam, cavity, rng_dim(ps, si, dim), p_obj, nsu, fr, V);
+                if (depth == 0) {
+                    float g[3];
+                    guide_rgb(M, cavity, g);
+                    P.galb[path] = make_float4(g[0], g[1], g[2], 0.0f);
+                    P.gnrm[path] = make_float4(nsu.x, nsu.y, nsu.z, 0.0f);
+                }
+                // next-event estimation (integrator.rs:130-178)
+                if (!bsdf_is_delta(bs)) {
+                    float ul = rng_dim(ps, si, dim + 1), u1 = rng_dim(ps, si, dim + 2), u2 = rng_dim(ps, si, dim + 3);
+                    if (S.env_kind == AG_ENV_MAP && ul < S.env_pick) {
+                        float pdf_dir;
+                        f3 dir = env_map_sample(S, u1, u2, &pdf_dir);
+                        if (pdf_dir > 0.0f && dot(dir, ngu) > 0.0f) {
+                            f3 wi = to_local(fr, dir);
+                            Spec f = bsdf_eval(bs, S.energy, wo, wi);
+                            if (!is_black(f)) {
+                                float p_light = S.env_pick * pdf_dir;
+                                float p_bsdf = bsdf_pdf(bs, wo, wi);
+                                float w = p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf);
+                                f3 org = offset_origin(p, ngu, dir);
+                                Spec le = env_eval(S, dir, lam);
+                                Spec base = beta * f * le;
+                                emit_shadow = true;
+                                sh_o = make_float4(org.x, org.y, org.z, __uint_as_float(path));
+                                sh_d = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0xFFFFFFFFu));
+                                sh_c = to4(base);
+                                sh_x = make_float4(fabsf(wi.z) * w, p_light, rough ? cm : -1.0f, __uint_as_float(1u));
+                            }
+                        }
+                    } else if (S.n_lights > 0) {
+                        float uu = S.env_pick > 0.0f ? (ul - S.env_pick) / (1.0f - S.env_pick) : ul;
+                        int k = 0;
+                        while (k < S.n_lights && S.light_cdf[k] < uu) k++;
+                        k = min(k, S.n_lights - 1);
+                        float pick = S.light_cdf[k] - (k == 0 ? 0.0f : S.light_cdf[k - 1]);
+                        int li = S.lights[k];
+                        pick = pick * (1.0f - S.env_pick);
+                        const DPanel& Pn = S.panel[li];
+                        f3 lp = ld3(Pn.c) + ld3(Pn.hu) * (2.0f * u1 - 1.0f) + ld3(Pn.hv) * (2.0f * u2 - 1.0f);
+                        f3 to = lp - p;
+                        float d2 = length_sq(to);
+                        f3 dir = to / sqrtf(d2);
+                        float cos_l = dot(-dir, ld3(Pn.n));
+                        if (cos_l > 0.0f && dot(dir, ngu) > 0.0f) {
+                            f3 wi = to_local(fr, dir);
+                            Spec f = bsdf_eval(bs, S.energy, wo, wi);
+                            if (!is_black(f)) {
+                                float p_light = pick * d2 / (cos_l * Pn.area);
+                                float p_bsdf = bsdf_pdf(bs, wo, wi);
+                                float w = p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf);
+                                f3 org = offset_origin(p, ngu, dir);
+                                Spec le = panel_le(Pn, lp, dir, lam);
+                                Spec c = beta * f * le * (fabsf(wi.z) * w / p_light);
+                                emit_shadow = true;
+                                sh_o = make_float4(org.x, org.y, org.z, __uint_as_float(path));
+                                sh_d = make_float4(lp.x, lp.y, lp.z, __uint_as_float((uint32_t)li));
+                                sh_c = to4(c);
+                                sh_x = make_float4(0.0f, 0.0f, rough ? cm : -1.0f, __uint_as_float(0u));
+                            }
+                        }
+                    }
+                }
+                // continue the path (integrator.rs:180-218)
+                BSample sm;
+                float u0 = rng_dim(ps, si, dim + 4), u1 = rng_dim(ps, si, dim + 5), u2 = rng_dim(ps, si, dim + 6);
+                bool ok = bsdf_sample(bs, S.energy, wo, u0, u1, u2, &sm);
+                if (ok && (sm.pdf <= 0.0f || is_black(sm.f))) ok = false;
+                f3 wi_w;
+                if (ok) {
+                    wi_w = to_world(fr, sm.wi);
+                    float side = dot(wi_w, ngu) * dot(wo_w, ngu);
+                    if ((!sm.transmit && side <= 0.0f) || (sm.transmit && side >= 0.0f)) ok = false;
+                }
+                if (ok) {
+                    if (sm.term2) terminate_secondary(pdf);
+                    float sc = fabsf(sm.wi.z) / sm.pdf;
+                    for (int i = 0; i < 4; i++) beta.v[i] = beta.v[i] * (sm.f.v[i] * sc);
+                    if (!is_finite(beta)) ok = false;
+                }
+                if (ok) {
+                    specular = sm.delta;
+                    if (!sm.delta) rough = true;
+                    prev_pdf = sm.pdf;
+                    if (sm.transmit) medium = (wo.z > 0.0f && M.kind == AG_MAT_GEM) ? mat_id : 255;
+                    nro = offset_origin(p, ngu, wi_w);
+                    nrd = wi_w;
+                    if ((uint32_t)depth >= A.rr_depth) {
+                        float survive = fminf(max_value(beta), 0.95f);
+                        if (survive <= 0.0f || next_random(rng) >= survive)
+                            ok = false;
+                        else
+                            beta = beta / survive;
+                    }
+                }
+                if (ok && (uint32_t)depth + 1 < A.max_depth) emit_ray = true;
+            }
+            P.beta[path] = to4(beta);
+            P.L[path] = to4(L);
+            P.pdf[path] = to4(pdf);
+            P.prev_pdf[path] = prev_pdf;
+            P.flags[path] = (specular ? 1u : 0u) | (rough ? 2u : 0u) | ((uint32_t)medium << 8);
+            P.rng[path] = rng;
+        }
+        uint32_t slot = wave_append(&cnt_next[0], emit_ray);
+        if (emit_ray) {
+            no[slot] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(path));
+            nd[slot] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
+        }
+        uint32_t sslot = wave_append(&cnt[1], emit_shadow);
+        if (emit_shadow) {
+            Q.sh_o[sslot] = sh_o;
+            Q.sh_d[sslot] = sh_d;
+            Q.sh_c[sslot] = sh_c;
+            Q.sh_x[sslot] = sh_x;
+        }
+    }
+}
+
+// scene.rs:120-140 (sky_visibility) and scene.rs:220-243 (visible)
+__global__ __launch_bounds__(TRAV_BLOCK) void k_shadow(const DScene* __restrict__ Sp, const DView* __restrict__ views,
+                                                       DPaths P, DQueues Q, int depth, uint2* spill) {
+    TRAV_STACK_DECL
+    const DScene& S = *Sp;
+    const uint32_t n = Q.counters[depth * CNT_STRIDE + 1];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 a = Q.sh_o[i], b = Q.sh_d[i], x = Q.sh_x[i];
+        uint32_t path = __float_as_uint(a.w);
+        const DView& V = views[P.view[path]];
+        f3 from{a.x, a.y, a.z};
+        uint32_t kind = __float_as_uint(x.w);
+        Spec c = from4(Q.sh_c[i]);
+        bool add = false;
+        if (kind == 0u) {
+            f3 to{b.x, b.y, b.z};
+            int ignore = (int)__float_as_uint(b.w);
+            f3 dd = to - from;
+            float dist = length(dd);
+            f3 dir = dd / dist;
+            float tmax = dist * (1.0f - 1e-4f);
+            f3 oo = from, od = dir;
+            if (V.turn_deg != 0.0f) {
+                oo = rot_y(from, V.tsn, V.tcn);
+                od = rot_y(dir, V.tsn, V.tcn);
+            }
+            bool vis = !bvh_occluded(S.nodes, S.tris, oo, od, tmax, st);
+            for (int k = 0; vis && k < S.n_panels; k++) {
+                float t;
+                if (k != ignore && panel_intersect(S.panel[k], from, dir, tmax, &t)) vis = false;
+            }
+            if (vis && S.floor_present) {
+                float t;
+                if (floor_intersect(S, from, dir, tmax, &t)) vis = false;
+            }
+            add = vis;
+        } else {
+            f3 dir{b.x, b.y, b.z};
+            f3 oo = from, od = dir;
+            if (V.turn_deg != 0.0f) {
+                oo = rot_y(from, V.tsn, V.tcn);
+                od = rot_y(dir, V.tsn, V.tcn);
+            }
+            float vis = 1.0f;
+            if (bvh_occluded(S.nodes, S.tris, oo, od, INF_F, st)) vis = 0.0f;
+            for (int k = 0; vis > 0.0f && k < S.n_panels; k++) {
+                float t;
+                if (panel_intersect(S.panel[k], from, dir, INF_F, &t)) vis = 0.0f;
+            }
+            if (vis > 0.0f && S.floor_present) {
+                float t;
+                if (floor_intersect(S, from, dir, INF_F, &t)) vis = 1.0f - floor_presence(S, from + dir * t);
+            }
+            if (vis > 0.0f) {
+                c = c * (x.x * vis / x.y);
+                add = true;
+            }
+        }
+        if (add) {
+            Spec L = from4(P.L[path]);
+            add_clamped(L, c, x.z >= 0.0f, x.z);
+            P.L[path] = to4(L);
+        }
+    }
+}
+
+// render.rs:96-111: per pixel, samples in order, into the pass accumulators
+__global__ __launch_bounds__(256) void k_accumulate(const DSeg* __restrict__ segs, int nseg,
+                                                    const DView* __restrict__ views, const float* __restrict__ cmf,
+                                                    DPaths P) {
+    const float s = 1.0f / (4.0f * 106.85692f);
+    for (int g = 0; g < nseg; g++) {
+        const DSeg sg = segs[g];
+        const DView& V = views[sg.view];
+        const size_t plane = (size_t)V.width * V.height * 3;
+        for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < sg.npix; q += gridDim.x * blockDim.x) {
+            uint32_t pix = sg.p0 + q;
+            float* ac = V.acc + (size_t)pix * 3;
+            float* aa = ac + plane;
+            float* an = ac + 2 * plane;
+            float acc[3] = {ac[0], ac[1], ac[2]}, aacc[3] = {aa[0], aa[1], aa[2]}, nacc[3] = {an[0], an[1], an[2]};
+            for (uint32_t k = 0; k < sg.ns; k++) {
+                uint32_t path = sg.begin + k * sg.npix + q;
+                Spec L = from4(P.L[path]), lam = from4(P.lam[path]), pdf = from4(P.pdf[path]);
+                float xyz[3] = {0.0f, 0.0f, 0.0f};
+                for (int i = 0; i < 4; i++) {
+                    float p = pdf.v[i];
+                    if (p == 0.0f || L.v[i] == 0.0f) continue;
+                    float c[3];
+                    xyz_bar(cmf, lam.v[i], c);
+                    float w = L.v[i] / p;
+                    xyz[0] += c[0] * w;
+                    xyz[1] += c[1] * w;
+                    xyz[2] += c[2] * w;
+                }
+                xyz[0] *= s;
+                xyz[1] *= s;
+                xyz[2] *= s;
+                const float* m = V.cam_m;
+                float rgb[3] = {m[0] * xyz[0] + m[1] * xyz[1] + m[2] * xyz[2], m[3] * xyz[0] + m[4] * xyz[1] + m[5] * xyz[2],
+                                m[6] * xyz[0] + m[7] * xyz[1] + m[8] * xyz[2]};
+                if (isfinite(rgb[0]) && isfinite(rgb[1]) && isfinite(rgb[2])) {
+                    acc[0] += rgb[0];
+                    acc[1] += rgb[1];
+                    acc[2] += rgb[2];
+                }
+                float4 ga = P.galb[path], gn = P.gnrm[path];
+                aacc[0] += ga.x; aacc[1] += ga.y; aacc[2] += ga.z;
+                nacc[0] += gn.x; nacc[1] += gn.y; nacc[2] += gn.z;
+            }
+            ac[0] = acc[0]; ac[1] = acc[1]; ac[2] = acc[2];
+            aa[0] = aacc[0]; aa[1] = aacc[1]; aa[2] = aacc[2];
+            an[0] = nacc[0]; an[1] = nacc[1]; an[2] = nacc[2];
+        }
+    }
+}
+
+// film += pass accumulators; accumulators back to zero
+__global__ void k_film_commit(float* __restrict__ film, float* __restrict__ acc, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        film[i] += acc[i];
+        acc[i] = 0.0f;
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(std::max(1u, std::min((n + 255) / 256, 1024u))), dim3(256), 0, s, p, n);
+}
+
+void launch_rays_intersect(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, HitsSoA h,
+                           uint32_t n, uint2* spill) {
+    hipLaunchKernelGGL(k_rays_intersect, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, h, n, spill);
+}
+void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, uint8_t* out,
+                          uint32_t n, uint2* spill) {
+    hipLaunchKernelGGL(k_rays_occluded, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, out, n, spill);
+}
+void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,
+                   DQueues Q) {
+    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, s, segs, nseg, npaths, views, P, Q);
+}
+void launch_extend(hipStream_t s, int grid, const DScene* S, const DView* views, DPaths P, DQueues Q, int depth,
+                   uint2* spill) {
+    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, S, views, P, Q, depth, spill);
+}
+void launch_shade(hipStream_t s, int grid, const DScene* S, const DView* views, const DMat* vmats, uint32_t max_depth,
+                  uint32_t rr_depth, float clamp_indirect, DPaths P, DQueues Q, int depth) {
+    ShadeArgs A{S, views, vmats, max_depth, rr_depth, clamp_indirect};
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
+}
+void launch_shadow(hipStream_t s, int grid, const DScene* S, const DView* views, DPaths P, DQueues Q, int depth,
+                   uint2* spill) {
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(TRAV_BLOCK), 0, s, S, views, P, Q, depth, spill);
+}
+void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
+                       DPaths P) {
+    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, s, segs, nseg, views, cmf, P);
+}
+void launch_film_commit(hipStream_t s, float* film, float* acc, size_t n) {
+    unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_film_commit, dim3(std::max(1u, g)), dim3(256), 0, s, film, acc, n);
+}
+
+}  // namespace ag

@@ -1,0 +1,36 @@
+// Launch wrappers of the device kernels (defined in kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dev_types.hpp"
+
+namespace ag {
+
+struct RaysSoA {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+};
+struct HitsSoA {
+    float *t, *u, *v;
+    uint32_t* prim;
+};
+
+void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n);
+void launch_rays_intersect(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, HitsSoA h,
+                           uint32_t n, uint2* spill);
+void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, uint8_t* out,
+                          uint32_t n, uint2* spill);
+void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,
+                   DQueues Q);
+void launch_extend(hipStream_t s, int grid, const DScene* S, const DView* views, DPaths P, DQueues Q, int depth,
+                   uint2* spill);
+void launch_shade(hipStream_t s, int grid, const DScene* S, const DView* views, const DMat* vmats, uint32_t max_depth,
+                  uint32_t rr_depth, float clamp_indirect, DPaths P, DQueues Q, int depth);
+void launch_shadow(hipStream_t s, int grid, const DScene* S, const DView* views, DPaths P, DQueues Q, int depth,
+                   uint2* spill);
+void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
+                       DPaths P);
+void launch_film_commit(hipStream_t s, float* film, float* acc, size_t n);
+
+}  // namespace ag

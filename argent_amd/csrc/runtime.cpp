@@ -1,0 +1,851 @@
+// Host runtime: contexts, device scenes, films and the wave scheduler behind
+// ag_render_pass / ag_render_views, plus the ray-batch queries.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/argent.h"
+#include "dev_types.hpp"
+#include "host_internal.hpp"
+#include "host_spectra.hpp"
+#include "kernels.hpp"
+#include "tables/spectral_data.h"
+
+using namespace ag;
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) return fail(AG_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t n) {
+        if (n <= bytes && p) return AG_OK;
+        release();
+        if (n == 0) return AG_OK;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            p = nullptr;
+            return fail(AG_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed");
+        }
+        bytes = n;
+        return AG_OK;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+uint32_t hash32_host(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+}  // namespace
+
+struct ag_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+    ag_tuning tuning{};
+    // wave buffers
+    uint32_t cap = 0;
+    int n_keys_alloc = 0;
+    DevBuf paths_buf, rays_buf, hit_buf, bucket_buf, shadow_buf, counter_buf, spill_buf, seg_buf, view_buf, vmat_buf,
+        vtab_buf;
+    DPaths P{};
+    DQueues Q{};
+    int grid_trav = 0, grid_work = 0;
+    size_t spill_threads = 0;
+    // timing
+    bool timing = false;
+    ag_stats stats{};
+    hipEvent_t ev[8] = {};
+};
+
+struct ag_scene {
+    ag_ctx* ctx = nullptr;
+    DScene hs{};  // host copy with device pointers
+    DevBuf d_scene, nodes, tris, pos, nrm, idx, tmat, cavity, tab, env_w, env_m, env_c;
+    std::vector<ag_material> mats;
+    size_t mat_stride = 0;  // floats per material in the table blob
+    size_t mat_base = 0;
+    uint32_t n_nodes = 0, n_leaves = 0, depth = 0;
+    size_t bytes = 0;
+    float lo[3], hi[3];
+    float turntable_deg = 0.0f;
+    int n_keys = 2;
+    float cam_m[9];  // white-balanced XYZ -> linear sRGB (color.rs:48-55)
+};
+
+struct ag_film {
+    ag_ctx* ctx = nullptr;
+    uint32_t w = 0, h = 0;
+    DevBuf sums, acc;
+    uint32_t samples = 0;
+    float cam_m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+};
+
+// ------------------------------------------------------------------ context
+extern "C" int ag_ctx_create(int device, ag_ctx** out) {
+    if (!out) return fail(AG_E_INVALID, "null output");
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(AG_E_INVALID, "no such device " + std::to_string(device));
+    HIPCHK(hipSetDevice(device));
+    auto* c = new ag_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(AG_E_DEVICE, "stream creation failed");
+    }
+    for (auto& e : c->ev) (void)hipEventCreate(&e);
+    c->tuning.paths_per_wave = 1u << 22;
+    c->tuning.block_traverse = TRAV_BLOCK;
+    c->tuning.blocks_per_cu = 4;
+    c->tuning.sort_by_material = 1;
+    *out = c;
+    return AG_OK;
+}
+
+extern "C" void ag_ctx_destroy(ag_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int ag_ctx_synchronize(ag_ctx* c) {
+    if (!c) return fail(AG_E_INVALID, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AG_OK;
+}
+
+extern "C" int ag_ctx_get_tuning(const ag_ctx* c, ag_tuning* t) {
+    if (!c || !t) return fail(AG_E_INVALID, "null argument");
+    *t = c->tuning;
+    return AG_OK;
+}
+
+extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
+    if (!c || !t) return fail(AG_E_INVALID, "null argument");
+    if (t->paths_per_wave < 1024 || t->paths_per_wave > (1u << 26)) return fail(AG_E_INVALID, "paths_per_wave out of range");
+    if (t->block_traverse != TRAV_BLOCK) return fail(AG_E_INVALID, "block_traverse must be 256");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 5) return fail(AG_E_INVALID, "blocks_per_cu must be 1..5");
+    c->tuning = *t;
+    return AG_OK;
+}
+
+extern "C" int ag_ctx_stats(const ag_ctx* c, ag_stats* s) {
+    if (!c || !s) return fail(AG_E_INVALID, "null argument");
+    *s = c->stats;
+    return AG_OK;
+}
+
+extern "C" int ag_ctx_enable_timing(ag_ctx* c, int32_t on) {
+    if (!c) return fail(AG_E_INVALID, "null context");
+    c->timing = on != 0;
+    return AG_OK;
+}
+
+static int ensure_spill(ag_ctx* c) {
+    c->grid_trav = c->n_cu * (int)c->tuning.blocks_per_cu;
+    c->grid_work = c->n_cu * 8;
+    size_t threads = (size_t)c->grid_trav * TRAV_BLOCK;
+    if (threads > c->spill_threads || !c->spill_buf.p) {
+        int rc = c->spill_buf.ensure(threads * STACK_SPILL * sizeof(uint2));
+        if (rc) return rc;
+        c->spill_threads = threads;
+    }
+    return AG_OK;
+}
+
+static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) {
+    int rc = ensure_spill(c);
+    if (rc) return rc;
+    if (cap != c->cap) {
+        size_t per_path = 6 * sizeof(float4) + sizeof(float) + 4 * sizeof(uint32_t) + sizeof(uint64_t);
+        if ((rc = c->paths_buf.ensure((size_t)cap * per_path))) return rc;
+        if ((rc = c->rays_buf.ensure((size_t)cap * 4 * sizeof(float4)))) return rc;
+        if ((rc = c->hit_buf.ensure((size_t)cap * sizeof(float4)))) return rc;
+        if ((rc = c->shadow_buf.ensure((size_t)cap * 4 * sizeof(float4)))) return rc;
+        char* b = c->paths_buf.as<char>();
+        auto take = [&](size_t bytes) {
+            char* p = b;
+            b += bytes;
+            return p;
+        };
+        c->P.beta = (float4*)take((size_t)cap * 16);
+        c->P.L = (float4*)take((size_t)cap * 16);
+        c->P.lam = (float4*)take((size_t)cap * 16);
+        c->P.pdf = (float4*)take((size_t)cap * 16);
+        c->P.galb = (float4*)take((size_t)cap * 16);
+        c->P.gnrm = (float4*)take((size_t)cap * 16);
+        c->P.rng = (uint64_t*)take((size_t)cap * 8);
+        c->P.prev_pdf = (float*)take((size_t)cap * 4);
+        c->P.flags = (uint32_t*)take((size_t)cap * 4);
+        c->P.pix_seed = (uint32_t*)take((size_t)cap * 4);
+        c->P.si = (uint32_t*)take((size_t)cap * 4);
+        c->P.view = (uint32_t*)take((size_t)cap * 4);
+        float4* r = c->rays_buf.as<float4>();
+        c->Q.ray_o[0] = r;
+        c->Q.ray_d[0] = r + cap;
+        c->Q.ray_o[1] = r + 2 * (size_t)cap;
+        c->Q.ray_d[1] = r + 3 * (size_t)cap;
+        c->Q.hit = c->hit_buf.as<float4>();
+        float4* sh = c->shadow_buf.as<float4>();
+        c->Q.sh_o = sh;
+        c->Q.sh_d = sh + cap;
+        c->Q.sh_c = sh + 2 * (size_t)cap;
+        c->Q.sh_x = sh + 3 * (size_t)cap;
+        c->cap = cap;
+        c->n_keys_alloc = 0;
+    }
+    if (n_keys > c->n_keys_alloc) {
+        if ((rc = c->bucket_buf.ensure((size_t)cap * n_keys * sizeof(uint32_t)))) return rc;
+        c->n_keys_alloc = n_keys;
+    }
+    if ((rc = c->counter_buf.ensure((size_t)(max_depth + 2) * CNT_STRIDE * sizeof(uint32_t)))) return rc;
+    c->Q.bucket = c->bucket_buf.as<uint32_t>();
+    c->Q.counters = c->counter_buf.as<uint32_t>();
+    c->Q.cap = cap;
+    c->Q.n_keys = n_keys;
+    if ((rc = c->seg_buf.ensure(MAX_SEGMENTS * sizeof(DSeg)))) return rc;
+    return AG_OK;
+}
+
+// ------------------------------------------------------------------ scene
+static size_t tables_per_material() { return 2 * AG_GRID_N + 3 * AG_SPEC_N; }
+
+static void fill_dmat(DMat& m, const ag_material& a, const float* dev_base) {
+    m.kind = a.kind;
+    m.alpha = a.alpha;
+    m.eta = a.eta;
+    m.tn = dev_base;
+    m.tk = dev_base + AG_GRID_N;
+    m.has_patina = a.has_patina;
+    m.p_strength = a.patina_strength;
+    m.p_lo = a.patina_lo;
+    m.p_hi = a.patina_hi;
+    m.p_alpha = a.patina_alpha;
+    m.p_eta = a.patina_eta;
+    m.tpatina = dev_base + 2 * AG_GRID_N;
+    m.ta = dev_base + 2 * AG_GRID_N + AG_SPEC_N;
+    m.tb = dev_base + 2 * AG_GRID_N + 2 * AG_SPEC_N;
+    m.opal_cell = a.opal_cell_mm;
+    m.opal_strength = a.opal_strength;
+}
+
+static void pack_material(const ag_material& a, float* dst) {
+    std::memcpy(dst, a.metal_n, sizeof(a.metal_n));
+    std::memcpy(dst + AG_GRID_N, a.metal_k, sizeof(a.metal_k));
+    std::memcpy(dst + 2 * AG_GRID_N, a.patina_base, sizeof(a.patina_base));
+    std::memcpy(dst + 2 * AG_GRID_N + AG_SPEC_N, a.spec_a, sizeof(a.spec_a));
+    std::memcpy(dst + 2 * AG_GRID_N + 2 * AG_SPEC_N, a.spec_b, sizeof(a.spec_b));
+}
+
+static int validate_desc(const ag_scene_desc* d) {
+    if (!d) return fail(AG_E_INVALID, "null descriptor");
+    if (d->n_triangles && (!d->positions || !d->triangles)) return fail(AG_E_INVALID, "missing mesh arrays");
+    if (d->n_materials < 1 || d->n_materials > AG_MAX_MATERIALS) return fail(AG_E_INVALID, "n_materials must be 1..32");
+    if (d->n_panels < 0 || d->n_panels > AG_MAX_PANELS) return fail(AG_E_INVALID, "n_panels must be 0..16");
+    for (uint32_t i = 0; i < 3 * d->n_triangles; i++)
+        if (d->triangles[i] >= d->n_vertices) return fail(AG_E_INVALID, "triangle index out of range");
+    if (d->n_triangles >= (1u << 27)) return fail(AG_E_INVALID, "too many triangles (max 2^27)");
+    if (d->tri_material)
+        for (uint32_t i = 0; i < d->n_triangles; i++)
+            if ((int)d->tri_material[i] >= d->n_materials) return fail(AG_E_INVALID, "triangle material out of range");
+    if (d->floor.present && (d->floor.material < 0 || d->floor.material >= d->n_materials))
+        return fail(AG_E_INVALID, "floor material out of range");
+    for (int i = 0; i < d->n_materials; i++)
+        if (d->materials[i].kind < AG_MAT_METAL || d->materials[i].kind > AG_MAT_OPAL)
+            return fail(AG_E_INVALID, "unknown material kind");
+    if (d->env.kind == AG_ENV_MAP && (d->env.width <= 0 || d->env.height <= 0 || !d->env.weights || !d->env.marginal ||
+                                      !d->env.conditional))
+        return fail(AG_E_INVALID, "incomplete environment map");
+    return AG_OK;
+}
+
+static int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
+    int rc = b.ensure(bytes);
+    if (rc) return rc;
+    if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+    return AG_OK;
+}
+
+extern "C" int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* d, ag_scene** out) {
+    if (!ctx || !out) return fail(AG_E_INVALID, "null argument");
+    int rc = validate_desc(d);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    std::unique_ptr<ag_scene> s(new ag_scene());
+    s->ctx = ctx;
+    hipStream_t st = ctx->stream;
+    WideBvh bvh;
+    build_wide_bvh(d->positions, d->triangles, d->n_triangles, &bvh);
+    if (bvh.depth * 3 + 1 > (uint32_t)(STACK_LDS + STACK_SPILL))
+        return fail(AG_E_INVALID, "BVH too deep for the traversal stack (depth " + std::to_string(bvh.depth) + ")");
+    s->n_nodes = bvh.n_nodes;
+    s->n_leaves = bvh.n_leaves;
+    s->depth = bvh.depth;
+    for (int a = 0; a < 3; a++) {
+        s->lo[a] = bvh.lo[a];
+        s->hi[a] = bvh.hi[a];
+    }
+    if ((rc = upload(s->nodes, bvh.nodes.data(), bvh.nodes.size() * 4, st))) return rc;
+    std::vector<float> dummy(12, 0.0f);
+    if ((rc = upload(s->tris, bvh.tris.empty() ? dummy.data() : bvh.tris.data(),
+                     std::max<size_t>(bvh.tris.size(), 12) * 4, st)))
+        return rc;
+    size_t nv = std::max<uint32_t>(d->n_vertices, 1), nt = std::max<uint32_t>(d->n_triangles, 1);
+    std::vector<float> zeros3(3 * nv, 0.0f);
+    std::vector<uint32_t> zidx(3 * nt, 0);
+    if ((rc = upload(s->pos, d->n_vertices ? d->positions : zeros3.data(), 3 * nv * 4, st))) return rc;
+    if ((rc = upload(s->idx, d->n_triangles ? d->triangles : zidx.data(), 3 * nt * 4, st))) return rc;
+    std::vector<uint16_t> tm(nt, 0);
+    if (d->tri_material && d->n_triangles) std::memcpy(tm.data(), d->tri_material, d->n_triangles * 2);
+    if ((rc = upload(s->tmat, tm.data(), nt * 2, st))) return rc;
+    if (d->normals && d->n_vertices)
+        if ((rc = upload(s->nrm, d->normals, 3 * nv * 4, st))) return rc;
+    if (d->cavity && d->n_vertices)
+        if ((rc = upload(s->cavity, d->cavity, nv * 4, st))) return rc;
+    // table blob: materials, panels, env spectra + basis, energy, CMF
+    s->mat_stride = tables_per_material();
+    size_t n_tab = AG_MAX_MATERIALS * s->mat_stride + AG_MAX_PANELS * AG_SPEC_N + 6 * AG_SPEC_N + 1024 + 3 * AG_SPEC_N;
+    std::vector<float> tab(n_tab, 0.0f);
+    if ((rc = s->tab.ensure(n_tab * 4))) return rc;
+    const float* dbase = s->tab.as<float>();
+    DScene& S = s->hs;
+    std::memset(&S, 0, sizeof(S));
+    s->mats.assign(d->materials, d->materials + d->n_materials);
+    for (int m = 0; m < d->n_materials; m++) {
+        pack_material(d->materials[m], &tab[m * s->mat_stride]);
+        fill_dmat(S.mat[m], d->materials[m], dbase + m * s->mat_stride);
+    }
+    size_t off = AG_MAX_MATERIALS * s->mat_stride;
+    S.n_panels = d->n_panels;
+    for (int i = 0; i < d->n_panels; i++) {
+        const ag_panel& p = d->panels[i];
+        DPanel& q = S.panel[i];
+        for (int a = 0; a < 3; a++) {
+            q.c[a] = p.center[a];
+            q.hu[a] = p.hu[a];
+            q.hv[a] = p.hv[a];
+        }
+        V3 hu(p.hu[0], p.hu[1], p.hu[2]), hv(p.hv[0], p.hv[1], p.hv[2]);
+        V3 c = cross(hu, hv);
+        V3 n = normalize(c);
+        q.n[0] = n.x; q.n[1] = n.y; q.n[2] = n.z;
+        q.area = 4.0f * length(c);
+        q.hu2 = length_sq(hu);
+        q.hv2 = length_sq(hv);
+        q.camera_visible = p.camera_visible;
+        q.feather = p.feather;
+        if (p.has_emission) {
+            std::memcpy(&tab[off + i * AG_SPEC_N], p.emission, AG_SPEC_N * 4);
+            q.emit = dbase + off + i * AG_SPEC_N;
+        } else {
+            q.emit = nullptr;
+        }
+    }
+    off += AG_MAX_PANELS * AG_SPEC_N;
+    std::memcpy(&tab[off], d->env.top, AG_SPEC_N * 4);
+    std::memcpy(&tab[off + AG_SPEC_N], d->env.horizon, AG_SPEC_N * 4);
+    std::memcpy(&tab[off + 2 * AG_SPEC_N], d->env.bottom, AG_SPEC_N * 4);
+    std::memcpy(&tab[off + 3 * AG_SPEC_N], d->env.basis, 3 * AG_SPEC_N * 4);
+    S.env_top = dbase + off;
+    S.env_hor = dbase + off + AG_SPEC_N;
+    S.env_bot = dbase + off + 2 * AG_SPEC_N;
+    S.env_basis = dbase + off + 3 * AG_SPEC_N;
+    off += 6 * AG_SPEC_N;
+    std::memcpy(&tab[off], d->energy, 1024 * 4);
+    S.energy = dbase + off;
+    off += 1024;
+    std::memcpy(&tab[off], ag_data::CMF_1931_2DEG, 3 * AG_SPEC_N * 4);
+    S.cmf = dbase + off;
+    HIPCHK(hipMemcpyAsync(s->tab.p, tab.data(), n_tab * 4, hipMemcpyHostToDevice, st));
+    S.env_kind = d->env.kind;
+    S.env_power = d->env.power;
+    if (d->env.kind == AG_ENV_MAP) {
+        size_t w = (size_t)d->env.width, h = (size_t)d->env.height;
+        if ((rc = upload(s->env_w, d->env.weights, w * h * 3 * 4, st))) return rc;
+        if ((rc = upload(s->env_m, d->env.marginal, (h + 1) * 4, st))) return rc;
+        if ((rc = upload(s->env_c, d->env.conditional, h * (w + 1) * 4, st))) return rc;
+        S.env_w = (int)w;
+        S.env_h = (int)h;
+        S.env_weights = s->env_w.as<float>();
+        S.env_marg = s->env_m.as<float>();
+        S.env_cond = s->env_c.as<float>();
+        const float k = 3.14159265358979323846f / 180.0f;
+        float r = d->env.rotation_deg * k, rn = (-d->env.rotation_deg) * k;
+        S.env_rs = std::sin(r);
+        S.env_rc = std::cos(r);
+        S.env_rsn = std::sin(rn);
+        S.env_rcn = std::cos(rn);
+    }
+    int nl, lights[AG_MAX_PANELS];
+    float cdf[AG_MAX_PANELS], env_pick;
+    light_cdf(d, &nl, lights, cdf, &env_pick);
+    S.n_lights = nl;
+    for (int i = 0; i < nl; i++) {
+        S.lights[i] = lights[i];
+        S.light_cdf[i] = cdf[i];
+    }
+    S.env_pick = env_pick;
+    S.floor_present = d->floor.present;
+    S.floor_y = d->floor.y;
+    S.floor_mat = d->floor.material;
+    S.floor_f0 = d->floor.fade_start;
+    S.floor_f1 = d->floor.fade_end;
+    S.nodes = s->nodes.as<float4>();
+    S.tris = s->tris.as<float4>();
+    S.pos = s->pos.as<float>();
+    S.nrm = (d->normals && d->n_vertices) ? s->nrm.as<float>() : nullptr;
+    S.idx = s->idx.as<uint32_t>();
+    S.tmat = s->tmat.as<uint16_t>();
+    S.cavity = (d->cavity && d->n_vertices) ? s->cavity.as<float>() : nullptr;
+    S.n_tris = d->n_triangles;
+    S.n_mat = d->n_materials;
+    if ((rc = upload(s->d_scene, &S, sizeof(DScene), st))) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    s->turntable_deg = d->turntable_deg;
+    s->n_keys = 2 + d->n_materials;
+    camera_matrix(d->has_white != 0, d->white_xyz, s->cam_m);
+    s->bytes = s->nodes.bytes + s->tris.bytes + s->pos.bytes + s->idx.bytes + s->tmat.bytes + s->nrm.bytes +
+               s->cavity.bytes + s->tab.bytes + s->env_w.bytes + s->env_m.bytes + s->env_c.bytes;
+    *out = s.release();
+    return AG_OK;
+}
+
+extern "C" void ag_scene_destroy(ag_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+}
+
+extern "C" int ag_scene_set_turntable(ag_scene* s, float deg) {
+    if (!s) return fail(AG_E_INVALID, "null scene");
+    s->turntable_deg = deg;
+    return AG_OK;
+}
+
+extern "C" int ag_scene_set_material(ag_scene* s, int32_t m, const ag_material* mat) {
+    if (!s || !mat) return fail(AG_E_INVALID, "null argument");
+    if (m < 0 || m >= s->hs.n_mat) return fail(AG_E_INVALID, "material slot out of range");
+    if (mat->kind < AG_MAT_METAL || mat->kind > AG_MAT_OPAL) return fail(AG_E_INVALID, "unknown material kind");
+    std::vector<float> t(s->mat_stride);
+    pack_material(*mat, t.data());
+    float* dst = s->tab.as<float>() + m * s->mat_stride;
+    hipStream_t st = s->ctx->stream;
+    HIPCHK(hipMemcpyAsync(dst, t.data(), t.size() * 4, hipMemcpyHostToDevice, st));
+    fill_dmat(s->hs.mat[m], *mat, dst);
+    HIPCHK(hipMemcpyAsync(s->d_scene.p, &s->hs, sizeof(DScene), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    s->mats[m] = *mat;
+    return AG_OK;
+}
+
+extern "C" int ag_scene_bounds(const ag_scene* s, float* lo, float* hi) {
+    if (!s) return fail(AG_E_INVALID, "null scene");
+    for (int a = 0; a < 3; a++) {
+        lo[a] = s->lo[a];
+        hi[a] = s->hi[a];
+    }
+    return AG_OK;
+}
+
+extern "C" int ag_scene_stats(const ag_scene* s, uint32_t* nodes, uint32_t* leaves, uint32_t* depth, size_t* bytes) {
+    if (!s) return fail(AG_E_INVALID, "null scene");
+    if (nodes) *nodes = s->n_nodes;
+    if (leaves) *leaves = s->n_leaves;
+    if (depth) *depth = s->depth;
+    if (bytes) *bytes = s->bytes;
+    return AG_OK;
+}
+
+// ------------------------------------------------------------------ film
+extern "C" int ag_film_create(ag_ctx* ctx, uint32_t w, uint32_t h, ag_film** out) {
+    if (!ctx || !out || w == 0 || h == 0) return fail(AG_E_INVALID, "bad film arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::unique_ptr<ag_film> f(new ag_film());
+    f->ctx = ctx;
+    f->w = w;
+    f->h = h;
+    size_t n = (size_t)w * h * 9 * sizeof(float);
+    int rc;
+    if ((rc = f->sums.ensure(n))) return rc;
+    if ((rc = f->acc.ensure(n))) return rc;
+    HIPCHK(hipMemsetAsync(f->sums.p, 0, n, ctx->stream));
+    HIPCHK(hipMemsetAsync(f->acc.p, 0, n, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *out = f.release();
+    return AG_OK;
+}
+
+extern "C" void ag_film_destroy(ag_film* f) {
+    if (!f) return;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    delete f;
+}
+
+extern "C" int ag_film_clear(ag_film* f) {
+    if (!f) return fail(AG_E_INVALID, "null film");
+    size_t n = (size_t)f->w * f->h * 9 * sizeof(float);
+    HIPCHK(hipMemsetAsync(f->sums.p, 0, n, f->ctx->stream));
+    HIPCHK(hipMemsetAsync(f->acc.p, 0, n, f->ctx->stream));
+    HIPCHK(hipStreamSynchronize(f->ctx->stream));
+    f->samples = 0;
+    return AG_OK;
+}
+
+extern "C" uint32_t ag_film_samples(const ag_film* f) { return f ? f->samples : 0; }
+
+extern "C" int ag_film_read_sums(const ag_film* f, int32_t which, float* out) {
+    if (!f || !out) return fail(AG_E_INVALID, "null argument");
+    if (which < AG_FILM_COLOR || which > AG_FILM_NORMAL) return fail(AG_E_INVALID, "bad plane");
+    size_t plane = (size_t)f->w * f->h * 3;
+    HIPCHK(hipMemcpyAsync(out, f->sums.as<float>() + which * plane, plane * 4, hipMemcpyDeviceToHost, f->ctx->stream));
+    HIPCHK(hipStreamSynchronize(f->ctx->stream));
+    return AG_OK;
+}
+
+// render.rs:35-64
+extern "C" int ag_film_read(const ag_film* f, int32_t which, float* out) {
+    if (!f || !out) return fail(AG_E_INVALID, "null argument");
+    int src = which == AG_FILM_XYZ ? AG_FILM_COLOR : which;
+    int rc = ag_film_read_sums(f, src, out);
+    if (rc) return rc;
+    size_t n = (size_t)f->w * f->h;
+    float k = 1.0f / (float)std::max(f->samples, 1u);
+    if (which == AG_FILM_NORMAL) {
+        for (size_t i = 0; i < n; i++) {
+            float v[3] = {out[3 * i] * k, out[3 * i + 1] * k, out[3 * i + 2] * k};
+            float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            for (int c = 0; c < 3; c++) out[3 * i + c] = l > 1e-6f ? v[c] / l : 0.0f;
+        }
+        return AG_OK;
+    }
+    for (size_t i = 0; i < 3 * n; i++) out[i] *= k;
+    if (which == AG_FILM_XYZ) {
+        // invert the camera matrix used for the last pass
+        double m[9], inv[9];
+        for (int i = 0; i < 9; i++) m[i] = f->cam_m[i];
+        double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+        inv[0] = (m[4] * m[8] - m[5] * m[7]) / det;
+        inv[1] = (m[2] * m[7] - m[1] * m[8]) / det;
+        inv[2] = (m[1] * m[5] - m[2] * m[4]) / det;
+        inv[3] = (m[5] * m[6] - m[3] * m[8]) / det;
+        inv[4] = (m[0] * m[8] - m[2] * m[6]) / det;
+        inv[5] = (m[2] * m[3] - m[0] * m[5]) / det;
+        inv[6] = (m[3] * m[7] - m[4] * m[6]) / det;
+        inv[7] = (m[1] * m[6] - m[0] * m[7]) / det;
+        inv[8] = (m[0] * m[4] - m[1] * m[3]) / det;
+        for (size_t i = 0; i < n; i++) {
+            double r = out[3 * i], g = out[3 * i + 1], b = out[3 * i + 2];
+            for (int c = 0; c < 3; c++) out[3 * i + c] = (float)(inv[3 * c] * r + inv[3 * c + 1] * g + inv[3 * c + 2] * b);
+        }
+    }
+    return AG_OK;
+}
+
+// ------------------------------------------------------------------ rendering
+namespace {
+
+struct ViewJob {
+    DView dv;
+    ag_film* film;
+    uint32_t count;
+    uint32_t first;
+};
+
+void make_dview(DView& v, const ag_camera* cam, float turn_deg, int mat0, const ag_film* film, uint32_t seed,
+                const float* cam_m_in) {
+    std::memset(&v, 0, sizeof(v));
+    for (int a = 0; a < 3; a++) {
+        v.pos[a] = cam->pos[a];
+        v.fwd[a] = cam->forward[a];
+        v.right[a] = cam->right[a];
+        v.up[a] = cam->up[a];
+    }
+    v.half_w = cam->half_w;
+    v.half_h = cam->half_h;
+    v.lens_radius = cam->lens_radius;
+    v.focus_dist = cam->focus_dist;
+    v.shift0 = cam->shift[0];
+    v.shift1 = cam->shift[1];
+    v.turn_deg = turn_deg;
+    const float k = 3.14159265358979323846f / 180.0f;
+    float r = turn_deg * k, rn = (-turn_deg) * k;
+    v.ts = std::sin(r);
+    v.tc = std::cos(r);
+    v.tsn = std::sin(rn);
+    v.tcn = std::cos(rn);
+    v.mat0 = mat0;
+    v.width = film->w;
+    v.height = film->h;
+    v.seed_h = hash32_host(seed ^ 0xa511e9b3u);
+    v.aspect_sx = 2.0f / (float)film->w;
+    v.aspect_sy = 2.0f / (float)film->h;
+    for (int i = 0; i < 9; i++) v.cam_m[i] = cam_m_in[i];
+    v.acc = film->acc.as<float>();
+}
+
+}  // namespace
+
+static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, const ag_path_settings* ps,
+                     const DMat* vmats_dev) {
+    if (ps->max_depth > 64) return fail(AG_E_INVALID, "max_depth above 64");
+    HIPCHK(hipSetDevice(c->device));
+    uint32_t cap = c->tuning.paths_per_wave;
+    int rc = ensure_wave(c, cap, s->n_keys, ps->max_depth);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    // upload views
+    std::vector<DView> dv(jobs.size());
+    for (size_t i = 0; i < jobs.size(); i++) dv[i] = jobs[i].dv;
+    if ((rc = c->view_buf.ensure(dv.size() * sizeof(DView)))) return rc;
+    HIPCHK(hipMemcpyAsync(c->view_buf.p, dv.data(), dv.size() * sizeof(DView), hipMemcpyHostToDevice, st));
+    const DView* views = c->view_buf.as<DView>();
+    // segments, view-major, sample-chunk-major inside a view
+    std::vector<DSeg> all;
+    for (size_t v = 0; v < jobs.size(); v++) {
+        uint32_t npix = jobs[v].film->w * jobs[v].film->h;
+        uint32_t s0 = jobs[v].first, s1 = jobs[v].first + jobs[v].count;
+        if (npix <= cap) {
+            uint32_t spw = std::max(1u, cap / npix);
+            for (uint32_t a = s0; a < s1; a += spw) all.push_back(DSeg{0, (uint32_t)v, 0, npix, a, std::min(spw, s1 - a)});
+        } else {
+            for (uint32_t a = s0; a < s1; a++)
+                for (uint32_t p = 0; p < npix; p += cap) all.push_back(DSeg{0, (uint32_t)v, p, std::min(cap, npix - p), a, 1});
+        }
+    }
+    // pack segments into waves, splitting sample ranges to fill
+    std::vector<std::vector<DSeg>> waves;
+    std::vector<DSeg> cur;
+    uint32_t used = 0;
+    for (size_t i = 0; i < all.size(); i++) {
+        DSeg sg = all[i];
+        while (sg.ns > 0) {
+            uint32_t room = cap - used;
+            uint32_t fit = room / sg.npix;
+            if (fit == 0 || cur.size() >= (size_t)MAX_SEGMENTS) {
+                waves.push_back(cur);
+                cur.clear();
+                used = 0;
+                continue;
+            }
+            uint32_t take = std::min(fit, sg.ns);
+            DSeg part = sg;
+            part.begin = used;
+            part.ns = take;
+            cur.push_back(part);
+            used += take * sg.npix;
+            sg.s0 += take;
+            sg.ns -= take;
+        }
+    }
+    if (!cur.empty()) waves.push_back(cur);
+    c->stats = ag_stats{};
+    auto t0 = std::chrono::steady_clock::now();
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[0], st));
+    const DScene* S = s->d_scene.as<DScene>();
+    for (auto& w : waves) {
+        uint32_t npaths = 0;
+        for (auto& sg : w) npaths = std::max(npaths, sg.begin + sg.npix * sg.ns);
+        HIPCHK(hipMemcpyAsync(c->seg_buf.p, w.data(), w.size() * sizeof(DSeg), hipMemcpyHostToDevice, st));
+        launch_zero_u32(st, c->Q.counters, (ps->max_depth + 2) * CNT_STRIDE);
+        launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), npaths, views, c->P, c->Q);
+        for (uint32_t d = 0; d < ps->max_depth; d++) {
+            launch_extend(st, c->grid_trav, S, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>());
+            launch_shade(st, c->grid_work, S, views, vmats_dev, ps->max_depth, ps->rr_depth, ps->clamp_indirect, c->P,
+                         c->Q, (int)d);
+            launch_shadow(st, c->grid_trav, S, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>());
+        }
+        launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P);
+        c->stats.samples += npaths;
+        if (c->timing) {
+            // ray counts of this wave
+            std::vector<uint32_t> cnt((ps->max_depth + 2) * CNT_STRIDE);
+            HIPCHK(hipMemcpyAsync(cnt.data(), c->Q.counters, cnt.size() * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (uint32_t d = 0; d < ps->max_depth; d++) {
+                c->stats.extension_rays += cnt[d * CNT_STRIDE];
+                c->stats.shadow_rays += cnt[d * CNT_STRIDE + 1];
+            }
+        }
+    }
+    for (auto& j : jobs) {
+        size_t n = (size_t)j.film->w * j.film->h * 9;
+        launch_film_commit(st, j.film->sums.as<float>(), j.film->acc.as<float>(), n);
+    }
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[1], st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipGetLastError());
+    auto t1 = std::chrono::steady_clock::now();
+    c->stats.ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    c->stats.extend_launches = (uint32_t)(waves.size() * ps->max_depth);
+    c->stats.shadow_launches = c->stats.extend_launches;
+    for (auto& j : jobs) {
+        j.film->samples += j.count;
+        std::memcpy(j.film->cam_m, j.dv.cam_m, sizeof(j.dv.cam_m));
+    }
+    return AG_OK;
+}
+
+extern "C" int ag_render_pass(ag_ctx* ctx, const ag_scene* scene, const ag_camera* cam, ag_film* film,
+                              const ag_path_settings* ps, uint32_t count, uint32_t seed) {
+    if (!ctx || !scene || !cam || !film || !ps) return fail(AG_E_INVALID, "null argument");
+    if (film->ctx != ctx || scene->ctx != ctx) return fail(AG_E_INVALID, "objects belong to another context");
+    if (count == 0) return AG_OK;
+    if ((uint64_t)film->samples + count > 0xFFFFFFFFull) return fail(AG_E_INVALID, "sample index overflow");
+    ViewJob j;
+    j.film = film;
+    j.count = count;
+    j.first = film->samples;
+    make_dview(j.dv, cam, scene->turntable_deg, -1, film, seed, scene->cam_m);
+    std::vector<ViewJob> jobs{j};
+    return run_views(ctx, scene, jobs, ps, nullptr);
+}
+
+extern "C" int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_material* materials, int32_t n_materials,
+                               const ag_view* views, int32_t n_views, const ag_path_settings* ps) {
+    if (!ctx || !scene || !views || !ps || n_views <= 0) return fail(AG_E_INVALID, "bad arguments");
+    if (n_materials < 0 || (n_materials > 0 && !materials)) return fail(AG_E_INVALID, "bad materials");
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc;
+    // view materials: tables in vtab_buf, descriptors in vmat_buf
+    size_t stride = tables_per_material();
+    std::vector<float> tab(std::max<size_t>(1, n_materials) * stride, 0.0f);
+    if ((rc = ctx->vtab_buf.ensure(tab.size() * 4))) return rc;
+    std::vector<DMat> dm(std::max(1, n_materials));
+    for (int i = 0; i < n_materials; i++) {
+        if (materials[i].kind < AG_MAT_METAL || materials[i].kind > AG_MAT_OPAL) return fail(AG_E_INVALID, "unknown material kind");
+        if (materials[i].kind == AG_MAT_GEM) return fail(AG_E_INVALID, "slot-0 override cannot be a transparent gem");
+        pack_material(materials[i], &tab[i * stride]);
+        fill_dmat(dm[i], materials[i], ctx->vtab_buf.as<float>() + i * stride);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->vtab_buf.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = ctx->vmat_buf.ensure(dm.size() * sizeof(DMat)))) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->vmat_buf.p, dm.data(), dm.size() * sizeof(DMat), hipMemcpyHostToDevice, ctx->stream));
+    std::vector<ViewJob> jobs;
+    for (int i = 0; i < n_views; i++) {
+        const ag_view& v = views[i];
+        if (!v.film || v.film->ctx != ctx) return fail(AG_E_INVALID, "view film missing or from another context");
+        if (v.material0 >= n_materials) return fail(AG_E_INVALID, "view material index out of range");
+        for (int k = 0; k < i; k++)
+            if (views[k].film == v.film) return fail(AG_E_INVALID, "two views share one film");
+        if (v.count == 0) continue;
+        ViewJob j;
+        j.film = v.film;
+        j.count = v.count;
+        j.first = v.film->samples;
+        make_dview(j.dv, &v.camera, v.turntable_deg, v.material0, v.film, v.seed, scene->cam_m);
+        jobs.push_back(j);
+    }
+    if (jobs.empty()) return AG_OK;
+    return run_views(ctx, scene, jobs, ps, ctx->vmat_buf.as<DMat>());
+}
+
+// ------------------------------------------------------------------ ray batches
+static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, ag_hits_soa* hits,
+                     uint8_t* occ, bool device_ptrs) {
+    if (!ctx || !s || !rays) return fail(AG_E_INVALID, "null argument");
+    if (n == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_spill(ctx);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    RaysSoA r{};
+    DevBuf in, outb;
+    if (device_ptrs) {
+        r = RaysSoA{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+    } else {
+        size_t f = (size_t)n * 4;
+        if ((rc = in.ensure(f * 7))) return rc;
+        float* b = in.as<float>();
+        const float* src[7] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+        for (int k = 0; k < 6; k++) {
+            if (!src[k]) return fail(AG_E_INVALID, "missing ray component");
+            HIPCHK(hipMemcpyAsync(b + (size_t)k * n, src[k], f, hipMemcpyHostToDevice, st));
+        }
+        if (rays->tmax) HIPCHK(hipMemcpyAsync(b + 6 * (size_t)n, rays->tmax, f, hipMemcpyHostToDevice, st));
+        r = RaysSoA{b, b + n, b + 2 * (size_t)n, b + 3 * (size_t)n, b + 4 * (size_t)n, b + 5 * (size_t)n,
+                    rays->tmax ? b + 6 * (size_t)n : nullptr};
+    }
+    const float4* nodes = s->nodes.as<float4>();
+    const float4* tris = s->tris.as<float4>();
+    int grid = ctx->grid_trav;
+    if (hits) {
+        HitsSoA h{};
+        if (device_ptrs) {
+            h = HitsSoA{hits->t, hits->u, hits->v, hits->prim};
+        } else {
+            if ((rc = outb.ensure((size_t)n * 16))) return rc;
+            float* b = outb.as<float>();
+            h = HitsSoA{b, b + n, b + 2 * (size_t)n, reinterpret_cast<uint32_t*>(b + 3 * (size_t)n)};
+        }
+        launch_rays_intersect(st, grid, nodes, tris, r, h, n, ctx->spill_buf.as<uint2>());
+        HIPCHK(hipGetLastError());
+        if (!device_ptrs) {
+            HIPCHK(hipMemcpyAsync(hits->t, h.t, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(hits->u, h.u, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(hits->v, h.v, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(hits->prim, h.prim, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        }
+    } else {
+        uint8_t* o = occ;
+        if (!device_ptrs) {
+            if ((rc = outb.ensure(n))) return rc;
+            o = outb.as<uint8_t>();
+        }
+        launch_rays_occluded(st, grid, nodes, tris, r, o, n, ctx->spill_buf.as<uint2>());
+        HIPCHK(hipGetLastError());
+        if (!device_ptrs) HIPCHK(hipMemcpyAsync(occ, o, n, hipMemcpyDeviceToHost, st));
+    }
+    if (!device_ptrs) HIPCHK(hipStreamSynchronize(st));
+    return AG_OK;
+}
+
+extern "C" int ag_intersect(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, ag_hits_soa* out) {
+    if (!out) return fail(AG_E_INVALID, "null output");
+    return ray_batch(ctx, s, rays, n, out, nullptr, false);
+}
+extern "C" int ag_occluded(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, uint8_t* out) {
+    if (!out) return fail(AG_E_INVALID, "null output");
+    return ray_batch(ctx, s, rays, n, nullptr, out, false);
+}
+extern "C" int ag_intersect_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, ag_hits_soa* out) {
+    if (!out) return fail(AG_E_INVALID, "null output");
+    return ray_batch(ctx, s, rays, n, out, nullptr, true);
+}
+extern "C" int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, uint8_t* out) {
+    if (!out) return fail(AG_E_INVALID, "null output");
+    return ray_batch(ctx, s, rays, n, nullptr, out, true);
+}

@@ -1,0 +1,311 @@
+/*
+ * argent.h — C ABI of the MI355X-native argent render core.
+ *
+ * The reference's seam is a Rust function call: argent_core::render::render_pass
+ * (crates/argent-core/src/render.rs:67), with Bvh::build / intersect / occluded
+ * (crates/argent-core/src/bvh.rs:117, :159, :216) for geometry queries and
+ * Scene::new (crates/argent-core/src/scene.rs:82) for scene construction.  Each
+ * entry point below names the reference item it replaces.  All functions are
+ * synchronous unless named *_async, return 0 on success and a nonzero AG_E_*
+ * code on failure; ag_last_error() returns a thread-local message for the last
+ * failure on the calling thread.
+ *
+ * Units follow the reference: millimetres, nanometres, degrees.  Spectra are
+ * tabulated at 1 nm from 360 to 830 nm (471 values, argent-core tab.rs:8-32);
+ * metal optical constants on the 5 nm grid 360..830 (95 rows,
+ * argent-spectra metals.rs:53-56).
+ */
+#ifndef ARGENT_H
+#define ARGENT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AG_SPEC_N 471  /* 1 nm table, 360..830 nm */
+#define AG_GRID_N 95   /* 5 nm grid, 360..830 nm */
+#define AG_MAX_PANELS 16
+#define AG_MAX_MATERIALS 32
+
+enum {
+    AG_OK = 0,
+    AG_E_INVALID = 1,   /* bad argument or descriptor */
+    AG_E_DEVICE = 2,    /* HIP runtime error */
+    AG_E_NOMEM = 3,     /* allocation failed or over budget */
+    AG_E_NOTFOUND = 4,  /* unknown name */
+    AG_E_STATE = 5      /* call not valid in the object's state */
+};
+
+/* ---------------------------------------------------------------- materials
+ * argent-core material.rs:27-36 (Material) and bsdf.rs:10-27 (Bsdf). */
+enum {
+    AG_MAT_METAL = 0,    /* Conductor: GGX + exact complex Fresnel + multi-scatter */
+    AG_MAT_GEM = 1,      /* transparent dispersive dielectric with Beer-Lambert */
+    AG_MAT_GEM_OPAQUE = 2, /* opaque stone: coated, GGX r 0.06, eta = n(550) */
+    AG_MAT_PLASTIC = 3,  /* coated diffuse */
+    AG_MAT_DIFFUSE = 4,
+    AG_MAT_BLACK = 5,
+    AG_MAT_OPAL = 6
+};
+
+typedef struct ag_material {
+    int32_t kind;            /* AG_MAT_* */
+    float alpha;             /* GGX alpha = max(r^2, 1e-4) (microfacet.rs:22-25) */
+    float eta;               /* coat index (plastic, opaque gem) */
+    /* metal: n and k on the 5 nm grid (metals.rs ComplexIor) */
+    float metal_n[AG_GRID_N];
+    float metal_k[AG_GRID_N];
+    /* patina layer of a metal (material.rs:15-24); has_patina = 0 disables */
+    int32_t has_patina;
+    float patina_strength, patina_lo, patina_hi, patina_alpha, patina_eta;
+    float patina_base[AG_SPEC_N];
+    /* spec_a: gem ior / plastic base / diffuse albedo / opal body (1 nm table)
+     * spec_b: gem absorption sigma_a in 1/mm (1 nm table) */
+    float spec_a[AG_SPEC_N];
+    float spec_b[AG_SPEC_N];
+    /* opal (material.rs:33, :62-66) */
+    float opal_cell_mm, opal_strength;
+} ag_material;
+
+/* ---------------------------------------------------------------- lights
+ * argent-core light.rs:8-98 (Panel).  has_emission = 0 is a flag (black card). */
+typedef struct ag_panel {
+    float center[3], hu[3], hv[3];
+    int32_t has_emission;
+    int32_t camera_visible;
+    float feather;
+    float emission[AG_SPEC_N];
+} ag_panel;
+
+enum { AG_ENV_GRADIENT = 0, AG_ENV_MAP = 1 };
+
+/* light.rs:110-127 (Env::Gradient) and envmap.rs:107-125 (EnvData / EnvMap).
+ * For a map, weights are the three LinearRgb basis weights per texel
+ * (envmap.rs:147), marginal has height+1 entries and conditional
+ * height*(width+1) (envmap.rs:149-175).  basis holds the three 1 nm basis
+ * spectra of LinearRgb (envmap.rs:31-47). */
+typedef struct ag_env {
+    int32_t kind;
+    float top[AG_SPEC_N], horizon[AG_SPEC_N], bottom[AG_SPEC_N];
+    float power;
+    int32_t width, height;
+    const float* weights;      /* width*height*3 */
+    const float* marginal;     /* height+1 */
+    const float* conditional;  /* height*(width+1) */
+    float total;
+    float rotation_deg;
+    float basis[3][AG_SPEC_N];
+} ag_env;
+
+/* light.rs:130-162 (Floor). */
+typedef struct ag_floor {
+    int32_t present;
+    float y;
+    int32_t material;
+    float fade_start, fade_end;
+} ag_floor;
+
+/* scene.rs:47-64 (Scene) with scene.rs:10-19 (Piece).  Arrays are host memory;
+ * ag_scene_create copies them.  normals and cavity may be NULL. */
+typedef struct ag_scene_desc {
+    const float* positions;      /* n_vertices*3 */
+    const float* normals;        /* n_vertices*3 or NULL */
+    const uint32_t* triangles;   /* n_triangles*3 */
+    const uint16_t* tri_material;/* n_triangles */
+    const float* cavity;         /* n_vertices or NULL */
+    uint32_t n_vertices, n_triangles;
+    int32_t n_materials;
+    ag_material materials[AG_MAX_MATERIALS];
+    int32_t n_panels;
+    ag_panel panels[AG_MAX_PANELS];
+    ag_env env;
+    ag_floor floor;
+    int32_t has_white;
+    float white_xyz[3];
+    float turntable_deg;
+    /* 32x32 GGX energy table E(mu, sqrt(alpha)) (microfacet.rs:93-140) */
+    float energy[32 * 32];
+} ag_scene_desc;
+
+/* camera.rs:15-27 (Camera), 15 floats. */
+typedef struct ag_camera {
+    float pos[3], forward[3], right[3], up[3];
+    float half_w, half_h, lens_radius, focus_dist;
+    float shift[2];
+} ag_camera;
+
+/* integrator.rs:10-24 (PathSettings). */
+typedef struct ag_path_settings {
+    uint32_t max_depth;
+    float clamp_indirect;
+    uint32_t rr_depth;
+} ag_path_settings;
+
+/* Ray batches in structure-of-arrays form (bvh.rs:159 Ray + t_max). */
+typedef struct ag_rays_soa {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+} ag_rays_soa;
+
+/* Closest-hit output (bvh.rs TriHit): prim = original triangle index,
+ * 0xFFFFFFFF on a miss. */
+typedef struct ag_hits_soa {
+    float *t, *u, *v;
+    uint32_t* prim;
+} ag_hits_soa;
+
+typedef struct ag_ctx ag_ctx;
+typedef struct ag_scene ag_scene;
+typedef struct ag_film ag_film;
+
+const char* ag_last_error(void);
+
+/* One context per device: a HIP stream and the device's scratch. */
+int ag_ctx_create(int device, ag_ctx** out);
+void ag_ctx_destroy(ag_ctx* ctx);
+int ag_ctx_synchronize(ag_ctx* ctx);
+
+/* ------------------------------------------------------------ scene build
+ * Host-side builders that fill descriptor parts the way the reference's
+ * constructors do.  They bake spectra to the 1 nm tables the kernels read. */
+
+/* Material::metal(alloy, roughness).with_patina(default_patina(strength, alloy))
+ * (material.rs:39-48, :142-149).  patina_strength < 0 leaves has_patina = 0. */
+int ag_material_metal(const char* alloy, float roughness, float patina_strength, ag_material* out);
+/* Material::gem(name) (material.rs:50-60). */
+int ag_material_gem(const char* name, ag_material* out);
+/* Material::plastic / ::diffuse (material.rs:73-79). */
+int ag_material_plastic(const float rgb[3], float eta, float roughness, ag_material* out);
+int ag_material_diffuse(const float rgb[3], ag_material* out);
+/* Alloy::ior (alloys.rs:142-171) on the 5 nm grid; F0 linear sRGB (alloys.rs:174-176). */
+int ag_alloy_ior(const char* alloy, float n_out[AG_GRID_N], float k_out[AG_GRID_N]);
+int ag_alloy_f0_srgb(const char* alloy, float rgb_out[3]);
+/* karat_gold(au, ag, cu) (alloys.rs:423-460). */
+int ag_karat_gold(float au, float ag, float cu, float n_out[AG_GRID_N], float k_out[AG_GRID_N]);
+/* Gem optics (gems.rs:395-440): kind 0 transparent, 1 opal, 2 opaque. */
+int ag_gem_optics(const char* name, int32_t* kind, float ior_out[AG_SPEC_N],
+                  float abs_out[AG_SPEC_N], float body_rgb_out[3], int32_t* has_body);
+/* studio.rs:15-25 emitter(kelvin, intensity), baked. */
+int ag_spectrum_emitter(float kelvin, float intensity, float out[AG_SPEC_N]);
+/* RgbAlbedo::new (spectrum.rs:194-203) and RgbIlluminant::new (spectrum.rs:395-402), baked. */
+int ag_spectrum_rgb_albedo(const float rgb[3], float out[AG_SPEC_N]);
+int ag_spectrum_rgb_illuminant(const float rgb[3], float out[AG_SPEC_N]);
+/* luminance_y over a 1 nm table (spectrum.rs:413-420). */
+float ag_luminance_y(const float spec[AG_SPEC_N]);
+/* color.rs:32-45 blackbody_white and color.rs:48-55 camera_matrix (row-major 3x3). */
+int ag_blackbody_white(float kelvin, float xyz_out[3]);
+int ag_camera_matrix(int32_t has_white, const float white_xyz[3], float m_out[9]);
+/* CIE helpers (cie.rs:38-49, :154-172). */
+int ag_cie_xyz_bar(float lambda, float out[3]);
+int ag_reflectance_srgb(const float spec[AG_SPEC_N], float rgb_out[3]);
+/* Complex Fresnel of one wavelength and the dielectric form (fresnel.rs:9-25, :81-93). */
+float ag_fresnel_conductor1(float cos_i, float eta, float k);
+float ag_fresnel_dielectric(float cos_i, float eta);
+/* The GGX energy table (microfacet.rs:112-140) into out[32*32]. */
+int ag_energy_table(float out[32 * 32]);
+
+/* Scene descriptor defaults: zeroed, gradient env of zero, no floor, energy
+ * table filled.  Studio::build(radius) + floor_at (studio.rs:87-190, :194-197)
+ * fill panels, env, floor and white; the floor material is appended to the
+ * material list as Prepared::scene does (shot.rs:226-230).  studio is
+ * "noir" | "white" | "velvet" | "ember" | "window". */
+int ag_desc_init(ag_scene_desc* d);
+int ag_desc_apply_studio(ag_scene_desc* d, const char* studio, float radius, float floor_y);
+/* Equirectangular HDR environment from linear RGB texels (EnvData::load,
+ * envmap.rs:141-178, without the file decode).  The arrays are owned by the
+ * library until ag_env_free. */
+int ag_env_from_rgb(const float* rgb, int32_t width, int32_t height, float exposure_ev,
+                    float rotation_deg, ag_env* out);
+void ag_env_free(ag_env* env);
+/* Rebuilds the scene's light CDF (scene.rs:88-110); exposed for tests. */
+int ag_desc_light_cdf(const ag_scene_desc* d, int32_t* n_lights, int32_t lights[AG_MAX_PANELS],
+                      float cdf[AG_MAX_PANELS], float* env_pick);
+
+/* Scene::new + Piece::new (scene.rs:21-32, :82-86): copies the arrays to the
+ * device and builds the BVH.  The scene owns its device memory. */
+int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* desc, ag_scene** out);
+void ag_scene_destroy(ag_scene* s);
+/* Scene::turntable_deg (scene.rs:53-55). */
+int ag_scene_set_turntable(ag_scene* s, float deg);
+/* Replace material slot m (metal variants without a rebuild). */
+int ag_scene_set_material(ag_scene* s, int32_t m, const ag_material* mat);
+/* Bvh::bounds (bvh.rs:145-148) and statistics of the device BVH. */
+int ag_scene_bounds(const ag_scene* s, float lo[3], float hi[3]);
+int ag_scene_stats(const ag_scene* s, uint32_t* nodes, uint32_t* leaves, uint32_t* depth, size_t* bytes);
+
+/* ------------------------------------------------------------ film
+ * render.rs:13-64 (Film): colour, albedo and normal running sums plus a
+ * uniform sample count; storage is device memory. */
+enum { AG_FILM_COLOR = 0, AG_FILM_ALBEDO = 1, AG_FILM_NORMAL = 2, AG_FILM_XYZ = 3 };
+int ag_film_create(ag_ctx* ctx, uint32_t width, uint32_t height, ag_film** out);
+void ag_film_destroy(ag_film* f);
+int ag_film_clear(ag_film* f);
+/* Averages as Film::color/albedo/normal do (render.rs:41-64); AG_FILM_XYZ
+ * returns M^-1 * colour.  out holds width*height*3 floats, host memory. */
+int ag_film_read(const ag_film* f, int32_t which, float* out);
+/* Raw running sums (no division), host memory, width*height*3 floats. */
+int ag_film_read_sums(const ag_film* f, int32_t which, float* out);
+uint32_t ag_film_samples(const ag_film* f);
+
+/* ------------------------------------------------------------ rendering
+ * render_pass(scene, cam, film, ps, count, seed) (render.rs:67-115): adds
+ * `count` samples per pixel to `film`, continuing sample indices from
+ * film->samples.  Synchronous. */
+int ag_render_pass(ag_ctx* ctx, const ag_scene* scene, const ag_camera* cam, ag_film* film,
+                   const ag_path_settings* ps, uint32_t count, uint32_t seed);
+
+/* Geometry queries over the scene's piece BVH in object space:
+ * Bvh::intersect (bvh.rs:159-213) and Bvh::occluded (bvh.rs:216-245).
+ * Host arrays; a ray's t_max is rays->tmax[i] (NULL = +inf). */
+int ag_intersect(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, ag_hits_soa* out);
+int ag_occluded(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, uint8_t* out);
+/* The same on device pointers already resident in HBM; asynchronous on the
+ * context stream.  Returns after the launch. */
+int ag_intersect_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_dev, uint32_t n,
+                        ag_hits_soa* out_dev);
+int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_dev, uint32_t n,
+                       uint8_t* out_dev);
+
+/* ------------------------------------------------------------ batching
+ * Many uneven views of one scene in one wavefront schedule: the turntable
+ * loop of jobs.rs:689-710 and metal variants, with each view's own camera,
+ * turntable angle, metal slot-0 material, film and sample count.  Each
+ * view's film equals what ag_render_pass would give for it alone. */
+typedef struct ag_view {
+    ag_camera camera;
+    float turntable_deg;
+    int32_t material0;      /* index into the materials array, -1 = the scene's slot 0 */
+    ag_film* film;
+    uint32_t count;
+    uint32_t seed;
+} ag_view;
+int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_material* materials, int32_t n_materials,
+                    const ag_view* views, int32_t n_views, const ag_path_settings* ps);
+
+/* ------------------------------------------------------------ tuning
+ * Launch shape of the wavefront kernels (the autotuner's knobs). */
+typedef struct ag_tuning {
+    uint32_t paths_per_wave;   /* path slots in flight */
+    uint32_t block_traverse;   /* threads per block, traversal kernels */
+    uint32_t blocks_per_cu;    /* persistent blocks per CU */
+    uint32_t sort_by_material; /* 1 = shade in material-sorted order */
+} ag_tuning;
+int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
+int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);
+
+/* Kernel timings of the last ag_render_pass / ag_render_views (HIP events on
+ * the context stream), milliseconds, and ray counts. */
+typedef struct ag_stats {
+    double ms_total, ms_extend, ms_shade, ms_shadow, ms_raygen, ms_accum;
+    uint64_t extension_rays, shadow_rays, samples;
+    uint32_t extend_launches, shadow_launches;
+} ag_stats;
+int ag_ctx_stats(const ag_ctx* ctx, ag_stats* out);
+int ag_ctx_enable_timing(ag_ctx* ctx, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARGENT_H */
