@@ -1,0 +1,379 @@
+"""argent_amd — MI355X-native render core of argent.
+
+A thin Python face over the C ABI in include/argent.h (libargent_hip.so,
+hand-written gfx950 HIP kernels).  The names mirror argent-core's API:
+
+    Scene / Piece          crates/argent-core/src/scene.rs:10-86
+    Material constructors  crates/argent-core/src/material.rs:39-79
+    Studio rigs            crates/argent-core/src/studio.rs:87-197
+    Camera.look_at, frame  crates/argent-core/src/camera.rs:38-55, views.rs:45-83
+    Film                   crates/argent-core/src/render.rs:13-64
+    render_pass            crates/argent-core/src/render.rs:67-115
+    Scene.intersect/occluded  crates/argent-core/src/bvh.rs:159-245
+
+Every compute call runs on the GPU through the native library; there is no
+CPU fallback in this package.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from ._abi import ArgentError, check
+
+__all__ = ["Context", "SceneDesc", "Scene", "Film", "PathSettings", "Material", "render_pass", "render_views",
+           "look_at", "frame", "VIEWS", "ArgentError", "alloy_f0_srgb", "alloy_ior"]
+
+VIEWS = {  # views.rs:17-31: (azimuth, elevation, fill)
+    "hero": (34.0, 26.0, 0.72), "front": (0.0, 6.0, 0.70), "face": (0.0, 6.0, 0.70),
+    "side": (90.0, 6.0, 0.72), "profile": (90.0, 6.0, 0.72), "top": (0.0, 62.0, 0.70),
+    "three-quarter": (45.0, 24.0, 0.72), "34": (45.0, 24.0, 0.72), "back": (160.0, 16.0, 0.72),
+    "low": (22.0, 3.0, 0.72), "detail": (14.0, 34.0, 0.92), "macro": (14.0, 34.0, 0.92),
+}
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _ptr(a, t=C.c_float):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+class PathSettings:
+    """integrator.rs:10-24; defaults (32, 24.0, 6)."""
+
+    def __init__(self, max_depth=32, clamp_indirect=24.0, rr_depth=6):
+        self.max_depth, self.clamp_indirect, self.rr_depth = int(max_depth), float(clamp_indirect), int(rr_depth)
+
+    def c(self):
+        return A.PathSettingsC(self.max_depth, self.clamp_indirect, self.rr_depth)
+
+
+class Material:
+    """Constructors of material.rs; each returns the baked ag_material."""
+
+    @staticmethod
+    def metal(alloy="sterling", roughness=0.07, patina=0.8):
+        m = A.Material()
+        check(A.lib().ag_material_metal(alloy.encode(), roughness, -1.0 if patina is None else patina, C.byref(m)))
+        return m
+
+    @staticmethod
+    def gem(name):
+        m = A.Material()
+        check(A.lib().ag_material_gem(name.encode(), C.byref(m)))
+        return m
+
+    @staticmethod
+    def plastic(rgb, eta, roughness):
+        m = A.Material()
+        check(A.lib().ag_material_plastic(_ptr(_f32(rgb)), eta, roughness, C.byref(m)))
+        return m
+
+    @staticmethod
+    def diffuse(rgb):
+        m = A.Material()
+        check(A.lib().ag_material_diffuse(_ptr(_f32(rgb)), C.byref(m)))
+        return m
+
+    @staticmethod
+    def black():
+        m = A.Material()
+        m.kind = A.MAT_BLACK
+        return m
+
+
+def alloy_ior(name):
+    n = np.zeros(A.GRID_N, np.float32)
+    k = np.zeros(A.GRID_N, np.float32)
+    check(A.lib().ag_alloy_ior(name.encode(), _ptr(n), _ptr(k)))
+    return n, k
+
+
+def alloy_f0_srgb(name):
+    rgb = np.zeros(3, np.float32)
+    check(A.lib().ag_alloy_f0_srgb(name.encode(), _ptr(rgb)))
+    return rgb
+
+
+class Context:
+    """One HIP device and stream (ag_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(A.lib().ag_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            A.lib().ag_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def synchronize(self):
+        check(A.lib().ag_ctx_synchronize(self.h))
+
+    @property
+    def tuning(self):
+        t = A.Tuning()
+        check(A.lib().ag_ctx_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in A.Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = A.Tuning()
+        check(A.lib().ag_ctx_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(A.lib().ag_ctx_set_tuning(self.h, C.byref(t)))
+
+    def enable_timing(self, on=True):
+        check(A.lib().ag_ctx_enable_timing(self.h, 1 if on else 0))
+
+    def stats(self):
+        s = A.Stats()
+        check(A.lib().ag_ctx_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in A.Stats._fields_}
+
+
+class SceneDesc:
+    """Host-side scene description (Scene::new inputs, scene.rs:82-86).
+
+    Holds the mesh arrays, materials, panels, environment and floor; passes
+    to both Scene (device) and the oracle unchanged."""
+
+    def __init__(self, positions, triangles, normals=None, tri_material=None, cavity=None):
+        self.positions = _f32(positions, (-1, 3))
+        self.triangles = np.ascontiguousarray(np.asarray(triangles, dtype=np.uint32).reshape(-1, 3))
+        self.normals = None if normals is None else _f32(normals, (-1, 3))
+        self.tri_material = (np.zeros(len(self.triangles), np.uint16) if tri_material is None
+                             else np.ascontiguousarray(np.asarray(tri_material, dtype=np.uint16)))
+        self.cavity = None if cavity is None else _f32(cavity)
+        self.d = A.SceneDesc()
+        check(A.lib().ag_desc_init(C.byref(self.d)))
+        self._env_owned = False
+        self._bind()
+
+    def _bind(self):
+        d = self.d
+        d.positions = _ptr(self.positions)
+        d.triangles = _ptr(self.triangles, C.c_uint32)
+        d.normals = _ptr(self.normals) if self.normals is not None else None
+        d.tri_material = _ptr(self.tri_material, C.c_uint16)
+        d.cavity = _ptr(self.cavity) if self.cavity is not None else None
+        d.n_vertices = len(self.positions)
+        d.n_triangles = len(self.triangles)
+
+    def add_material(self, m):
+        i = self.d.n_materials
+        if i >= A.MAX_MATERIALS:
+            raise ArgentError("too many materials")
+        self.d.materials[i] = m
+        self.d.n_materials = i + 1
+        return i
+
+    def apply_studio(self, studio, radius, floor_y):
+        """Studio::build(radius) + floor_at(floor_y - 0.005) (studio.rs:87-197, shot.rs:226-241)."""
+        check(A.lib().ag_desc_apply_studio(C.byref(self.d), studio.encode(), radius, floor_y))
+
+    def set_env_map(self, rgb, exposure_ev=0.0, rotation_deg=0.0):
+        """EnvData::load on in-memory linear RGB texels (envmap.rs:141-178)."""
+        rgb = _f32(rgb)
+        h, w = rgb.shape[0], rgb.shape[1]
+        keep = (self.d.env.top[:], self.d.env.horizon[:], self.d.env.bottom[:], self.d.env.power)
+        self.free_env()
+        env = A.Env()
+        check(A.lib().ag_env_from_rgb(_ptr(rgb.reshape(-1)), w, h, exposure_ev, rotation_deg, C.byref(env)))
+        env.top[:], env.horizon[:], env.bottom[:], env.power = keep
+        self.d.env = env
+        self._env_owned = True
+
+    def free_env(self):
+        if self._env_owned:
+            A.lib().ag_env_free(C.byref(self.d.env))
+            self._env_owned = False
+
+    def __del__(self):
+        try:
+            self.free_env()
+        except Exception:
+            pass
+
+    def add_panel(self, center, hu, hv, emission=None, camera_visible=False, feather=0.35):
+        i = self.d.n_panels
+        p = self.d.panels[i]
+        p.center[:] = list(map(float, center))
+        p.hu[:] = list(map(float, hu))
+        p.hv[:] = list(map(float, hv))
+        p.has_emission = 0 if emission is None else 1
+        if emission is not None:
+            p.emission[:] = list(map(float, _f32(emission)))
+        p.camera_visible = 1 if camera_visible else 0
+        p.feather = feather
+        self.d.n_panels = i + 1
+        return i
+
+    def set_gradient_env(self, top, horizon, bottom, power):
+        for name, rgb in (("top", top), ("horizon", horizon), ("bottom", bottom)):
+            out = np.zeros(A.SPEC_N, np.float32)
+            check(A.lib().ag_spectrum_rgb_illuminant(_ptr(_f32(rgb)), _ptr(out)))
+            getattr(self.d.env, name)[:] = out.tolist()
+        self.d.env.power = power
+
+    def set_white_kelvin(self, kelvin):
+        xyz = np.zeros(3, np.float32)
+        check(A.lib().ag_blackbody_white(kelvin, _ptr(xyz)))
+        self.d.has_white = 1
+        self.d.white_xyz[:] = xyz.tolist()
+
+    def bounds(self):
+        return self.positions.min(axis=0), self.positions.max(axis=0)
+
+
+class Scene:
+    """Device scene (ag_scene): BVH, mesh attributes, tables resident in HBM."""
+
+    def __init__(self, ctx, desc):
+        self.ctx = ctx
+        self.desc = desc
+        h = C.c_void_p()
+        desc._bind()
+        check(A.lib().ag_scene_create(ctx.h, C.byref(desc.d), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            A.lib().ag_scene_destroy(self.h)
+            self.h = None
+
+    def set_turntable(self, deg):
+        check(A.lib().ag_scene_set_turntable(self.h, deg))
+
+    def set_material(self, m, mat):
+        check(A.lib().ag_scene_set_material(self.h, m, C.byref(mat)))
+
+    def stats(self):
+        n, l, dp = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        b = C.c_size_t()
+        check(A.lib().ag_scene_stats(self.h, C.byref(n), C.byref(l), C.byref(dp), C.byref(b)))
+        return {"nodes": n.value, "leaves": l.value, "depth": dp.value, "bytes": b.value}
+
+    def intersect(self, o, d, tmax=None):
+        """Bvh::intersect on the GPU for rays (n,3) o and d: returns t, u, v, prim."""
+        o, d = _f32(o, (-1, 3)), _f32(d, (-1, 3))
+        n = len(o)
+        cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + [np.ascontiguousarray(d[:, k]) for k in range(3)]
+        tm = None if tmax is None else _f32(tmax)
+        rays = A.RaysSoA(*[_ptr(c) for c in cols], _ptr(tm) if tm is not None else None)
+        t, u, v = (np.empty(n, np.float32) for _ in range(3))
+        prim = np.empty(n, np.uint32)
+        hits = A.HitsSoA(_ptr(t), _ptr(u), _ptr(v), _ptr(prim, C.c_uint32))
+        check(A.lib().ag_intersect(self.ctx.h, self.h, C.byref(rays), n, C.byref(hits)))
+        return t, u, v, prim
+
+    def occluded(self, o, d, tmax=None):
+        o, d = _f32(o, (-1, 3)), _f32(d, (-1, 3))
+        n = len(o)
+        cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + [np.ascontiguousarray(d[:, k]) for k in range(3)]
+        tm = None if tmax is None else _f32(tmax)
+        rays = A.RaysSoA(*[_ptr(c) for c in cols], _ptr(tm) if tm is not None else None)
+        out = np.empty(n, np.uint8)
+        check(A.lib().ag_occluded(self.ctx.h, self.h, C.byref(rays), n, _ptr(out, C.c_uint8)))
+        return out.astype(bool)
+
+    def bsdf_probe(self, mat, lam, mask, u, wo, wi):
+        mat = np.ascontiguousarray(np.asarray(mat, np.int32))
+        n = len(mat)
+        out = np.empty((n, 18), np.float32)
+        args = [_f32(lam, (n, 4)), _f32(mask, (n,)), _f32(u, (n, 4)), _f32(wo, (n, 3)), _f32(wi, (n, 3))]
+        check(A.lib().ag_bsdf_probe(self.ctx.h, self.h, n, _ptr(mat, C.c_int32), *[_ptr(a) for a in args],
+                                    _ptr(out)))
+        return out
+
+
+class Film:
+    """render.rs:13-64; sums live in device memory."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx, self.width, self.height = ctx, int(width), int(height)
+        h = C.c_void_p()
+        check(A.lib().ag_film_create(ctx.h, self.width, self.height, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            A.lib().ag_film_destroy(self.h)
+            self.h = None
+
+    @property
+    def samples(self):
+        return A.lib().ag_film_samples(self.h)
+
+    def clear(self):
+        check(A.lib().ag_film_clear(self.h))
+
+    def _read(self, which, sums=False):
+        out = np.empty((self.height, self.width, 3), np.float32)
+        f = A.lib().ag_film_read_sums if sums else A.lib().ag_film_read
+        check(f(self.h, which, _ptr(out)))
+        return out
+
+    def color(self):
+        return self._read(A.FILM_COLOR)
+
+    def albedo(self):
+        return self._read(A.FILM_ALBEDO)
+
+    def normal(self):
+        return self._read(A.FILM_NORMAL)
+
+    def xyz(self):
+        return self._read(A.FILM_XYZ)
+
+    def sums(self):
+        return np.stack([self._read(k, sums=True) for k in (A.FILM_COLOR, A.FILM_ALBEDO, A.FILM_NORMAL)])
+
+
+def look_at(pos, target, up_hint=(0.0, 1.0, 0.0), focal_mm=100.0, f_stop=0.0, aspect=1.0):
+    cam = A.Camera()
+    check(A.lib().ag_camera_look_at(_ptr(_f32(pos)), _ptr(_f32(target)), _ptr(_f32(up_hint)), focal_mm, f_stop,
+                                    aspect, C.byref(cam)))
+    return cam
+
+
+def frame(points, view="hero", focal_mm=100.0, f_stop=0.0, aspect=1.0):
+    az, el, fill = VIEWS[view] if isinstance(view, str) else view
+    pts = _f32(points, (-1, 3))
+    cam = A.Camera()
+    check(A.lib().ag_camera_frame(_ptr(pts.reshape(-1)), len(pts), az, el, fill, focal_mm, f_stop, aspect,
+                                  C.byref(cam)))
+    return cam
+
+
+def render_pass(ctx, scene, cam, film, ps, count, seed):
+    """render.rs:67-115 on the GPU: add `count` samples per pixel to `film`."""
+    check(A.lib().ag_render_pass(ctx.h, scene.h, C.byref(cam), film.h, C.byref(ps.c()), count, seed))
+
+
+def render_views(ctx, scene, views, ps, materials=()):
+    """Many views of one scene in one wavefront schedule.  views: list of
+    dicts with camera, film, count, seed, turntable_deg, material0 (index
+    into `materials` or -1)."""
+    arr = (A.View * len(views))()
+    for i, v in enumerate(views):
+        arr[i].camera = v["camera"]
+        arr[i].turntable_deg = v.get("turntable_deg", 0.0)
+        arr[i].material0 = v.get("material0", -1)
+        arr[i].film = v["film"].h
+        arr[i].count = v["count"]
+        arr[i].seed = v["seed"]
+    mats = (A.Material * max(1, len(materials)))(*materials) if materials else None
+    check(A.lib().ag_render_views(ctx.h, scene.h, mats, len(materials), arr, len(views), C.byref(ps.c())))

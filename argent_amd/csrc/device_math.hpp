@@ -116,11 +116,52 @@ AGD uint32_t hash32(uint32_t x) {
     x ^= x >> 16;
     return x;
 }
-// The stratified dimensions of sampler.rs:42-55 as a counter-based stream:
-// value = hash of (pixel seed, sample index, dimension), 24-bit float.
+// The stratified dimensions of sampler.rs:42-55: Owen-scrambled Sobol in
+// 4-dimension sets (Burley 2020, "Practical Hash-based Owen Scrambling":
+// nested uniform scrambling by a Laine-Karras hash, shuffled index), with
+// Joe-Kuo direction numbers for the first four Sobol dimensions.  A pure
+// function of (pixel seed, sample index, dimension).  Past 256 dimensions or
+// 2^16 samples it is a plain hash of the same three numbers.
+__device__ __constant__ uint32_t SOBOL_DIRS[4][32] = {
+    {0x80000000u, 0x40000000u, 0x20000000u, 0x10000000u, 0x08000000u, 0x04000000u, 0x02000000u, 0x01000000u,
+     0x00800000u, 0x00400000u, 0x00200000u, 0x00100000u, 0x00080000u, 0x00040000u, 0x00020000u, 0x00010000u,
+     0x00008000u, 0x00004000u, 0x00002000u, 0x00001000u, 0x00000800u, 0x00000400u, 0x00000200u, 0x00000100u,
+     0x00000080u, 0x00000040u, 0x00000020u, 0x00000010u, 0x00000008u, 0x00000004u, 0x00000002u, 0x00000001u},
+    {0x80000000u, 0xc0000000u, 0xa0000000u, 0xf0000000u, 0x88000000u, 0xcc000000u, 0xaa000000u, 0xff000000u,
+     0x80800000u, 0xc0c00000u, 0xa0a00000u, 0xf0f00000u, 0x88880000u, 0xcccc0000u, 0xaaaa0000u, 0xffff0000u,
+     0x80008000u, 0xc000c000u, 0xa000a000u, 0xf000f000u, 0x88008800u, 0xcc00cc00u, 0xaa00aa00u, 0xff00ff00u,
+     0x80808080u, 0xc0c0c0c0u, 0xa0a0a0a0u, 0xf0f0f0f0u, 0x88888888u, 0xccccccccu, 0xaaaaaaaau, 0xffffffffu},
+    {0x80000000u, 0xc0000000u, 0x60000000u, 0x90000000u, 0xe8000000u, 0x5c000000u, 0x8e000000u, 0xc5000000u,
+     0x68800000u, 0x9cc00000u, 0xee600000u, 0x55900000u, 0x80680000u, 0xc09c0000u, 0x60ee0000u, 0x90550000u,
+     0xe8808000u, 0x5cc0c000u, 0x8e606000u, 0xc5909000u, 0x6868e800u, 0x9c9c5c00u, 0xeeee8e00u, 0x5555c500u,
+     0x8000e880u, 0xc0005cc0u, 0x60008e60u, 0x9000c590u, 0xe8006868u, 0x5c009c9cu, 0x8e00eeeeu, 0xc5005555u},
+    {0x80000000u, 0xc0000000u, 0x20000000u, 0x50000000u, 0xf8000000u, 0x74000000u, 0xa2000000u, 0x93000000u,
+     0xd8800000u, 0x25400000u, 0x59e00000u, 0xe6d00000u, 0x78080000u, 0xb40c0000u, 0x82020000u, 0xc3050000u,
+     0x208f8000u, 0x51474000u, 0xfbea2000u, 0x75d93000u, 0xa0858800u, 0x914e5400u, 0xdbe79e00u, 0x25db6d00u,
+     0x58800080u, 0xe54000c0u, 0x79e00020u, 0xb6d00050u, 0x800800f8u, 0xc00c0074u, 0x200200a2u, 0x50050093u}};
+AGD uint32_t lk_permute(uint32_t x, uint32_t seed) {
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return x;
+}
+AGD uint32_t nested_scramble(uint32_t x, uint32_t seed) { return __brev(lk_permute(__brev(x), seed)); }
 AGD float rng_dim(uint32_t pix_seed, uint32_t si, uint32_t dim) {
-    uint32_t h = hash32(pix_seed ^ hash32(si ^ hash32(dim ^ 0x2f0b6e35u)));
-    return (float)(h >> 8) * (1.0f / 16777216.0f);
+    if (dim >= 256u || si >= 65536u) {
+        uint32_t h = hash32(pix_seed ^ hash32(si ^ hash32(dim ^ 0x2f0b6e35u)));
+        return (float)(h >> 8) * (1.0f / 16777216.0f);
+    }
+    uint32_t set = dim >> 2, k = dim & 3u;
+    uint32_t s = hash32(pix_seed ^ hash32(set * 0x9e3779b9u + 0x632be5abu));
+    uint32_t i = nested_scramble(si, s);
+    uint32_t x = 0;
+    for (int b = 0; b < 32 && i; b++, i >>= 1)
+        if (i & 1u) x ^= SOBOL_DIRS[k][b];
+    uint32_t hk = hash32(k + 1u);
+    x = nested_scramble(x, s ^ (hk + (s << 6) + (s >> 2)));
+    return (float)(x >> 8) * (1.0f / 16777216.0f);
 }
 // sampler.rs:64-70
 AGD float next_random(uint64_t& s) {

@@ -452,3 +452,98 @@ extern "C" int ag_desc_light_cdf(const ag_scene_desc* d, int32_t* n_lights, int3
     for (int i = 0; i < n; i++) lights[i] = l[i];
     return AG_OK;
 }
+
+// ------------------------------------------------------------------ camera
+// camera.rs:38-55 (Camera::look_at)
+static void look_at(V3 pos, V3 target, V3 up_hint, float focal_mm, float f_stop, float aspect, ag_camera* c) {
+    V3 forward = normalize(target - pos);
+    V3 right = cross(forward, up_hint);
+    if (length_sq(right) < 1e-8f) right = cross(forward, V3(0, 0, 1));
+    right = normalize(right);
+    V3 up = cross(right, forward);
+    float long_half = 18.0f / focal_mm;
+    float half_w, half_h;
+    if (aspect >= 1.0f) {
+        half_w = long_half;
+        half_h = long_half / aspect;
+    } else {
+        half_w = long_half * aspect;
+        half_h = long_half;
+    }
+    std::memset(c, 0, sizeof(*c));
+    c->pos[0] = pos.x; c->pos[1] = pos.y; c->pos[2] = pos.z;
+    c->forward[0] = forward.x; c->forward[1] = forward.y; c->forward[2] = forward.z;
+    c->right[0] = right.x; c->right[1] = right.y; c->right[2] = right.z;
+    c->up[0] = up.x; c->up[1] = up.y; c->up[2] = up.z;
+    c->half_w = half_w;
+    c->half_h = half_h;
+    c->focus_dist = length(target - pos);
+    c->lens_radius = f_stop > 0.0f ? 0.5f * focal_mm / f_stop : 0.0f;
+}
+
+extern "C" int ag_camera_look_at(const float* pos, const float* target, const float* up_hint, float focal_mm,
+                                 float f_stop, float aspect, ag_camera* out) {
+    if (!pos || !target || !up_hint || !out) return fail(AG_E_INVALID, "null argument");
+    look_at(V3(pos[0], pos[1], pos[2]), V3(target[0], target[1], target[2]), V3(up_hint[0], up_hint[1], up_hint[2]),
+            focal_mm, f_stop, aspect, out);
+    return AG_OK;
+}
+
+// camera.rs:73-80 (Camera::project)
+static bool project(const ag_camera* c, V3 p, float* x, float* y) {
+    V3 d = p - V3(c->pos[0], c->pos[1], c->pos[2]);
+    float z = dot(d, V3(c->forward[0], c->forward[1], c->forward[2]));
+    if (z <= 1e-6f) return false;
+    *x = dot(d, V3(c->right[0], c->right[1], c->right[2])) / z / c->half_w - c->shift[0];
+    *y = dot(d, V3(c->up[0], c->up[1], c->up[2])) / z / c->half_h - c->shift[1];
+    return true;
+}
+
+// views.rs:45-83 (frame): place a camera along the view so the points fill
+// `fill` of the frame, centred.
+extern "C" int ag_camera_frame(const float* points, uint32_t n, float azimuth_deg, float elevation_deg, float fill,
+                               float focal_mm, float f_stop, float aspect, ag_camera* out) {
+    if (!points || !out || n == 0) return fail(AG_E_INVALID, "no points to frame");
+    const float k = 3.14159265358979323846f / 180.0f;
+    V3 lo(3.4028235e38f, 3.4028235e38f, 3.4028235e38f), hi(-3.4028235e38f, -3.4028235e38f, -3.4028235e38f);
+    for (uint32_t i = 0; i < n; i++) {
+        V3 p(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
+        lo = vmin(lo, p);
+        hi = vmax(hi, p);
+    }
+    V3 center = (lo + hi) * 0.5f;
+    float radius = std::fmax(length((hi - lo) * 0.5f), 1e-3f);
+    float sa = std::sin(azimuth_deg * k), ca = std::cos(azimuth_deg * k);
+    float se = std::sin(elevation_deg * k), ce = std::cos(elevation_deg * k);
+    V3 dir(sa * ce, se, ca * ce);
+    V3 up = std::fabs(dir.y) > 0.98f ? V3(0, 0, 1) : V3(0, 1, 0);
+    float half = std::fmin(18.0f / focal_mm, 18.0f / focal_mm * (aspect < 1.0f ? aspect : 1.0f / aspect));
+    float fl = std::fmax(fill, 0.05f);
+    float dist = radius / std::sin(std::atan(half)) / fl;
+    V3 target = center;
+    ag_camera cam;
+    look_at(target + dir * dist, target, up, focal_mm, f_stop, aspect, &cam);
+    for (int it = 0; it < 6; it++) {
+        float mnx = 3.4028235e38f, mxx = -3.4028235e38f, mny = 3.4028235e38f, mxy = -3.4028235e38f;
+        for (uint32_t i = 0; i < n; i++) {
+            float x, y;
+            if (project(&cam, V3(points[3 * i], points[3 * i + 1], points[3 * i + 2]), &x, &y)) {
+                mnx = std::fmin(mnx, x);
+                mxx = std::fmax(mxx, x);
+                mny = std::fmin(mny, y);
+                mxy = std::fmax(mxy, y);
+            }
+        }
+        if (mnx > mxx) break;
+        float ext = std::fmax((mxx - mnx) * 0.5f, (mxy - mny) * 0.5f);
+        float cx = (mxx + mnx) * 0.5f * cam.half_w;
+        float cy = (mxy + mny) * 0.5f * cam.half_h;
+        V3 cpos(cam.pos[0], cam.pos[1], cam.pos[2]);
+        float pd = dot(center - cpos, V3(cam.forward[0], cam.forward[1], cam.forward[2]));
+        target = target + V3(cam.right[0], cam.right[1], cam.right[2]) * (cx * pd) + V3(cam.up[0], cam.up[1], cam.up[2]) * (cy * pd);
+        dist *= ext / fl;
+        look_at(target + dir * dist, target, up, focal_mm, f_stop, aspect, &cam);
+    }
+    *out = cam;
+    return AG_OK;
+}

@@ -72,6 +72,8 @@ AGD void st_push(const TravStack& s, int& sp, uint32_t e, float t) {
         s.tn[sp * s.nthreads + s.tid] = t;
     } else if (sp < STACK_LDS + STACK_SPILL) {
         s.spill[sp - STACK_LDS] = make_uint2(e, __float_as_uint(t));
+    } else {
+        return;  // full: unreachable, the host rejects trees deeper than the stack
     }
     sp++;
 }
@@ -169,7 +171,7 @@ AGD TriHit bvh_closest(const float4* __restrict__ nodes, const float4* __restric
                 float u, v;
                 uint32_t prim;
                 float t = tri_test(tris + (size_t)(first + k) * 3, o, d, &u, &v, &prim);
-                if (t > 1e-7f && (t < best.t || (t == best.t && prim < best.prim))) {
+                if (t > 1e-7f && (t < best.t || (t == best.t && best.prim != 0xFFFFFFFFu && prim < best.prim))) {
                     best.t = t;
                     best.u = u;
                     best.v = v;
@@ -1015,6 +1017,41 @@ __global__ void k_film_commit(float* __restrict__ film, float* __restrict__ acc,
     }
 }
 
+// Per-wavelength BSDF evaluation on given inputs (material.rs:96-138 +
+// bsdf.rs:53-239) in the local frame with +Z the shading normal; the hit
+// context is p_obj = (0.1, 0.2, 0.3), n_world = +Z.  out: 18 floats per probe.
+__global__ void k_bsdf_probe(const DScene* __restrict__ Sp, ProbeIn in, float* out, uint32_t n) {
+    const DScene& S = *Sp;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const DMat& M = S.mat[in.mat[i]];
+        Spec lam{{in.lam[4 * i], in.lam[4 * i + 1], in.lam[4 * i + 2], in.lam[4 * i + 3]}};
+        DView V{};
+        V.ts = 0.0f;
+        V.tc = 1.0f;
+        f3 nz{0.0f, 0.0f, 1.0f};
+        Frame fr = make_frame(nz);
+        const float* u = in.u + 4 * i;
+        Bsdf b = material_bsdf(M, lam, in.mask[i], u[0], f3{0.1f, 0.2f, 0.3f}, nz, fr, V);
+        f3 wo = ld3(in.wo + 3 * i), wi = ld3(in.wi + 3 * i);
+        Spec e = bsdf_eval(b, S.energy, wo, wi);
+        float* o = out + 18 * (size_t)i;
+        for (int k = 0; k < 4; k++) o[k] = e.v[k];
+        o[4] = bsdf_pdf(b, wo, wi);
+        BSample sm{};
+        bool ok = bsdf_sample(b, S.energy, wo, u[1], u[2], u[3], &sm);
+        o[5] = ok ? 1.0f : 0.0f;
+        o[6] = ok ? sm.wi.x : 0.0f;
+        o[7] = ok ? sm.wi.y : 0.0f;
+        o[8] = ok ? sm.wi.z : 0.0f;
+        for (int k = 0; k < 4; k++) o[9 + k] = ok ? sm.f.v[k] : 0.0f;
+        o[13] = ok ? sm.pdf : 0.0f;
+        o[14] = ok && sm.delta ? 1.0f : 0.0f;
+        o[15] = ok && sm.transmit ? 1.0f : 0.0f;
+        o[16] = ok && sm.term2 ? 1.0f : 0.0f;
+        o[17] = bsdf_is_delta(b) ? 1.0f : 0.0f;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n) {
     hipLaunchKernelGGL(k_zero_u32, dim3(std::max(1u, std::min((n + 255) / 256, 1024u))), dim3(256), 0, s, p, n);
@@ -1048,6 +1085,9 @@ void launch_shadow(hipStream_t s, int grid, const DScene* S, const DView* views,
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
                        DPaths P) {
     hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, s, segs, nseg, views, cmf, P);
+}
+void launch_bsdf_probe(hipStream_t s, const DScene* S, ProbeIn in, float* out, uint32_t n) {
+    hipLaunchKernelGGL(k_bsdf_probe, dim3(std::max(1u, std::min((n + 255) / 256, 1024u))), dim3(256), 0, s, S, in, out, n);
 }
 void launch_film_commit(hipStream_t s, float* film, float* acc, size_t n) {
     unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);

@@ -16,6 +16,15 @@ struct HitsSoA {
     uint32_t* prim;
 };
 
+struct ProbeIn {
+    const int32_t* mat;
+    const float* lam;   // 4 per probe
+    const float* mask;
+    const float* u;     // 4 per probe: layer, u0, u1, u2
+    const float* wo;    // 3 per probe
+    const float* wi;    // 3 per probe
+};
+void launch_bsdf_probe(hipStream_t s, const DScene* S, ProbeIn in, float* out, uint32_t n);
 void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n);
 void launch_rays_intersect(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, HitsSoA h,
                            uint32_t n, uint2* spill);

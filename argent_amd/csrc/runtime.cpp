@@ -849,3 +849,30 @@ extern "C" int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_
     if (!out) return fail(AG_E_INVALID, "null output");
     return ray_batch(ctx, s, rays, n, nullptr, out, true);
 }
+
+extern "C" int ag_bsdf_probe(ag_ctx* ctx, const ag_scene* s, uint32_t n, const int32_t* mat, const float* lam,
+                             const float* mask, const float* u, const float* wo, const float* wi, float* out) {
+    if (!ctx || !s || !mat || !lam || !mask || !u || !wo || !wi || !out) return fail(AG_E_INVALID, "null argument");
+    if (n == 0) return AG_OK;
+    for (uint32_t i = 0; i < n; i++)
+        if (mat[i] < 0 || mat[i] >= s->hs.n_mat) return fail(AG_E_INVALID, "material index out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    size_t floats = (size_t)n * (4 + 1 + 4 + 3 + 3);
+    DevBuf in, ob, mb;
+    int rc;
+    if ((rc = in.ensure(floats * 4)) || (rc = ob.ensure((size_t)n * 18 * 4)) || (rc = mb.ensure((size_t)n * 4))) return rc;
+    float* b = in.as<float>();
+    HIPCHK(hipMemcpyAsync(mb.p, mat, (size_t)n * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b, lam, (size_t)n * 16, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b + 4 * (size_t)n, mask, (size_t)n * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b + 5 * (size_t)n, u, (size_t)n * 16, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b + 9 * (size_t)n, wo, (size_t)n * 12, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b + 12 * (size_t)n, wi, (size_t)n * 12, hipMemcpyHostToDevice, st));
+    ProbeIn pi{mb.as<int32_t>(), b, b + 4 * (size_t)n, b + 5 * (size_t)n, b + 9 * (size_t)n, b + 12 * (size_t)n};
+    launch_bsdf_probe(st, s->d_scene.as<DScene>(), pi, ob.as<float>(), n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ob.p, (size_t)n * 18 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return AG_OK;
+}

@@ -137,6 +137,16 @@ typedef struct ag_camera {
     float shift[2];
 } ag_camera;
 
+/* Camera::look_at (camera.rs:38-55): 36 mm long side, lens radius
+ * 0.5 * focal / f_stop (f_stop <= 0 disables depth of field). */
+int ag_camera_look_at(const float pos[3], const float target[3], const float up_hint[3], float focal_mm,
+                      float f_stop, float aspect, ag_camera* out);
+/* views.rs:45-83 (frame): a camera along the view direction (azimuth from +Z
+ * toward +X, elevation above the horizon) that fits the points (n x 3) to
+ * `fill` of the frame, centred. */
+int ag_camera_frame(const float* points, uint32_t n, float azimuth_deg, float elevation_deg, float fill,
+                    float focal_mm, float f_stop, float aspect, ag_camera* out);
+
 /* integrator.rs:10-24 (PathSettings). */
 typedef struct ag_path_settings {
     uint32_t max_depth;
@@ -267,6 +277,14 @@ int ag_intersect_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_
                         ag_hits_soa* out_dev);
 int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_dev, uint32_t n,
                        uint8_t* out_dev);
+
+/* Material::bsdf + Bsdf::{eval,pdf,sample} (material.rs:96-138, bsdf.rs:53-239)
+ * evaluated on the device for n given inputs, for checking the shading code:
+ * material index, 4 wavelengths, cavity mask, u = [layer, u0, u1, u2], wo and
+ * wi in the local frame.  out: n x 18 floats = eval(4), pdf, sampled(0/1),
+ * wi(3), f(4), pdf, delta, transmit, terminate_secondary, is_delta. */
+int ag_bsdf_probe(ag_ctx* ctx, const ag_scene* s, uint32_t n, const int32_t* mat, const float* lam4,
+                  const float* mask, const float* u4, const float* wo3, const float* wi3, float* out18);
 
 /* ------------------------------------------------------------ batching
  * Many uneven views of one scene in one wavefront schedule: the turntable

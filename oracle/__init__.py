@@ -1,0 +1,173 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the oracle (liboracle.so).
+
+The oracle is the CPU restatement of the reference's render core (see
+oracle.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg import this module, as the checker of the HIP path; argent_amd never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from argent_amd import _abi as A
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+MODE_REFERENCE, MODE_ROBUST = 0, 1
+
+_f = C.POINTER(C.c_float)
+_u32 = C.POINTER(C.c_uint32)
+_u8 = C.POINTER(C.c_uint8)
+_u64 = C.POINTER(C.c_uint64)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = C.CDLL(LIB)
+        rays = [_f] * 7
+        sigs = [
+            ("or_scene_create", C.c_void_p, [C.POINTER(A.SceneDesc)]),
+            ("or_scene_destroy", None, [C.c_void_p]),
+            ("or_scene_set_turntable", None, [C.c_void_p, C.c_float]),
+            ("or_scene_set_material", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(A.Material)]),
+            ("or_scene_stats", C.c_int, [C.c_void_p, _u32, _u32]),
+            ("or_intersect", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _f, _f, _f, _u32]),
+            ("or_occluded", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _u8]),
+            ("or_brute_intersect", C.c_int, [_f, _u32, C.c_uint32, *rays, C.c_uint32, C.c_int32, _f, _f, _f, _u32]),
+            ("or_render_pass", C.c_int, [C.c_void_p, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, _f, C.c_uint32,
+                                         C.POINTER(A.PathSettingsC), C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32,
+                                         C.c_uint32, _u64]),
+            ("or_energy_table", C.c_int, [_f]),
+            ("or_fresnel_dielectric", C.c_float, [C.c_float, C.c_float]),
+            ("or_fresnel_conductor1", C.c_float, [C.c_float, C.c_float, C.c_float]),
+            ("or_sample_wavelengths", C.c_int, [C.c_float, _f, _f]),
+            ("or_camera_matrix", C.c_int, [C.c_int32, _f, _f]),
+            ("or_rng_dim", C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            ("or_camera_ray", C.c_int, [C.POINTER(A.Camera), C.c_float, C.c_float, C.c_float, C.c_float, _f, _f]),
+            ("or_bsdf_probe", C.c_int, [C.c_void_p, C.c_int32, _f, C.c_float, _f, _f, _f, _f]),
+        ]
+        for name, res, args in sigs:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_float):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def _rays(o, d, tmax):
+    o = np.ascontiguousarray(np.asarray(o, np.float32).reshape(-1, 3))
+    d = np.ascontiguousarray(np.asarray(d, np.float32).reshape(-1, 3))
+    cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + [np.ascontiguousarray(d[:, k]) for k in range(3)]
+    tm = None if tmax is None else np.ascontiguousarray(np.asarray(tmax, np.float32))
+    return cols, tm, len(o)
+
+
+class OracleScene:
+    def __init__(self, desc):
+        desc._bind()
+        self.desc = desc
+        self.h = lib().or_scene_create(C.byref(desc.d))
+        if not self.h:
+            raise RuntimeError("oracle scene creation failed")
+
+    def close(self):
+        if self.h:
+            lib().or_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_turntable(self, deg):
+        lib().or_scene_set_turntable(self.h, deg)
+
+    def set_material(self, m, mat):
+        assert lib().or_scene_set_material(self.h, m, C.byref(mat)) == 0
+
+    def intersect(self, o, d, tmax=None, mode=MODE_ROBUST, threads=0):
+        cols, tm, n = _rays(o, d, tmax)
+        t, u, v = (np.empty(n, np.float32) for _ in range(3))
+        prim = np.empty(n, np.uint32)
+        lib().or_intersect(self.h, *[_p(c) for c in cols], _p(tm), n, mode, threads, _p(t), _p(u), _p(v),
+                           _p(prim, C.c_uint32))
+        return t, u, v, prim
+
+    def occluded(self, o, d, tmax=None, mode=MODE_ROBUST, threads=0):
+        cols, tm, n = _rays(o, d, tmax)
+        out = np.empty(n, np.uint8)
+        lib().or_occluded(self.h, *[_p(c) for c in cols], _p(tm), n, mode, threads, _p(out, C.c_uint8))
+        return out.astype(bool)
+
+    def render_pass(self, cam, width, height, sums, first, ps, count, seed, threads=0, rows=None):
+        """sums: float32 (3, h, w, 3) array updated in place.  Returns (segments, shadows, samples)."""
+        assert sums.dtype == np.float32 and sums.flags.c_contiguous and sums.size == 9 * width * height
+        r0, r1 = rows if rows else (0, height)
+        st = np.zeros(3, np.uint64)
+        lib().or_render_pass(self.h, C.byref(cam), width, height, _p(sums), first, C.byref(ps.c()), count, seed,
+                             threads, r0, r1, _p(st, C.c_uint64))
+        return tuple(int(x) for x in st)
+
+    def bsdf_probe(self, m, lam, mask, u, wo, wi):
+        out = np.empty(18, np.float32)
+        args = [np.ascontiguousarray(np.asarray(a, np.float32)) for a in (lam, u, wo, wi)]
+        lib().or_bsdf_probe(self.h, m, _p(args[0]), float(mask), _p(args[1]), _p(args[2]), _p(args[3]), _p(out))
+        return out
+
+
+def brute_intersect(positions, triangles, o, d, tmax=None, threads=0):
+    pos = np.ascontiguousarray(np.asarray(positions, np.float32).reshape(-1))
+    tri = np.ascontiguousarray(np.asarray(triangles, np.uint32).reshape(-1))
+    cols, tm, n = _rays(o, d, tmax)
+    t, u, v = (np.empty(n, np.float32) for _ in range(3))
+    prim = np.empty(n, np.uint32)
+    lib().or_brute_intersect(_p(pos), _p(tri, C.c_uint32), len(tri) // 3, *[_p(c) for c in cols], _p(tm), n, threads,
+                             _p(t), _p(u), _p(v), _p(prim, C.c_uint32))
+    return t, u, v, prim
+
+
+def energy_table():
+    out = np.empty(1024, np.float32)
+    lib().or_energy_table(_p(out))
+    return out
+
+
+def fresnel_dielectric(c, eta):
+    return lib().or_fresnel_dielectric(c, eta)
+
+
+def fresnel_conductor1(c, eta, k):
+    return lib().or_fresnel_conductor1(c, eta, k)
+
+
+def sample_wavelengths(u):
+    lam = np.empty(4, np.float32)
+    pdf = np.empty(4, np.float32)
+    lib().or_sample_wavelengths(u, _p(lam), _p(pdf))
+    return lam, pdf
+
+
+def camera_matrix(white=None):
+    m = np.empty(9, np.float32)
+    w = np.zeros(3, np.float32) if white is None else np.asarray(white, np.float32)
+    lib().or_camera_matrix(0 if white is None else 1, _p(w), _p(m))
+    return m.reshape(3, 3)
+
+
+def rng_dim(ps, si, dim):
+    return lib().or_rng_dim(ps, si, dim)

@@ -1,0 +1,74 @@
+/*
+ * oracle.h — TEST INFRASTRUCTURE ONLY.  A CPU restatement of the reference's
+ * render core (argent-core render.rs, integrator.rs, scene.rs, light.rs,
+ * envmap.rs, bvh.rs, bsdf.rs, microfacet.rs, material.rs, camera.rs,
+ * sampler.rs, color.rs and argent-spectra spec.rs / fresnel.rs / cie.rs).
+ * Only the tests, the smoke check and the benchmark's CPU-baseline leg load
+ * it, as the checker of the HIP path and as the timed CPU baseline; the
+ * product library never links it.
+ *
+ * It consumes the same ag_scene_desc as ag_scene_create.  The stratified
+ * sample dimensions use the same counter-based stream as the GPU path
+ * (hash of pixel seed, sample index, dimension) in place of the reference's
+ * Owen-scrambled Sobol (sobol_burley 0.5.0, not available here).
+ */
+#ifndef ARGENT_ORACLE_H
+#define ARGENT_ORACLE_H
+#include <stdint.h>
+
+#include "../include/argent.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_scene or_scene;
+
+/* traversal modes of or_intersect */
+enum {
+    OR_MODE_REFERENCE = 0, /* bvh.rs:159-213 exactly: SSE min/max slab, first hit wins ties */
+    OR_MODE_ROBUST = 1     /* same tree, boxes widened by 2^-21, ties to the lower triangle id */
+};
+
+or_scene* or_scene_create(const ag_scene_desc* d);
+void or_scene_destroy(or_scene* s);
+void or_scene_set_turntable(or_scene* s, float deg);
+int or_scene_set_material(or_scene* s, int32_t m, const ag_material* mat);
+int or_scene_stats(const or_scene* s, uint32_t* binary_nodes, uint32_t* wide_nodes);
+
+int or_intersect(const or_scene* s, const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
+                 const float* dz, const float* tmax, uint32_t n, int32_t mode, int32_t threads, float* t, float* u,
+                 float* v, uint32_t* prim);
+int or_occluded(const or_scene* s, const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
+                const float* dz, const float* tmax, uint32_t n, int32_t mode, int32_t threads, uint8_t* out);
+/* Brute-force closest hit over every triangle (bvh.rs:557-570), ties to the lower id. */
+int or_brute_intersect(const float* positions, const uint32_t* triangles, uint32_t n_tris, const float* ox,
+                       const float* oy, const float* oz, const float* dx, const float* dy, const float* dz,
+                       const float* tmax, uint32_t n, int32_t threads, float* t, float* u, float* v, uint32_t* prim);
+
+/* render_pass (render.rs:67-115) over rows [row0, row1) of a w x h film.
+ * sums: 9*w*h floats (colour, albedo, normal planes), updated in place;
+ * first = samples already in the film.  stats[0] = extension segments,
+ * stats[1] = shadow rays, stats[2] = samples. */
+int or_render_pass(const or_scene* s, const ag_camera* cam, uint32_t w, uint32_t h, float* sums, uint32_t first,
+                   const ag_path_settings* ps, uint32_t count, uint32_t seed, int32_t threads, uint32_t row0,
+                   uint32_t row1, uint64_t* stats);
+
+/* Known-answer helpers */
+int or_energy_table(float* out1024);
+float or_fresnel_dielectric(float cos_i, float eta);
+float or_fresnel_conductor1(float cos_i, float eta, float k);
+int or_sample_wavelengths(float u, float* lam4, float* pdf4);
+int or_camera_matrix(int32_t has_white, const float* white, float* m9);
+float or_rng_dim(uint32_t pix_seed, uint32_t si, uint32_t dim);
+int or_camera_ray(const ag_camera* cam, float sx, float sy, float lu, float lv, float* o3, float* d3);
+/* BSDF probe (material.rs:96-138 + bsdf.rs:53-239) for material m of the
+ * scene at wavelengths lam4: out = [eval(4), pdf, sample ok, wi(3), f(4),
+ * pdf, delta, transmit, term2] (18 floats).  u = [layer, u0, u1, u2]. */
+int or_bsdf_probe(const or_scene* s, int32_t m, const float* lam4, float mask, const float* u4, const float* wo3,
+                  const float* wi3, float* out18);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
