@@ -375,5 +375,6 @@ def render_views(ctx, scene, views, ps, materials=()):
         arr[i].film = v["film"].h
         arr[i].count = v["count"]
         arr[i].seed = v["seed"]
+        arr[i].scene = v["scene"].h if v.get("scene") is not None else None
     mats = (A.Material * max(1, len(materials)))(*materials) if materials else None
     check(A.lib().ag_render_views(ctx.h, scene.h, mats, len(materials), arr, len(views), C.byref(ps.c())))

@@ -79,7 +79,7 @@ class HitsSoA(C.Structure):
 
 class View(C.Structure):
     _fields_ = [("camera", Camera), ("turntable_deg", C.c_float), ("material0", C.c_int32),
-                ("film", C.c_void_p), ("count", C.c_uint32), ("seed", C.c_uint32)]
+                ("film", C.c_void_p), ("count", C.c_uint32), ("seed", C.c_uint32), ("scene", C.c_void_p)]
 
 
 class Tuning(C.Structure):

@@ -82,6 +82,7 @@ struct DScene {
 
 // One view of a scene: camera, turntable, slot-0 material, film buffers.
 struct DView {
+    const DScene* scene;  // the scene this view renders (views of different scenes share waves)
     float pos[3], fwd[3], right[3], up[3];
     float half_w, half_h, lens_radius, focus_dist, shift0, shift1;
     float turn_deg;      // 0 = identity (no rotation, as the reference skips it)

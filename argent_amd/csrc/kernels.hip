@@ -580,10 +580,9 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
 }
 
 // scene.rs:169-218 (closest hit over the piece, the panels and the floor)
-__global__ __launch_bounds__(TRAV_BLOCK) void k_extend(const DScene* __restrict__ Sp, const DView* __restrict__ views,
-                                                       DPaths P, DQueues Q, int depth, uint2* spill) {
+__global__ __launch_bounds__(TRAV_BLOCK) void k_extend(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
+                                                       uint2* spill) {
     TRAV_STACK_DECL
-    const DScene& S = *Sp;
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     const uint32_t n = cnt[0];
     const float4* ro = Q.ray_o[depth & 1];
@@ -596,6 +595,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) void k_extend(const DScene* __restrict_
             uint32_t path = __float_as_uint(a.w);
             f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
             const DView& V = views[P.view[path]];
+            const DScene& S = *V.scene;
             f3 oo = o, od = d;
             if (V.turn_deg != 0.0f) {
                 oo = rot_y(o, V.tsn, V.tcn);
@@ -639,7 +639,6 @@ __global__ __launch_bounds__(TRAV_BLOCK) void k_extend(const DScene* __restrict_
 }
 
 struct ShadeArgs {
-    const DScene* S;
     const DView* views;
     const DMat* vmats;
     uint32_t max_depth, rr_depth;
@@ -648,7 +647,6 @@ struct ShadeArgs {
 
 // integrator.rs:50-221, one bounce
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q, int depth) {
-    const DScene& S = *A.S;
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     uint32_t* cnt_next = Q.counters + (depth + 1) * CNT_STRIDE;
     __shared__ uint32_t pre[MAX_KEYS + 1];
@@ -683,6 +681,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
             uint64_t rng = P.rng[path];
             const uint32_t ps = P.pix_seed[path], si = P.si[path];
             const DView& V = A.views[P.view[path]];
+            const DScene& S = *V.scene;
             bool specular = flags & 1u, rough = (flags & 2u) != 0;
             int medium = (int)((flags >> 8) & 255u);
             const bool clamp = rough;
@@ -893,15 +892,15 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
 }
 
 // scene.rs:120-140 (sky_visibility) and scene.rs:220-243 (visible)
-__global__ __launch_bounds__(TRAV_BLOCK) void k_shadow(const DScene* __restrict__ Sp, const DView* __restrict__ views,
-                                                       DPaths P, DQueues Q, int depth, uint2* spill) {
+__global__ __launch_bounds__(TRAV_BLOCK) void k_shadow(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
+                                                       uint2* spill) {
     TRAV_STACK_DECL
-    const DScene& S = *Sp;
     const uint32_t n = Q.counters[depth * CNT_STRIDE + 1];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 a = Q.sh_o[i], b = Q.sh_d[i], x = Q.sh_x[i];
         uint32_t path = __float_as_uint(a.w);
         const DView& V = views[P.view[path]];
+        const DScene& S = *V.scene;
         f3 from{a.x, a.y, a.z};
         uint32_t kind = __float_as_uint(x.w);
         Spec c = from4(Q.sh_c[i]);
@@ -1009,6 +1008,19 @@ __global__ __launch_bounds__(256) void k_accumulate(const DSeg* __restrict__ seg
     }
 }
 
+// ray tallies of one wave into 64-bit totals: [extension rays, shadow rays]
+__global__ void k_tally(const uint32_t* __restrict__ counters, uint32_t depth_slots, unsigned long long* totals) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        unsigned long long e = 0, s = 0;
+        for (uint32_t d = 0; d < depth_slots; d++) {
+            e += counters[d * CNT_STRIDE];
+            s += counters[d * CNT_STRIDE + 1];
+        }
+        totals[0] += e;
+        totals[1] += s;
+    }
+}
+
 // film += pass accumulators; accumulators back to zero
 __global__ void k_film_commit(float* __restrict__ film, float* __restrict__ acc, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -1069,18 +1081,16 @@ void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t
                    DQueues Q) {
     hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, s, segs, nseg, npaths, views, P, Q);
 }
-void launch_extend(hipStream_t s, int grid, const DScene* S, const DView* views, DPaths P, DQueues Q, int depth,
-                   uint2* spill) {
-    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, S, views, P, Q, depth, spill);
+void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint2* spill) {
+    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, spill);
 }
-void launch_shade(hipStream_t s, int grid, const DScene* S, const DView* views, const DMat* vmats, uint32_t max_depth,
-                  uint32_t rr_depth, float clamp_indirect, DPaths P, DQueues Q, int depth) {
-    ShadeArgs A{S, views, vmats, max_depth, rr_depth, clamp_indirect};
+void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
+                  float clamp_indirect, DPaths P, DQueues Q, int depth) {
+    ShadeArgs A{views, vmats, max_depth, rr_depth, clamp_indirect};
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
 }
-void launch_shadow(hipStream_t s, int grid, const DScene* S, const DView* views, DPaths P, DQueues Q, int depth,
-                   uint2* spill) {
-    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(TRAV_BLOCK), 0, s, S, views, P, Q, depth, spill);
+void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint2* spill) {
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, spill);
 }
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
                        DPaths P) {
@@ -1088,6 +1098,9 @@ void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, cons
 }
 void launch_bsdf_probe(hipStream_t s, const DScene* S, ProbeIn in, float* out, uint32_t n) {
     hipLaunchKernelGGL(k_bsdf_probe, dim3(std::max(1u, std::min((n + 255) / 256, 1024u))), dim3(256), 0, s, S, in, out, n);
+}
+void launch_tally(hipStream_t s, const uint32_t* counters, uint32_t depth_slots, unsigned long long* totals) {
+    hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, s, counters, depth_slots, totals);
 }
 void launch_film_commit(hipStream_t s, float* film, float* acc, size_t n) {
     unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);

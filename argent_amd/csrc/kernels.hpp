@@ -32,14 +32,13 @@ void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const fl
                           uint32_t n, uint2* spill);
 void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,
                    DQueues Q);
-void launch_extend(hipStream_t s, int grid, const DScene* S, const DView* views, DPaths P, DQueues Q, int depth,
-                   uint2* spill);
-void launch_shade(hipStream_t s, int grid, const DScene* S, const DView* views, const DMat* vmats, uint32_t max_depth,
-                  uint32_t rr_depth, float clamp_indirect, DPaths P, DQueues Q, int depth);
-void launch_shadow(hipStream_t s, int grid, const DScene* S, const DView* views, DPaths P, DQueues Q, int depth,
-                   uint2* spill);
+void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint2* spill);
+void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
+                  float clamp_indirect, DPaths P, DQueues Q, int depth);
+void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint2* spill);
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
                        DPaths P);
+void launch_tally(hipStream_t s, const uint32_t* counters, uint32_t depth_slots, unsigned long long* totals);
 void launch_film_commit(hipStream_t s, float* film, float* acc, size_t n);
 
 }  // namespace ag

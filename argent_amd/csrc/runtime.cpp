@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -78,10 +79,22 @@ struct ag_ctx {
     DQueues Q{};
     int grid_trav = 0, grid_work = 0;
     size_t spill_threads = 0;
-    // timing
+    // timing: HIP event pairs around every stage launch on the context stream
     bool timing = false;
     ag_stats stats{};
     hipEvent_t ev[8] = {};
+    std::vector<hipEvent_t> evpool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_marks;  // (stage, index of the first event of the pair)
+    DevBuf tally_buf;
+    hipEvent_t next_event() {
+        if (ev_used == evpool.size()) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            evpool.push_back(e);
+        }
+        return evpool[ev_used++];
+    }
 };
 
 struct ag_scene {
@@ -137,6 +150,7 @@ extern "C" void ag_ctx_destroy(ag_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->evpool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -579,15 +593,18 @@ extern "C" int ag_film_read(const ag_film* f, int32_t which, float* out) {
 namespace {
 
 struct ViewJob {
+    const ag_scene* scene = nullptr;
     DView dv;
     ag_film* film;
     uint32_t count;
     uint32_t first;
 };
 
-void make_dview(DView& v, const ag_camera* cam, float turn_deg, int mat0, const ag_film* film, uint32_t seed,
-                const float* cam_m_in) {
+void make_dview(DView& v, const ag_scene* sc, const ag_camera* cam, float turn_deg, int mat0, const ag_film* film,
+                uint32_t seed) {
     std::memset(&v, 0, sizeof(v));
+    v.scene = sc->d_scene.as<DScene>();
+    const float* cam_m_in = sc->cam_m;
     for (int a = 0; a < 3; a++) {
         v.pos[a] = cam->pos[a];
         v.fwd[a] = cam->forward[a];
@@ -624,7 +641,9 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
     if (ps->max_depth > 64) return fail(AG_E_INVALID, "max_depth above 64");
     HIPCHK(hipSetDevice(c->device));
     uint32_t cap = c->tuning.paths_per_wave;
-    int rc = ensure_wave(c, cap, s->n_keys, ps->max_depth);
+    int n_keys = s->n_keys;
+    for (auto& j : jobs) n_keys = std::max(n_keys, j.scene->n_keys);
+    int rc = ensure_wave(c, cap, n_keys, ps->max_depth);
     if (rc) return rc;
     hipStream_t st = c->stream;
     // upload views
@@ -673,45 +692,67 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
     }
     if (!cur.empty()) waves.push_back(cur);
     c->stats = ag_stats{};
+    c->ev_used = 0;
+    c->ev_marks.clear();
+    if ((rc = c->tally_buf.ensure(2 * sizeof(unsigned long long)))) return rc;
+    HIPCHK(hipMemsetAsync(c->tally_buf.p, 0, 2 * sizeof(unsigned long long), st));
+    auto timed = [&](int stage, auto&& launch) {
+        if (!c->timing) {
+            launch();
+            return;
+        }
+        size_t i0 = c->ev_used;
+        hipEvent_t a = c->next_event();
+        hipEvent_t b = c->next_event();
+        (void)hipEventRecord(a, st);
+        launch();
+        (void)hipEventRecord(b, st);
+        c->ev_marks.push_back({stage, i0});
+    };
     auto t0 = std::chrono::steady_clock::now();
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[0], st));
-    const DScene* S = s->d_scene.as<DScene>();
+    uint32_t launches = 0;
     for (auto& w : waves) {
         uint32_t npaths = 0;
         for (auto& sg : w) npaths = std::max(npaths, sg.begin + sg.npix * sg.ns);
         HIPCHK(hipMemcpyAsync(c->seg_buf.p, w.data(), w.size() * sizeof(DSeg), hipMemcpyHostToDevice, st));
         launch_zero_u32(st, c->Q.counters, (ps->max_depth + 2) * CNT_STRIDE);
-        launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), npaths, views, c->P, c->Q);
+        timed(3, [&] { launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), npaths, views, c->P, c->Q); });
         for (uint32_t d = 0; d < ps->max_depth; d++) {
-            launch_extend(st, c->grid_trav, S, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>());
-            launch_shade(st, c->grid_work, S, views, vmats_dev, ps->max_depth, ps->rr_depth, ps->clamp_indirect, c->P,
-                         c->Q, (int)d);
-            launch_shadow(st, c->grid_trav, S, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>());
+            timed(0, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>()); });
+            timed(1, [&] {
+                launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, ps->rr_depth, ps->clamp_indirect, c->P,
+                             c->Q, (int)d);
+            });
+            timed(2, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>()); });
+            launches++;
         }
-        launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P);
+        timed(4, [&] { launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P); });
+        launch_tally(st, c->Q.counters, ps->max_depth, c->tally_buf.as<unsigned long long>());
         c->stats.samples += npaths;
-        if (c->timing) {
-            // ray counts of this wave
-            std::vector<uint32_t> cnt((ps->max_depth + 2) * CNT_STRIDE);
-            HIPCHK(hipMemcpyAsync(cnt.data(), c->Q.counters, cnt.size() * 4, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            for (uint32_t d = 0; d < ps->max_depth; d++) {
-                c->stats.extension_rays += cnt[d * CNT_STRIDE];
-                c->stats.shadow_rays += cnt[d * CNT_STRIDE + 1];
-            }
-        }
     }
     for (auto& j : jobs) {
         size_t n = (size_t)j.film->w * j.film->h * 9;
         launch_film_commit(st, j.film->sums.as<float>(), j.film->acc.as<float>(), n);
     }
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[1], st));
+    unsigned long long tally[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(tally, c->tally_buf.p, sizeof(tally), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipGetLastError());
     auto t1 = std::chrono::steady_clock::now();
     c->stats.ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    c->stats.extend_launches = (uint32_t)(waves.size() * ps->max_depth);
-    c->stats.shadow_launches = c->stats.extend_launches;
+    c->stats.extension_rays = tally[0];
+    c->stats.shadow_rays = tally[1];
+    c->stats.extend_launches = launches;
+    c->stats.shadow_launches = launches;
+    for (auto& m : c->ev_marks) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->evpool[m.second], c->evpool[m.second + 1]);
+        double* dst = m.first == 0 ? &c->stats.ms_extend
+                    : m.first == 1 ? &c->stats.ms_shade
+                    : m.first == 2 ? &c->stats.ms_shadow
+                    : m.first == 3 ? &c->stats.ms_raygen : &c->stats.ms_accum;
+        *dst += ms;
+    }
     for (auto& j : jobs) {
         j.film->samples += j.count;
         std::memcpy(j.film->cam_m, j.dv.cam_m, sizeof(j.dv.cam_m));
@@ -726,21 +767,17 @@ extern "C" int ag_render_pass(ag_ctx* ctx, const ag_scene* scene, const ag_camer
     if (count == 0) return AG_OK;
     if ((uint64_t)film->samples + count > 0xFFFFFFFFull) return fail(AG_E_INVALID, "sample index overflow");
     ViewJob j;
+    j.scene = scene;
     j.film = film;
     j.count = count;
     j.first = film->samples;
-    make_dview(j.dv, cam, scene->turntable_deg, -1, film, seed, scene->cam_m);
+    make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed);
     std::vector<ViewJob> jobs{j};
     return run_views(ctx, scene, jobs, ps, nullptr);
 }
 
-extern "C" int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_material* materials, int32_t n_materials,
-                               const ag_view* views, int32_t n_views, const ag_path_settings* ps) {
-    if (!ctx || !scene || !views || !ps || n_views <= 0) return fail(AG_E_INVALID, "bad arguments");
-    if (n_materials < 0 || (n_materials > 0 && !materials)) return fail(AG_E_INVALID, "bad materials");
-    HIPCHK(hipSetDevice(ctx->device));
+static int upload_view_materials(ag_ctx* ctx, const ag_material* materials, int32_t n_materials) {
     int rc;
-    // view materials: tables in vtab_buf, descriptors in vmat_buf
     size_t stride = tables_per_material();
     std::vector<float> tab(std::max<size_t>(1, n_materials) * stride, 0.0f);
     if ((rc = ctx->vtab_buf.ensure(tab.size() * 4))) return rc;
@@ -754,6 +791,16 @@ extern "C" int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_mate
     HIPCHK(hipMemcpyAsync(ctx->vtab_buf.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     if ((rc = ctx->vmat_buf.ensure(dm.size() * sizeof(DMat)))) return rc;
     HIPCHK(hipMemcpyAsync(ctx->vmat_buf.p, dm.data(), dm.size() * sizeof(DMat), hipMemcpyHostToDevice, ctx->stream));
+    return AG_OK;
+}
+
+extern "C" int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_material* materials, int32_t n_materials,
+                               const ag_view* views, int32_t n_views, const ag_path_settings* ps) {
+    if (!ctx || !scene || !views || !ps || n_views <= 0) return fail(AG_E_INVALID, "bad arguments");
+    if (n_materials < 0 || (n_materials > 0 && !materials)) return fail(AG_E_INVALID, "bad materials");
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = upload_view_materials(ctx, materials, n_materials);
+    if (rc) return rc;
     std::vector<ViewJob> jobs;
     for (int i = 0; i < n_views; i++) {
         const ag_view& v = views[i];
@@ -762,15 +809,175 @@ extern "C" int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_mate
         for (int k = 0; k < i; k++)
             if (views[k].film == v.film) return fail(AG_E_INVALID, "two views share one film");
         if (v.count == 0) continue;
+        if ((uint64_t)v.film->samples + v.count > 0xFFFFFFFFull) return fail(AG_E_INVALID, "sample index overflow");
         ViewJob j;
         j.film = v.film;
         j.count = v.count;
         j.first = v.film->samples;
-        make_dview(j.dv, &v.camera, v.turntable_deg, v.material0, v.film, v.seed, scene->cam_m);
+        const ag_scene* vs = v.scene ? v.scene : scene;
+        if (vs->ctx != ctx) return fail(AG_E_INVALID, "view scene from another context");
+        make_dview(j.dv, vs, &v.camera, v.turntable_deg, v.material0, v.film, v.seed);
+        j.scene = vs;
         jobs.push_back(j);
     }
     if (jobs.empty()) return AG_OK;
     return run_views(ctx, scene, jobs, ps, ctx->vmat_buf.as<DMat>());
+}
+
+// ------------------------------------------------------------------ server
+struct ag_server {
+    ag_ctx* ctx = nullptr;
+    size_t budget = 0, used = 0;
+    uint64_t step_samples = 1u << 24;
+    ag_path_settings ps{};
+    std::chrono::steady_clock::time_point t0;
+    struct Req {
+        ag_request r;
+        ag_request_status st;
+        ag_film* film = nullptr;
+        uint32_t next_chunk = 0;
+        size_t cost = 0;
+    };
+    std::map<uint64_t, Req> reqs;
+    std::vector<uint64_t> queue;   // FIFO of queued ids
+    std::vector<uint64_t> active;  // admission order
+    uint64_t next_id = 1;
+    double now() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+extern "C" int ag_server_create(ag_ctx* ctx, size_t budget, uint64_t step_samples, const ag_path_settings* ps,
+                                ag_server** out) {
+    if (!ctx || !ps || !out || step_samples == 0) return fail(AG_E_INVALID, "bad arguments");
+    auto* s = new ag_server();
+    s->ctx = ctx;
+    s->budget = budget;
+    s->step_samples = step_samples;
+    s->ps = *ps;
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return AG_OK;
+}
+
+extern "C" void ag_server_destroy(ag_server* s) {
+    if (!s) return;
+    for (auto& kv : s->reqs)
+        if (kv.second.film) ag_film_destroy(kv.second.film);
+    delete s;
+}
+
+extern "C" int ag_server_submit(ag_server* s, const ag_request* r, uint64_t* id) {
+    if (!s || !r || !id) return fail(AG_E_INVALID, "null argument");
+    if (!r->scene || r->scene->ctx != s->ctx) return fail(AG_E_INVALID, "request scene missing or from another context");
+    if (r->width == 0 || r->height == 0 || r->spp == 0) return fail(AG_E_INVALID, "empty request");
+    if (r->has_material0 && (r->material0.kind < AG_MAT_METAL || r->material0.kind > AG_MAT_OPAL ||
+                             r->material0.kind == AG_MAT_GEM))
+        return fail(AG_E_INVALID, "bad material override");
+    ag_server::Req q;
+    q.r = *r;
+    if (q.r.preview_spp == 0) q.r.preview_spp = 2;
+    q.st.state = AG_REQ_QUEUED;
+    q.st.samples = 0;
+    q.st.t_submit = s->now();
+    q.st.t_admit = q.st.t_first_preview = q.st.t_done = -1.0;
+    q.cost = (size_t)r->width * r->height * 9 * sizeof(float) * 2;
+    q.next_chunk = std::min(q.r.preview_spp, q.r.spp);
+    uint64_t k = s->next_id++;
+    if (q.cost > s->budget) q.st.state = AG_REQ_REJECTED;
+    s->reqs[k] = q;
+    if (q.st.state == AG_REQ_QUEUED) s->queue.push_back(k);
+    *id = k;
+    return AG_OK;
+}
+
+extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queued) {
+    if (!s) return fail(AG_E_INVALID, "null server");
+    ag_ctx* ctx = s->ctx;
+    // admit FIFO while the films fit
+    while (!s->queue.empty()) {
+        auto& q = s->reqs[s->queue.front()];
+        if (s->used + q.cost > s->budget) break;
+        int rc = ag_film_create(ctx, q.r.width, q.r.height, &q.film);
+        if (rc) return rc;
+        s->used += q.cost;
+        q.st.state = AG_REQ_ACTIVE;
+        q.st.t_admit = s->now();
+        s->active.push_back(s->queue.front());
+        s->queue.erase(s->queue.begin());
+    }
+    if (!s->active.empty()) {
+        std::vector<ag_material> mats;
+        std::vector<ViewJob> jobs;
+        for (uint64_t k : s->active) {
+            auto& q = s->reqs[k];
+            ViewJob j;
+            j.film = q.film;
+            j.count = q.next_chunk;
+            j.first = q.film->samples;
+            int m0 = -1;
+            if (q.r.has_material0) {
+                m0 = (int)mats.size();
+                mats.push_back(q.r.material0);
+            }
+            make_dview(j.dv, q.r.scene, &q.r.camera, q.r.turntable_deg, m0, q.film, q.r.seed);
+            j.scene = q.r.scene;
+            jobs.push_back(j);
+        }
+        int rc = upload_view_materials(ctx, mats.data(), (int32_t)mats.size());
+        if (rc) return rc;
+        rc = run_views(ctx, jobs[0].scene, jobs, &s->ps, ctx->vmat_buf.as<DMat>());
+        if (rc) return rc;
+        double t = s->now();
+        std::vector<uint64_t> still;
+        for (uint64_t k : s->active) {
+            auto& q = s->reqs[k];
+            q.st.samples = q.film->samples;
+            if (q.st.t_first_preview < 0.0) q.st.t_first_preview = t;
+            uint32_t left = q.r.spp - q.st.samples;
+            if (left == 0) {
+                q.st.state = AG_REQ_DONE;
+                q.st.t_done = t;
+                continue;
+            }
+            uint64_t px = (uint64_t)q.r.width * q.r.height;
+            uint32_t cap = (uint32_t)std::max<uint64_t>(1, s->step_samples / px);
+            q.next_chunk = std::min(left, std::max(q.next_chunk, std::min(cap, q.next_chunk * 2)));
+            still.push_back(k);
+        }
+        s->active = still;
+    }
+    if (n_active) *n_active = (uint32_t)s->active.size();
+    if (n_queued) *n_queued = (uint32_t)s->queue.size();
+    return AG_OK;
+}
+
+extern "C" int ag_server_status(const ag_server* s, uint64_t id, ag_request_status* out) {
+    if (!s || !out) return fail(AG_E_INVALID, "null argument");
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(AG_E_NOTFOUND, "no such request");
+    *out = it->second.st;
+    return AG_OK;
+}
+
+extern "C" int ag_server_read(const ag_server* s, uint64_t id, int32_t which, float* out) {
+    if (!s || !out) return fail(AG_E_INVALID, "null argument");
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(AG_E_NOTFOUND, "no such request");
+    if (!it->second.film) return fail(AG_E_STATE, "request not admitted yet");
+    return ag_film_read(it->second.film, which, out);
+}
+
+extern "C" int ag_server_release(ag_server* s, uint64_t id) {
+    if (!s) return fail(AG_E_INVALID, "null server");
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(AG_E_NOTFOUND, "no such request");
+    if (it->second.st.state == AG_REQ_ACTIVE) return fail(AG_E_STATE, "request still rendering");
+    if (it->second.film) {
+        ag_film_destroy(it->second.film);
+        s->used -= it->second.cost;
+    }
+    s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), id), s->queue.end());
+    s->reqs.erase(it);
+    return AG_OK;
 }
 
 // ------------------------------------------------------------------ ray batches

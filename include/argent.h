@@ -298,9 +298,50 @@ typedef struct ag_view {
     ag_film* film;
     uint32_t count;
     uint32_t seed;
+    const ag_scene* scene;  /* NULL = the call's scene; views of different scenes share waves */
 } ag_view;
 int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_material* materials, int32_t n_materials,
                     const ag_view* views, int32_t n_views, const ag_path_settings* ps);
+
+/* ------------------------------------------------------------ serving
+ * A continuous-batching render server: the role of the studio's live worker
+ * and job queue (live.rs:637-823, queue.rs:475-691) for many clients at once.
+ * Requests are admitted FIFO while their films fit the HBM budget; every
+ * ag_server_step renders the next sample chunk of every active request in one
+ * shared wavefront schedule (requests on different scenes share waves), so a
+ * request that arrives while others run joins at the next step.  A request's
+ * chunk sizes depend only on its own spp and the server's settings (first
+ * chunk preview_spp, then doubling up to a cap of step_samples / pixels), so
+ * its film is bit-identical whether it is served alone or in a mix.  Films
+ * stay resident until released; ag_server_read gives the progressive image. */
+enum { AG_REQ_QUEUED = 0, AG_REQ_ACTIVE = 1, AG_REQ_DONE = 2, AG_REQ_REJECTED = 3 };
+typedef struct ag_request {
+    const ag_scene* scene;
+    ag_camera camera;
+    float turntable_deg;
+    int32_t has_material0;   /* 1: material0 replaces the scene's slot 0 (a metal variant) */
+    ag_material material0;
+    uint32_t width, height, spp, seed;
+    uint32_t preview_spp;    /* samples in the first chunk; 0 = 2 (the live view's first look, live.rs:745) */
+} ag_request;
+typedef struct ag_request_status {
+    int32_t state;
+    uint32_t samples;
+    /* milliseconds since the server was created; -1 until reached */
+    double t_submit, t_admit, t_first_preview, t_done;
+} ag_request_status;
+typedef struct ag_server ag_server;
+int ag_server_create(ag_ctx* ctx, size_t hbm_budget_bytes, uint64_t step_samples, const ag_path_settings* ps,
+                     ag_server** out);
+void ag_server_destroy(ag_server* s);
+int ag_server_submit(ag_server* s, const ag_request* req, uint64_t* id);
+/* One scheduling round: admit, render one chunk of every active request,
+ * retire finished ones.  Returns the counts left active and queued. */
+int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queued);
+int ag_server_status(const ag_server* s, uint64_t id, ag_request_status* out);
+/* Progressive image of a request (as ag_film_read), host memory w*h*3. */
+int ag_server_read(const ag_server* s, uint64_t id, int32_t which, float* out);
+int ag_server_release(ag_server* s, uint64_t id);
 
 /* ------------------------------------------------------------ tuning
  * Launch shape of the wavefront kernels (the autotuner's knobs). */
