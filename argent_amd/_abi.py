@@ -82,6 +82,20 @@ class View(C.Structure):
                 ("film", C.c_void_p), ("count", C.c_uint32), ("seed", C.c_uint32), ("scene", C.c_void_p)]
 
 
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("turntable_deg", C.c_float),
+                ("has_material0", C.c_int32), ("material0", Material), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("spp", C.c_uint32), ("seed", C.c_uint32), ("preview_spp", C.c_uint32)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("samples", C.c_uint32), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_first_preview", C.c_double), ("t_done", C.c_double)]
+
+
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_REJECTED = range(4)
+
+
 class Tuning(C.Structure):
     _fields_ = [("paths_per_wave", C.c_uint32), ("block_traverse", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("sort_by_material", C.c_uint32)]
@@ -151,6 +165,14 @@ _SIGS = [
     ("ag_bsdf_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, i32p, f32p, f32p, f32p, f32p, f32p, f32p]),
     ("ag_render_views", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Material), C.c_int32, C.POINTER(View),
                                   C.c_int32, C.POINTER(PathSettingsC)]),
+    ("ag_server_create", C.c_int, [C.c_void_p, C.c_size_t, C.c_uint64, C.POINTER(PathSettingsC),
+                                   C.POINTER(C.c_void_p)]),
+    ("ag_server_destroy", None, [C.c_void_p]),
+    ("ag_server_submit", C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    ("ag_server_step", C.c_int, [C.c_void_p, u32p, u32p]),
+    ("ag_server_status", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStatus)]),
+    ("ag_server_read", C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, f32p]),
+    ("ag_server_release", C.c_int, [C.c_void_p, C.c_uint64]),
     ("ag_ctx_get_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("ag_ctx_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("ag_ctx_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
