@@ -378,3 +378,53 @@ def render_views(ctx, scene, views, ps, materials=()):
         arr[i].scene = v["scene"].h if v.get("scene") is not None else None
     mats = (A.Material * max(1, len(materials)))(*materials) if materials else None
     check(A.lib().ag_render_views(ctx.h, scene.h, mats, len(materials), arr, len(views), C.byref(ps.c())))
+
+
+class DeviceRays:
+    """A ray batch resident in HBM (structure of arrays) with room for hits:
+    the input of Scene.intersect_device / occluded_device."""
+
+    def __init__(self, ctx, o, d, tmax=None):
+        o, d = _f32(o, (-1, 3)), _f32(d, (-1, 3))
+        self.ctx, self.n = ctx, len(o)
+        cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + [np.ascontiguousarray(d[:, k]) for k in range(3)]
+        if tmax is not None:
+            cols.append(_f32(tmax))
+        self.bufs = []
+        for c in cols:
+            p = C.c_void_p()
+            check(A.lib().ag_device_alloc(ctx.h, c.nbytes, C.byref(p)))
+            check(A.lib().ag_memcpy_to_device(ctx.h, p, c.ctypes.data_as(C.c_void_p), c.nbytes))
+            self.bufs.append(p)
+        self.out = []
+        for _ in range(4):
+            p = C.c_void_p()
+            check(A.lib().ag_device_alloc(ctx.h, 4 * self.n, C.byref(p)))
+            self.out.append(p)
+        f = lambda p: C.cast(p, A.f32p)
+        self.rays = A.RaysSoA(*[f(p) for p in self.bufs[:6]], f(self.bufs[6]) if tmax is not None else None)
+        self.hits = A.HitsSoA(f(self.out[0]), f(self.out[1]), f(self.out[2]), C.cast(self.out[3], A.u32p))
+
+    def hits_host(self):
+        res = []
+        for k, dt in enumerate((np.float32, np.float32, np.float32, np.uint32)):
+            a = np.empty(self.n, dt)
+            check(A.lib().ag_memcpy_to_host(self.ctx.h, a.ctypes.data_as(C.c_void_p), self.out[k], a.nbytes))
+            res.append(a)
+        return tuple(res)
+
+    def close(self):
+        for p in self.bufs + self.out:
+            A.lib().ag_device_free(self.ctx.h, p)
+        self.bufs, self.out = [], []
+
+
+def intersect_device(ctx, scene, rays):
+    """Closest hit for a resident batch; returns the kernel time in ms (HIP events)."""
+    check(A.lib().ag_intersect_device(ctx.h, scene.h, C.byref(rays.rays), rays.n, C.byref(rays.hits)))
+    return A.lib().ag_last_query_ms(ctx.h)
+
+
+def occluded_device(ctx, scene, rays):
+    check(A.lib().ag_occluded_device(ctx.h, scene.h, C.byref(rays.rays), rays.n, C.cast(rays.out[3], A.u8p)))
+    return A.lib().ag_last_query_ms(ctx.h)

@@ -173,6 +173,11 @@ _SIGS = [
     ("ag_server_status", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStatus)]),
     ("ag_server_read", C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, f32p]),
     ("ag_server_release", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("ag_device_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("ag_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("ag_memcpy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("ag_memcpy_to_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("ag_last_query_ms", C.c_double, [C.c_void_p]),
     ("ag_ctx_get_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("ag_ctx_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("ag_ctx_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),

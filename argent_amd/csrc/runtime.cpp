@@ -87,6 +87,7 @@ struct ag_ctx {
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> ev_marks;  // (stage, index of the first event of the pair)
     DevBuf tally_buf;
+    double last_query_ms = 0.0;
     hipEvent_t next_event() {
         if (ev_used == evpool.size()) {
             hipEvent_t e;
@@ -1018,7 +1019,9 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
             float* b = outb.as<float>();
             h = HitsSoA{b, b + n, b + 2 * (size_t)n, reinterpret_cast<uint32_t*>(b + 3 * (size_t)n)};
         }
+        HIPCHK(hipEventRecord(ctx->ev[2], st));
         launch_rays_intersect(st, grid, nodes, tris, r, h, n, ctx->spill_buf.as<uint2>());
+        HIPCHK(hipEventRecord(ctx->ev[3], st));
         HIPCHK(hipGetLastError());
         if (!device_ptrs) {
             HIPCHK(hipMemcpyAsync(hits->t, h.t, (size_t)n * 4, hipMemcpyDeviceToHost, st));
@@ -1032,13 +1035,43 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
             if ((rc = outb.ensure(n))) return rc;
             o = outb.as<uint8_t>();
         }
+        HIPCHK(hipEventRecord(ctx->ev[2], st));
         launch_rays_occluded(st, grid, nodes, tris, r, o, n, ctx->spill_buf.as<uint2>());
+        HIPCHK(hipEventRecord(ctx->ev[3], st));
         HIPCHK(hipGetLastError());
         if (!device_ptrs) HIPCHK(hipMemcpyAsync(occ, o, n, hipMemcpyDeviceToHost, st));
     }
-    if (!device_ptrs) HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->last_query_ms = ms;
     return AG_OK;
 }
+
+extern "C" int ag_device_alloc(ag_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(AG_E_INVALID, "null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (hipMalloc(out, std::max<size_t>(bytes, 1)) != hipSuccess) return fail(AG_E_NOMEM, "hipMalloc failed");
+    return AG_OK;
+}
+extern "C" int ag_device_free(ag_ctx* ctx, void* p) {
+    if (!ctx) return fail(AG_E_INVALID, "null context");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(p));
+    return AG_OK;
+}
+extern "C" int ag_memcpy_to_device(ag_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(AG_E_INVALID, "null context");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return AG_OK;
+}
+extern "C" int ag_memcpy_to_host(ag_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(AG_E_INVALID, "null context");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return AG_OK;
+}
+extern "C" double ag_last_query_ms(const ag_ctx* ctx) { return ctx ? ctx->last_query_ms : 0.0; }
 
 extern "C" int ag_intersect(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, ag_hits_soa* out) {
     if (!out) return fail(AG_E_INVALID, "null output");

@@ -6,7 +6,18 @@ on a bounded sample of the same image.
 One step = one full 256-spp render_pass of the still.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+HBM traffic of the traversal kernel: run the bench twice under
+`rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` with `--copy-calibration
+MiB` (a plain streaming copy of known size in the same process, whose counter
+readings scale both counters to bytes), then
+
+    python bench.py --reduce-traffic FETCH_DIR WRITE_DIR
+
+writes profiles/k_extend_signet_traffic.json, which later runs report as
+roofline.traffic.
 """
+import glob
 import argparse
 import json
 import os
@@ -81,6 +92,66 @@ def _cpu_model():
     return "unknown"
 
 
+EXTEND_KERNEL = "ag::k_extend("
+COPY_KERNEL = "neg_kernel"
+
+
+def copy_calibration(mib):
+    """A plain streaming kernel of known traffic (read mib MiB, write mib MiB):
+    torch's elementwise negation of a float32 buffer, run twice."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    assert float(b[0]) == -1.0
+    del a, b
+    torch.cuda.empty_cache()
+
+
+def _pmc_rows(d, counter):
+    import csv
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] == counter:
+                    rows.append((r["Kernel_Name"], float(r["Counter_Value"]), int(r.get("Grid_Size", 0) or 0)))
+    return rows
+
+
+def reduce_traffic(fetch_dir, write_dir, mib):
+    """HBM bytes per k_extend dispatch from two counter runs.  FETCH_SIZE and
+    WRITE_SIZE read KiB per dispatch; each is scaled by known/counted bytes of
+    the calibration copy in the same run."""
+    out = {}
+    for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _pmc_rows(d, name)
+        cal = [v for k, v, _ in rows if COPY_KERNEL in k]
+        ext = [v for k, v, _ in rows if k.startswith(EXTEND_KERNEL)]
+        if not cal or not ext:
+            raise SystemExit(f"{name}: calibration dispatches {len(cal)}, k_extend dispatches {len(ext)}")
+        known_kib = mib * 1024.0
+        scale = known_kib / (sum(cal) / len(cal))
+        out[name] = {"calibration_kib_read": round(sum(cal) / len(cal), 1), "calibration_kib_known": known_kib,
+                     "scale": round(scale, 4), "k_extend_dispatches": len(ext),
+                     "k_extend_kib_raw_mean": round(sum(ext) / len(ext), 1),
+                     "k_extend_bytes_mean": round(sum(ext) / len(ext) * scale * 1024.0)}
+    total = out["FETCH_SIZE"]["k_extend_bytes_mean"] + out["WRITE_SIZE"]["k_extend_bytes_mean"]
+    res = {"kernel": "k_extend", "workload": "signet still 1080x1920 16spp depth 12 (same wave size as 256 spp)",
+           "bytes_per_dispatch": int(total), "fetch": out["FETCH_SIZE"], "write": out["WRITE_SIZE"]}
+    path = os.path.join(HERE, "profiles", "k_extend_signet_traffic.json")
+    with open(path, "w") as fh:
+        json.dump(res, fh, indent=1)
+        fh.write("\n")
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +160,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="run a streaming copy of this size first (counter calibration)")
+    ap.add_argument("--reduce-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="turn two rocprofv3 --pmc runs into profiles/k_extend_signet_traffic.json")
     args = ap.parse_args()
+    if args.reduce_traffic:
+        reduce_traffic(*args.reduce_traffic, args.copy_calibration or 1024)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "
-              f"supported (sharding is by image tiles / sample ranges / requests, see DESIGN.md)", file=sys.stderr)
+              f"supported; a still shards by image tiles or sample ranges, a server by requests", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
     import argent_amd as ag
     from argent_amd import workloads

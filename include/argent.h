@@ -286,6 +286,16 @@ int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_d
 int ag_bsdf_probe(ag_ctx* ctx, const ag_scene* s, uint32_t n, const int32_t* mat, const float* lam4,
                   const float* mask, const float* u4, const float* wo3, const float* wi3, float* out18);
 
+/* Device buffers for callers that keep ray batches resident in HBM (used with
+ * the *_device queries).  Copies are synchronous on the context stream. */
+int ag_device_alloc(ag_ctx* ctx, size_t bytes, void** out);
+int ag_device_free(ag_ctx* ctx, void* p);
+int ag_memcpy_to_device(ag_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ag_memcpy_to_host(ag_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Elapsed device time of the last *_device query on the context stream (HIP
+ * events around the kernel), milliseconds. */
+double ag_last_query_ms(const ag_ctx* ctx);
+
 /* ------------------------------------------------------------ batching
  * Many uneven views of one scene in one wavefront schedule: the turntable
  * loop of jobs.rs:689-710 and metal variants, with each view's own camera,

@@ -153,28 +153,26 @@ def test_bsdf_probe_matches_oracle(ctx):
     g = sc.bsdf_probe(m, lam, mask, u, wo, wi)
     worst = 0.0
 
-    worst_sampled = 0.0
-
     def close(a, b, what, tol=1e-5):
-        nonlocal worst, worst_sampled
+        nonlocal worst
         if not np.isfinite(b):
             assert not np.isfinite(a), what
             return
         rel = abs(float(a) - float(b)) / max(abs(float(b)), 1e-6)
-        if tol == 1e-5:
-            worst = max(worst, rel)
-        else:
-            worst_sampled = max(worst_sampled, rel)
+        worst = max(worst, rel)
         assert rel <= tol, (what, float(a), float(b))
 
-    # Sampling draws the half-vector through sinf/cosf (ocml on the GPU, glibc
-    # here, both within 2 ulp); in the VNDF sample sqrt(1 - p1^2 - p2^2)
-    # cancels near the disk rim, and bsdf.rs takes the sampled pdf from that
-    # half-vector.  With a narrow lobe (alpha ~ 5e-3, D ~ 1e4) the ulp
-    # difference shows as up to ~1 % in the sampled f/pdf, while the
-    # directions agree to 1e-5; sampled values are held to 1e-2 relative.
-    SAMPLED_TOL = 1e-2
-
+    # A sample's f is eval(wo, wi) and, except for the rough conductor, its
+    # pdf is pdf(wo, wi) (bsdf.rs:149-150, 166-170, 176-181, 228-233): both
+    # are compared with the oracle's eval at the GPU's own sampled direction.
+    # The rough conductor takes its pdf from the sampled half-vector h
+    # (bsdf.rs:148), not from wi = reflect(wo, h); for a narrow lobe
+    # (alpha ~ 5e-3) that value moves by up to ~3 % under ulp changes of h
+    # which leave wi unchanged, in the oracle as on the GPU (sinf/cosf in
+    # sample_h are ocml there, glibc here).  There each side's sampled pdf
+    # must lie within 5 % of its own pdf(wo, wi), and those agree to 1e-5.
+    g2 = sc.bsdf_probe(m, lam, mask, u, wo, np.ascontiguousarray(g[:, 6:9]))
+    worst_h = 0.0
     for i in range(n):
         name = MATS[m[i]][0]
         r = osc.bsdf_probe(int(m[i]), lam[i], mask[i], u[i], wo[i], wi[i])
@@ -189,10 +187,21 @@ def test_bsdf_probe_matches_oracle(ctx):
         if r[14]:  # delta lobes: f and pdf follow from the direction in closed form
             for k in range(9, 14):
                 close(g[i, k], r[k], (name, i, k))
-        else:  # the sample's f and pdf equal the oracle's eval/pdf in the sampled direction
-            for k in range(9, 14):
-                close(g[i, k], r[k], (name, i, k), SAMPLED_TOL)
-    print("worst relative BSDF evaluation difference", worst, "sampled", worst_sampled)
+            continue
+        e_g = osc.bsdf_probe(int(m[i]), lam[i], mask[i], u[i], wo[i], g[i, 6:9])
+        for k in range(5):  # GPU eval/pdf at its sampled direction vs the oracle's there
+            close(g2[i, k], e_g[k], (name, i, "at sampled", k))
+        for k in range(4):  # the sample's f is that eval
+            close(g[i, 9 + k], e_g[k], (name, i, "f", k))
+        if name.startswith("metal"):
+            e_r = osc.bsdf_probe(int(m[i]), lam[i], mask[i], u[i], wo[i], r[6:9])
+            for smp, ev in ((g[i, 13], g2[i, 4]), (r[13], e_r[4])):
+                dh = abs(float(smp) - float(ev)) / float(ev)
+                worst_h = max(worst_h, dh)
+                assert dh <= 5e-2, (name, i, float(smp), float(ev))
+        else:
+            close(g[i, 13], e_g[4], (name, i, "pdf"))
+    print("worst relative BSDF difference", worst, "conductor h round trip", worst_h)
     sc.close()
 
 
