@@ -1080,7 +1080,10 @@ struct Bvh {
     }
 
     // bvh.rs:280-299, with SSE minps/maxps semantics; robust mode widens the
-    // interval by 2^-21 on both ends
+    // interval by 2^-16 relative on both ends.  f32 Moller-Trumbore can report
+    // a hit a little outside its triangle for grazing rays, so the margin is
+    // wider than the box test's own rounding needs: it keeps such hits
+    // visible to the traversal whatever the tree.
     static uint32_t slab4(const Node4& n, const Ray& r, V3 inv, float t_max, bool robust, float tmin_out[4]) {
         auto mn = [](float a, float b) { return a < b ? a : b; };
         auto mx = [](float a, float b) { return a > b ? a : b; };
@@ -1093,8 +1096,8 @@ struct Bvh {
             float tmax;
             if (robust) {
                 tmax = mn(mn(mx(tx0, tx1), mx(ty0, ty1)), mx(tz0, tz1));
-                tmin *= 1.0f - 4.76837158203125e-7f;
-                tmax = mn(tmax * (1.0f + 4.76837158203125e-7f), t_max);
+                tmin *= 1.0f - 1.52587890625e-5f;
+                tmax = mn(tmax * (1.0f + 1.52587890625e-5f), t_max);
             } else {
                 tmax = mn(mn(mn(mx(tx0, tx1), mx(ty0, ty1)), mx(tz0, tz1)), t_max);
             }

@@ -52,6 +52,42 @@ def test_bvh_random_soup_vs_brute_force():
     assert np.array_equal(rt[h], bt[h]) and np.array_equal(ru[h], bu[h]) and np.array_equal(rv[h], bv[h])
 
 
+# A ray of the 2^24-ray signet batch (sphere_rays seed 3, index 476550) that
+# grazes two triangles of the signet (seed 2, triangles 95285 and 95286).
+GRAZE_O = [25.70743179321289, 10.960657119750977, -12.40622615814209]
+GRAZE_D = [-0.8504688143730164, -0.07241949439048767, 0.5210164785385132]
+GRAZE_V = [[4.1826887130737305, 9.131837844848633, 0.7744090557098389],
+           [4.118640899658203, 9.139716148376465, 0.7998011112213135],
+           [4.174643516540527, 9.114273071289062, 0.7998011112213135],
+           [4.110450267791748, 9.121540069580078, 0.8247113227844238]]
+
+
+def test_grazing_hit_outside_triangle_is_an_f32_artefact():
+    """f32 Moller-Trumbore reports triangle 1 at t = 25.344927 although the
+    ray passes outside it (f64 barycentric -0.017, cos 5.7e-4): the reported
+    point lies beyond the triangle's box.  Brute force and the robust
+    traversal (box margin 2^-16) both see it; a tree whose boxes cull it
+    returns triangle 0, which the closest-hit agreement check accepts only
+    because the closer hit is such an artefact."""
+    from hitcheck import assert_same_closest, f64_barycentric_min
+    P = np.array(GRAZE_V, np.float32)
+    T = np.array([[0, 1, 2], [2, 1, 3]], np.uint32)
+    o = np.array([GRAZE_O], np.float32)
+    d = np.array([GRAZE_D], np.float32)
+    bt, _, _, bp = O.brute_intersect(P, T, o, d)
+    assert bp[0] == 1 and abs(float(bt[0]) - 25.344927) < 1e-5
+    assert f64_barycentric_min(P, T, 1, o[0], d[0]) < -0.01
+    assert f64_barycentric_min(P, T, 0, o[0], d[0]) > 0.0
+    desc = ag.SceneDesc(P, T)
+    desc.add_material(ag.Material.diffuse([0.5, 0.5, 0.5]))
+    rt, _, _, rp = O.OracleScene(desc).intersect(o, d, mode=O.MODE_ROBUST)
+    assert rp[0] == 1 and rt[0] == bt[0]
+    # the permitted difference, and a forbidden one (missing a genuine closer hit)
+    assert assert_same_closest(P, T, o, d, (np.float32([25.346556]), np.uint32([0])), (bt, bp)) == 1
+    with pytest.raises(AssertionError):
+        assert_same_closest(P, T, o, d, (np.float32([30.0]), np.uint32([1])), (np.float32([25.3466]), np.uint32([0])))
+
+
 def test_wavelength_sampling():
     for i in range(1000):
         lam, pdf = O.sample_wavelengths((i + 0.5) / 1000)
