@@ -196,6 +196,18 @@ class SceneDesc:
         self.d.env = env
         self._env_owned = True
 
+    def light_cdf(self):
+        """Scene::rebuild_lights (scene.rs:88-110): (number of emitting panels,
+        their indices, the CDF by power, the probability of picking the
+        environment map)."""
+        n = C.c_int32()
+        idx = np.zeros(A.MAX_PANELS, np.int32)
+        cdf = np.zeros(A.MAX_PANELS, np.float32)
+        pick = C.c_float()
+        check(A.lib().ag_desc_light_cdf(C.byref(self.d), C.byref(n), idx.ctypes.data_as(A.i32p),
+                                        cdf.ctypes.data_as(A.f32p), C.byref(pick)))
+        return n.value, idx[:n.value], cdf[:n.value], pick.value
+
     def free_env(self):
         if self._env_owned:
             A.lib().ag_env_free(C.byref(self.d.env))

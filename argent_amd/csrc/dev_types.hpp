@@ -14,7 +14,10 @@ namespace ag {
 
 constexpr int MAX_KEYS = 2 + AG_MAX_MATERIALS;  // miss, panel, one per material
 constexpr int TRAV_BLOCK = 256;                  // threads per traversal block
-constexpr int STACK_LDS = 16;                    // stack entries per lane in LDS
+#ifndef AG_STACK_LDS
+#define AG_STACK_LDS 8
+#endif
+constexpr int STACK_LDS = AG_STACK_LDS;          // stack entries per lane in LDS
 constexpr int STACK_SPILL = 64;                  // further entries per thread in HBM
 constexpr int MAX_SEGMENTS = 4096;               // work segments per wave
 
@@ -130,10 +133,20 @@ struct DQueues {
     float4* sh_c;         // contribution spectrum
     float4* sh_x;         // (a, p_light, clamp or -1, kind bits)
     uint32_t* counters;   // per depth: [ray count, shadow count, key counts...]
+    uint32_t* cursors;    // per depth: extension set, shadow set (CURSOR_SET u32 each)
     uint32_t cap;
     int32_t n_keys;
 };
 
-constexpr int CNT_STRIDE = 2 + MAX_KEYS + 2;  // counters per depth slot
+// Counters per depth slot: ray count, shadow count, one per bucket key.
+constexpr int CNT_STRIDE = 4 + MAX_KEYS;
+// Ray cursors of the persistent traversal kernels: a queue of n rays is cut
+// into CURSOR_SLICES equal slices, one per XCD, each with its own cursor on
+// its own 128-byte line; per depth slot there is one set for extension rays
+// and one for shadow rays.
+constexpr int CURSOR_SLICES = 8;
+constexpr int CURSOR_STRIDE = 32;  // u32 per cursor (128 B)
+constexpr int CURSOR_SET = CURSOR_SLICES * CURSOR_STRIDE;
+constexpr int CURSORS_PER_DEPTH = 2 * CURSOR_SET;
 
 }  // namespace ag

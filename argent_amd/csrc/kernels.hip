@@ -135,107 +135,238 @@ AGD float tri_test(const float4* tp, f3 o, f3 d, float* uo, float* vo, uint32_t*
     return t;
 }
 
-struct TriHit {
-    float t, u, v;
-    uint32_t prim;
+// ------------------------------------------------------------------ persistent traversal
+// Rays are pulled by wavefronts, not assigned to threads.  A queue of n rays
+// is cut into CURSOR_SLICES slices, one per XCD (blocks are dealt to the XCDs
+// round-robin, so block b starts on slice b % 8), each with its own cursor;
+// a wavefront takes GRAB rays from a slice per atomic and hands them to its
+// idle lanes whenever at least REFILL_MIN lanes are idle, moving on to the
+// next slice when its own is spent.  Every busy lane advances by one inner
+// node or one leaf per iteration, and the lanes that finished are handed to
+// the job together.  A lane's ray therefore never waits for the slowest ray
+// of a static batch of 64, and the atomics stay few and spread over lines.
+constexpr uint32_t NO_RAY = 0xFFFFFFFFu;
+constexpr uint32_t TRAV_DONE = 0xFFFFFFFEu;  // never an entry: leaf entries stay below 0xA0000000
+#ifndef AG_REFILL_MIN
+#define AG_REFILL_MIN 16
+#endif
+#ifndef AG_GRAB
+#define AG_GRAB 64
+#endif
+#ifndef AG_TRAV_WAVES_PER_EU
+#define AG_TRAV_WAVES_PER_EU 8
+#endif
+constexpr uint32_t REFILL_MIN = AG_REFILL_MIN;  // idle lanes that trigger a refill (64: whole batches)
+constexpr uint32_t GRAB = AG_GRAB;              // rays taken per cursor atomic
+
+AGD uint32_t lane_prefix(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wavefront-uniform state of the ray supply.
+struct RaySupply {
+    uint32_t n;
+    uint32_t* cursors;
+    uint32_t lo, hi;      // grabbed range not yet handed out
+    uint32_t slice, tried;
 };
 
-AGD TriHit bvh_closest(const float4* __restrict__ nodes, const float4* __restrict__ tris, f3 o, f3 d, float tmax,
-                       const TravStack& st) {
-    TriHit best{tmax, 0.0f, 0.0f, 0xFFFFFFFFu};
-    f3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    int sp = 0;
-    uint32_t cur = 0;  // root (always an inner node)
-    for (;;) {
-        if (!(cur & 0x80000000u)) {
-            float tm[4];
-            uint32_t ch[4];
-            slab4(nodes + (size_t)cur * 8, o, inv, best.t, tm, ch);
-            // sort ascending (network on 4)
+AGD uint32_t slice_begin(uint32_t n, uint32_t s) { return (uint32_t)(((uint64_t)n * s) / CURSOR_SLICES); }
+
+// Up to `want` consecutive ray indices from *first; 0 once every slice is
+// spent.  Reached by all lanes of the wavefront.
+AGD uint32_t supply_take(RaySupply& R, uint32_t want, uint32_t* first) {
+    while (R.lo == R.hi) {
+        if (R.tried >= CURSOR_SLICES) return 0;
+        uint32_t b = slice_begin(R.n, R.slice), e = slice_begin(R.n, R.slice + 1);
+        uint32_t got = 0;
+        if (lane_id() == 0) got = atomicAdd(R.cursors + R.slice * CURSOR_STRIDE, GRAB);
+        got = __shfl(got, 0);
+        if (got < e - b) {
+            R.lo = b + got;
+            R.hi = b + min(got + GRAB, e - b);
+        } else {
+            R.slice = (R.slice + 1) % CURSOR_SLICES;
+            R.tried++;
+        }
+    }
+    uint32_t k = min(want, R.hi - R.lo);
+    *first = R.lo;
+    R.lo += k;
+    return k;
+}
+
+struct TravLane {
+    const float4* nodes;
+    const float4* tris;
+    f3 o, d, inv;
+    float t;        // closest hit: best t so far (starts at t_max); any hit: t_max
+    float u, v;
+    uint32_t prim;  // closest hit: original triangle or NO_RAY; any hit: 1 once occluded
+    uint32_t cur;   // entry to visit next, or TRAV_DONE
+    int sp;
+};
+
+AGD void lane_begin(TravLane& L, const float4* nodes, const float4* tris, f3 o, f3 d, float tmax) {
+    L.nodes = nodes;
+    L.tris = tris;
+    L.o = o;
+    L.d = d;
+    L.inv = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    L.t = tmax;
+    L.u = 0.0f;
+    L.v = 0.0f;
+    L.prim = NO_RAY;
+    L.cur = 0;  // the root is always an inner node
+    L.sp = 0;
+}
+
+// Bvh::intersect (bvh.rs:159-213), one step: an inner node pushes its hit
+// children far to near and continues with the nearest; a leaf tests its
+// triangles (ties on t go to the lower original index).  Then the next stack
+// entry not beyond the best hit, or done.
+AGD void step_closest(TravLane& L, const TravStack& st) {
+    if (!(L.cur & 0x80000000u)) {
+        float tm[4];
+        uint32_t ch[4];
+        slab4(L.nodes + (size_t)L.cur * 8, L.o, L.inv, L.t, tm, ch);
 #define CSWAP(i, j)                                       \
     if (tm[j] < tm[i]) {                                  \
         float tt = tm[i]; tm[i] = tm[j]; tm[j] = tt;      \
         uint32_t cc = ch[i]; ch[i] = ch[j]; ch[j] = cc;   \
     }
-            CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
+        CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
 #undef CSWAP
-            if (tm[0] != INF_F) {
-                if (tm[3] != INF_F) st_push(st, sp, ch[3], tm[3]);
-                if (tm[2] != INF_F) st_push(st, sp, ch[2], tm[2]);
-                if (tm[1] != INF_F) st_push(st, sp, ch[1], tm[1]);
-                cur = ch[0];
-                continue;
-            }
-        } else {
-            uint32_t first = cur & 0x07FFFFFFu;
-            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
-            for (uint32_t k = 0; k < cnt; k++) {
-                float u, v;
-                uint32_t prim;
-                float t = tri_test(tris + (size_t)(first + k) * 3, o, d, &u, &v, &prim);
-                if (t > 1e-7f && (t < best.t || (t == best.t && best.prim != 0xFFFFFFFFu && prim < best.prim))) {
-                    best.t = t;
-                    best.u = u;
-                    best.v = v;
-                    best.prim = prim;
-                }
+        if (tm[0] != INF_F) {
+            if (tm[3] != INF_F) st_push(st, L.sp, ch[3], tm[3]);
+            if (tm[2] != INF_F) st_push(st, L.sp, ch[2], tm[2]);
+            if (tm[1] != INF_F) st_push(st, L.sp, ch[1], tm[1]);
+            L.cur = ch[0];
+            return;
+        }
+    } else {
+        uint32_t first = L.cur & 0x07FFFFFFu;
+        uint32_t cnt = ((L.cur >> 27) & 0xFu) + 1u;
+        for (uint32_t k = 0; k < cnt; k++) {
+            float u, v;
+            uint32_t prim;
+            float t = tri_test(L.tris + (size_t)(first + k) * 3, L.o, L.d, &u, &v, &prim);
+            if (t > 1e-7f && (t < L.t || (t == L.t && L.prim != NO_RAY && prim < L.prim))) {
+                L.t = t;
+                L.u = u;
+                L.v = v;
+                L.prim = prim;
             }
         }
-        // pop the next entry not beyond the current best
-        bool found = false;
-        while (sp > 0) {
-            float tn;
-            st_pop(st, sp, cur, tn);
-            if (!(tn > best.t)) {
-                found = true;
-                break;
-            }
-        }
-        if (!found) break;
     }
-    return best;
+    L.cur = TRAV_DONE;
+    while (L.sp > 0) {
+        uint32_t e;
+        float tn;
+        st_pop(st, L.sp, e, tn);
+        if (!(tn > L.t)) {
+            L.cur = e;
+            break;
+        }
+    }
 }
 
-AGD bool bvh_occluded(const float4* __restrict__ nodes, const float4* __restrict__ tris, f3 o, f3 d, float tmax,
-                      const TravStack& st) {
-    f3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    int sp = 0;
-    uint32_t cur = 0;
-    for (;;) {
-        if (!(cur & 0x80000000u)) {
-            float tm[4];
-            uint32_t ch[4];
-            slab4(nodes + (size_t)cur * 8, o, inv, tmax, tm, ch);
-            int first = -1;
+// Bvh::occluded (bvh.rs:216-245), one step: no ordering, stop at the first
+// triangle hit in (1e-7, t_max).
+AGD void step_any(TravLane& L, const TravStack& st) {
+    if (!(L.cur & 0x80000000u)) {
+        float tm[4];
+        uint32_t ch[4];
+        slab4(L.nodes + (size_t)L.cur * 8, L.o, L.inv, L.t, tm, ch);
+        int first = -1;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (tm[k] != INF_F) {
-                    if (first < 0)
-                        first = k;
-                    else
-                        st_push(st, sp, ch[k], tm[k]);
-                }
-            }
-            if (first >= 0) {
-                cur = ch[first];
-                continue;
-            }
-        } else {
-            uint32_t firstt = cur & 0x07FFFFFFu;
-            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
-            for (uint32_t k = 0; k < cnt; k++) {
-                float u, v;
-                uint32_t prim;
-                float t = tri_test(tris + (size_t)(firstt + k) * 3, o, d, &u, &v, &prim);
-                if (t > 1e-7f && t < tmax) return true;
+        for (int k = 0; k < 4; k++) {
+            if (tm[k] != INF_F) {
+                if (first < 0)
+                    first = k;
+                else
+                    st_push(st, L.sp, ch[k], tm[k]);
             }
         }
-        if (sp == 0) break;
-        float tn;
-        st_pop(st, sp, cur, tn);
+        if (first >= 0) {
+            L.cur = ch[first];
+            return;
+        }
+    } else {
+        uint32_t first = L.cur & 0x07FFFFFFu;
+        uint32_t cnt = ((L.cur >> 27) & 0xFu) + 1u;
+        for (uint32_t k = 0; k < cnt; k++) {
+            float u, v;
+            uint32_t prim;
+            float t = tri_test(L.tris + (size_t)(first + k) * 3, L.o, L.d, &u, &v, &prim);
+            if (t > 1e-7f && t < L.t) {
+                L.prim = 1;
+                L.cur = TRAV_DONE;
+                return;
+            }
+        }
     }
-    return false;
+    if (L.sp > 0) {
+        float tn;
+        st_pop(st, L.sp, L.cur, tn);
+    } else {
+        L.cur = TRAV_DONE;
+    }
 }
+
+// Job: bool begin(uint32_t i, TravLane& L) sets up ray i (false: nothing to
+// trace, finish at once); void finish(uint32_t i, const TravLane& L, bool mine)
+// is called by every lane of the wavefront, `mine` where ray i just finished;
+// void drain() once by every lane after the last ray.
+template <bool ANY, class Job>
+AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, const TravStack& st) {
+    TravLane L;
+    L.cur = TRAV_DONE;
+    L.sp = 0;
+    uint32_t idx = NO_RAY;
+    RaySupply R{n, cursors, 0u, 0u, blockIdx.x % CURSOR_SLICES, 0u};
+    bool more = n > 0;  // uniform across the wavefront
+    for (;;) {
+        if (more) {
+            uint64_t idle = __ballot(idx == NO_RAY);
+            uint32_t cnt = (uint32_t)__popcll(idle);
+            if (cnt >= REFILL_MIN) {
+                uint32_t first = 0;
+                uint32_t k = supply_take(R, cnt, &first);
+                if (k == 0) {
+                    more = false;
+                } else if (idx == NO_RAY) {
+                    uint32_t r = lane_prefix(idle);
+                    if (r < k) {
+                        idx = first + r;
+                        if (!J.begin(idx, L)) L.cur = TRAV_DONE;
+                    }
+                }
+            }
+        }
+        if (__ballot(idx != NO_RAY) == 0) break;  // only once every slice is spent
+        if (idx != NO_RAY && L.cur != TRAV_DONE) {
+            if (ANY)
+                step_any(L, st);
+            else
+                step_closest(L, st);
+        }
+        bool fin = idx != NO_RAY && L.cur == TRAV_DONE;
+        if (__ballot(fin) != 0) {
+            J.finish(idx, L, fin);
+            if (fin) idx = NO_RAY;
+        }
+    }
+    J.drain();
+}
+
+// Traversal kernels keep AG_TRAV_WAVES_PER_EU wavefronts per SIMD (8: 64
+// VGPRs): latency hiding matters more to them than registers.
+#if AG_TRAV_WAVES_PER_EU > 0
+#define TRAV_WAVES __attribute__((amdgpu_waves_per_eu(AG_TRAV_WAVES_PER_EU, AG_TRAV_WAVES_PER_EU)))
+#else
+#define TRAV_WAVES
+#endif
 
 #define TRAV_STACK_DECL                                                     \
     __shared__ uint32_t s_node[STACK_LDS * TRAV_BLOCK];                     \
@@ -248,31 +379,48 @@ AGD bool bvh_occluded(const float4* __restrict__ nodes, const float4* __restrict
     st.spill = spill + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * STACK_SPILL;
 
 // ------------------------------------------------------------------ ray-batch kernels
-__global__ __launch_bounds__(TRAV_BLOCK) void k_rays_intersect(const float4* __restrict__ nodes,
-                                                               const float4* __restrict__ tris, RaysSoA rays,
-                                                               HitsSoA hits, uint32_t n, uint2* spill) {
-    TRAV_STACK_DECL
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+struct BatchJob {
+    const float4* nodes;
+    const float4* tris;
+    RaysSoA rays;
+    HitsSoA hits;
+    uint8_t* occ;
+    AGD bool begin(uint32_t i, TravLane& L) {
         f3 o{rays.ox[i], rays.oy[i], rays.oz[i]}, d{rays.dx[i], rays.dy[i], rays.dz[i]};
-        float tmax = rays.tmax ? rays.tmax[i] : INF_F;
-        TriHit h = bvh_closest(nodes, tris, o, d, tmax, st);
-        bool hit = h.prim != 0xFFFFFFFFu;
-        hits.t[i] = hit ? h.t : INF_F;
-        hits.u[i] = hit ? h.u : 0.0f;
-        hits.v[i] = hit ? h.v : 0.0f;
-        hits.prim[i] = h.prim;
+        lane_begin(L, nodes, tris, o, d, rays.tmax ? rays.tmax[i] : INF_F);
+        return true;
     }
+    AGD void finish(uint32_t i, const TravLane& L, bool mine) {
+        if (!mine) return;
+        if (occ) {
+            occ[i] = L.prim != NO_RAY ? 1 : 0;
+            return;
+        }
+        bool hit = L.prim != NO_RAY;
+        hits.t[i] = hit ? L.t : INF_F;
+        hits.u[i] = hit ? L.u : 0.0f;
+        hits.v[i] = hit ? L.v : 0.0f;
+        hits.prim[i] = L.prim;
+    }
+    AGD void drain() {}
+};
+
+__global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_rays_intersect(const float4* __restrict__ nodes,
+                                                               const float4* __restrict__ tris, RaysSoA rays,
+                                                               HitsSoA hits, uint32_t n, uint32_t* cursor,
+                                                               uint2* spill) {
+    TRAV_STACK_DECL
+    BatchJob J{nodes, tris, rays, hits, nullptr};
+    traverse_persistent<false>(J, n, cursor, st);
 }
 
-__global__ __launch_bounds__(TRAV_BLOCK) void k_rays_occluded(const float4* __restrict__ nodes,
+__global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_rays_occluded(const float4* __restrict__ nodes,
                                                               const float4* __restrict__ tris, RaysSoA rays,
-                                                              uint8_t* out, uint32_t n, uint2* spill) {
+                                                              uint8_t* out, uint32_t n, uint32_t* cursor,
+                                                              uint2* spill) {
     TRAV_STACK_DECL
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        f3 o{rays.ox[i], rays.oy[i], rays.oz[i]}, d{rays.dx[i], rays.dy[i], rays.dz[i]};
-        float tmax = rays.tmax ? rays.tmax[i] : INF_F;
-        out[i] = bvh_occluded(nodes, tris, o, d, tmax, st) ? 1 : 0;
-    }
+    BatchJob J{nodes, tris, rays, HitsSoA{}, out};
+    traverse_persistent<true>(J, n, cursor, st);
 }
 
 // ------------------------------------------------------------------ scene pieces
@@ -581,34 +729,46 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
     }
 }
 
-// scene.rs:169-218 (closest hit over the piece, the panels and the floor)
-__global__ __launch_bounds__(TRAV_BLOCK) void k_extend(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
-                                                       uint2* spill) {
-    TRAV_STACK_DECL
-    uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
-    const uint32_t n = cnt[0];
-    const float4* ro = Q.ray_o[depth & 1];
-    const float4* rd = Q.ray_d[depth & 1];
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        uint32_t i = base + threadIdx.x;
+// scene.rs:169-218 (closest hit over the piece, the panels and the floor):
+// the piece in object space (ray turned by -turntable), then panels and floor
+// in world space.  Finished rays are staged per wavefront in LDS and appended
+// to the material buckets 64 or more at a time (one atomic per key per flush).
+constexpr int STAGE = 128;  // staging entries per wavefront: ray index | key << 26
+
+struct ExtendJob {
+    const DView* views;
+    DPaths P;
+    DQueues Q;
+    int depth;
+    uint32_t* cnt;
+    const float4* ro;
+    const float4* rd;
+    uint32_t* stage;  // this wavefront's STAGE entries
+    uint32_t staged;  // uniform
+    AGD bool begin(uint32_t i, TravLane& L) {
+        float4 a = ro[i], b = rd[i];
+        const DView& V = views[P.view[__float_as_uint(a.w)]];
+        const DScene& S = *V.scene;
+        f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+        if (V.turn_deg != 0.0f) {
+            o = rot_y(o, V.tsn, V.tcn);
+            d = rot_y(d, V.tsn, V.tcn);
+        }
+        lane_begin(L, S.nodes, S.tris, o, d, INF_F);
+        return true;
+    }
+    AGD void finish(uint32_t i, const TravLane& L, bool mine) {
         int key = -1;
-        if (i < n) {
+        if (mine) {
             float4 a = ro[i], b = rd[i];
-            uint32_t path = __float_as_uint(a.w);
-            f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
-            const DView& V = views[P.view[path]];
+            const DView& V = views[P.view[__float_as_uint(a.w)]];
             const DScene& S = *V.scene;
-            f3 oo = o, od = d;
-            if (V.turn_deg != 0.0f) {
-                oo = rot_y(o, V.tsn, V.tcn);
-                od = rot_y(d, V.tsn, V.tcn);
-            }
-            TriHit h = bvh_closest(S.nodes, S.tris, oo, od, INF_F, st);
+            f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
             float tmax = INF_F;
             uint32_t code = HIT_MISS;
-            if (h.prim != 0xFFFFFFFFu) {
-                tmax = h.t;
-                code = h.prim;
+            if (L.prim != NO_RAY) {
+                tmax = L.t;
+                code = L.prim;
             }
             for (int k = 0; k < S.n_panels; k++) {
                 if (depth == 0 && !S.panel[k].camera_visible) continue;
@@ -625,7 +785,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) void k_extend(const DView* __restrict__
                     code = HIT_FLOOR;
                 }
             }
-            Q.hit[i] = make_float4(tmax, h.u, h.v, __uint_as_float(code));
+            Q.hit[i] = make_float4(tmax, L.u, L.v, __uint_as_float(code));
             if (code == HIT_MISS)
                 key = 0;
             else if (code == HIT_FLOOR)
@@ -635,9 +795,44 @@ __global__ __launch_bounds__(TRAV_BLOCK) void k_extend(const DView* __restrict__
             else
                 key = 2 + (int)S.tmat[code];
         }
-        uint32_t slot = wave_append_keyed(cnt + 2, key);
-        if (key >= 0) Q.bucket[(size_t)key * Q.cap + slot] = i;
+        uint64_t m = __ballot(key >= 0);
+        if (key >= 0) stage[staged + lane_prefix(m)] = i | ((uint32_t)key << 26);
+        staged += (uint32_t)__popcll(m);
+        if (staged >= 64) flush();
     }
+    AGD void flush() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t base = 0; base < staged; base += 64) {
+            uint32_t e = base + lane_id();
+            int key = -1;
+            uint32_t i = 0;
+            if (e < staged) {
+                uint32_t v = stage[e];
+                i = v & 0x03FFFFFFu;
+                key = (int)(v >> 26);
+            }
+            uint32_t slot = wave_append_keyed(cnt + 2, key);
+            if (key >= 0) Q.bucket[(size_t)key * Q.cap + slot] = i;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        staged = 0;
+    }
+    AGD void drain() {
+        if (staged) flush();
+    }
+};
+
+__global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
+                                                       uint2* spill) {
+    TRAV_STACK_DECL
+    __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
+    uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
+    ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64], 0u};
+    traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, st);
 }
 
 struct ShadeArgs {
@@ -893,70 +1088,90 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
     }
 }
 
-// scene.rs:120-140 (sky_visibility) and scene.rs:220-243 (visible)
-__global__ __launch_bounds__(TRAV_BLOCK) void k_shadow(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
-                                                       uint2* spill) {
-    TRAV_STACK_DECL
-    const uint32_t n = Q.counters[depth * CNT_STRIDE + 1];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float4 a = Q.sh_o[i], b = Q.sh_d[i], x = Q.sh_x[i];
-        uint32_t path = __float_as_uint(a.w);
+// scene.rs:120-140 (sky_visibility) and scene.rs:220-243 (visible): the
+// piece first (any hit, object space), then the other panels and the floor.
+struct ShadowJob {
+    const DView* views;
+    DPaths P;
+    DQueues Q;
+    // kind 0: segment to a light point (t_max = dist * (1 - 1e-4)); else a sky direction
+    AGD void ray(uint32_t i, f3& from, f3& dir, float& tmax, int& ignore, uint32_t& kind, uint32_t& path) const {
+        float4 a = Q.sh_o[i], b = Q.sh_d[i];
+        path = __float_as_uint(a.w);
+        from = f3{a.x, a.y, a.z};
+        kind = __float_as_uint(Q.sh_x[i].w);
+        if (kind == 0u) {
+            f3 dd = f3{b.x, b.y, b.z} - from;
+            float dist = length(dd);
+            dir = dd / dist;
+            tmax = dist * (1.0f - 1e-4f);
+            ignore = (int)__float_as_uint(b.w);
+        } else {
+            dir = f3{b.x, b.y, b.z};
+            tmax = INF_F;
+            ignore = -1;
+        }
+    }
+    AGD bool begin(uint32_t i, TravLane& L) {
+        f3 from, dir;
+        float tmax;
+        int ignore;
+        uint32_t kind, path;
+        ray(i, from, dir, tmax, ignore, kind, path);
         const DView& V = views[P.view[path]];
         const DScene& S = *V.scene;
-        f3 from{a.x, a.y, a.z};
-        uint32_t kind = __float_as_uint(x.w);
+        if (V.turn_deg != 0.0f) {
+            from = rot_y(from, V.tsn, V.tcn);
+            dir = rot_y(dir, V.tsn, V.tcn);
+        }
+        lane_begin(L, S.nodes, S.tris, from, dir, tmax);
+        return true;
+    }
+    AGD void finish(uint32_t i, const TravLane& L, bool mine) {
+        if (!mine || L.prim != NO_RAY) return;  // blocked by the piece
+        f3 from, dir;
+        float tmax;
+        int ignore;
+        uint32_t kind, path;
+        ray(i, from, dir, tmax, ignore, kind, path);
+        const DScene& S = *views[P.view[path]].scene;
         Spec c = from4(Q.sh_c[i]);
-        bool add = false;
+        float4 x = Q.sh_x[i];
         if (kind == 0u) {
-            f3 to{b.x, b.y, b.z};
-            int ignore = (int)__float_as_uint(b.w);
-            f3 dd = to - from;
-            float dist = length(dd);
-            f3 dir = dd / dist;
-            float tmax = dist * (1.0f - 1e-4f);
-            f3 oo = from, od = dir;
-            if (V.turn_deg != 0.0f) {
-                oo = rot_y(from, V.tsn, V.tcn);
-                od = rot_y(dir, V.tsn, V.tcn);
-            }
-            bool vis = !bvh_occluded(S.nodes, S.tris, oo, od, tmax, st);
-            for (int k = 0; vis && k < S.n_panels; k++) {
+            for (int k = 0; k < S.n_panels; k++) {
                 float t;
-                if (k != ignore && panel_intersect(S.panel[k], from, dir, tmax, &t)) vis = false;
+                if (k != ignore && panel_intersect(S.panel[k], from, dir, tmax, &t)) return;
             }
-            if (vis && S.floor_present) {
+            if (S.floor_present) {
                 float t;
-                if (floor_intersect(S, from, dir, tmax, &t)) vis = false;
+                if (floor_intersect(S, from, dir, tmax, &t)) return;
             }
-            add = vis;
         } else {
-            f3 dir{b.x, b.y, b.z};
-            f3 oo = from, od = dir;
-            if (V.turn_deg != 0.0f) {
-                oo = rot_y(from, V.tsn, V.tcn);
-                od = rot_y(dir, V.tsn, V.tcn);
-            }
             float vis = 1.0f;
-            if (bvh_occluded(S.nodes, S.tris, oo, od, INF_F, st)) vis = 0.0f;
-            for (int k = 0; vis > 0.0f && k < S.n_panels; k++) {
+            for (int k = 0; k < S.n_panels; k++) {
                 float t;
-                if (panel_intersect(S.panel[k], from, dir, INF_F, &t)) vis = 0.0f;
+                if (panel_intersect(S.panel[k], from, dir, INF_F, &t)) return;
             }
-            if (vis > 0.0f && S.floor_present) {
+            if (S.floor_present) {
                 float t;
                 if (floor_intersect(S, from, dir, INF_F, &t)) vis = 1.0f - floor_presence(S, from + dir * t);
             }
-            if (vis > 0.0f) {
-                c = c * (x.x * vis / x.y);
-                add = true;
-            }
+            if (!(vis > 0.0f)) return;
+            c = c * (x.x * vis / x.y);
         }
-        if (add) {
-            Spec L = from4(P.L[path]);
-            add_clamped(L, c, x.z >= 0.0f, x.z);
-            P.L[path] = to4(L);
-        }
+        Spec Lr = from4(P.L[path]);
+        add_clamped(Lr, c, x.z >= 0.0f, x.z);
+        P.L[path] = to4(Lr);
     }
+    AGD void drain() {}
+};
+
+__global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
+                                                       uint2* spill) {
+    TRAV_STACK_DECL
+    uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
+    ShadowJob J{views, P, Q};
+    traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, st);
 }
 
 // render.rs:96-111: per pixel, samples in order, into the pass accumulators
@@ -1072,12 +1287,23 @@ void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n) {
 }
 
 void launch_rays_intersect(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, HitsSoA h,
-                           uint32_t n, uint2* spill) {
-    hipLaunchKernelGGL(k_rays_intersect, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, h, n, spill);
+                           uint32_t n, uint32_t* cursor, uint2* spill) {
+    hipLaunchKernelGGL(k_rays_intersect, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, h, n, cursor, spill);
 }
 void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, uint8_t* out,
-                          uint32_t n, uint2* spill) {
-    hipLaunchKernelGGL(k_rays_occluded, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, out, n, spill);
+                          uint32_t n, uint32_t* cursor, uint2* spill) {
+    hipLaunchKernelGGL(k_rays_occluded, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, out, n, cursor, spill);
+}
+int traversal_blocks_per_cu() {
+    int lo = 1 << 30;
+    const void* ks[4] = {(const void*)k_extend, (const void*)k_shadow, (const void*)k_rays_intersect,
+                         (const void*)k_rays_occluded};
+    for (const void* k : ks) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, TRAV_BLOCK, 0) != hipSuccess || b < 1) b = 1;
+        lo = std::min(lo, b);
+    }
+    return lo;
 }
 void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,
                    DQueues Q) {

@@ -74,10 +74,12 @@ struct ag_ctx {
     uint32_t cap = 0;
     int n_keys_alloc = 0;
     DevBuf paths_buf, rays_buf, hit_buf, bucket_buf, shadow_buf, counter_buf, spill_buf, seg_buf, view_buf, vmat_buf,
+        cursor_buf,
         vtab_buf;
     DPaths P{};
     DQueues Q{};
     int grid_trav = 0, grid_work = 0;
+    int max_trav_blocks = 1;  // resident traversal blocks per CU (occupancy)
     size_t spill_threads = 0;
     // timing: HIP event pairs around every stage launch on the context stream
     bool timing = false;
@@ -139,7 +141,8 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     for (auto& e : c->ev) (void)hipEventCreate(&e);
     c->tuning.paths_per_wave = 1u << 22;
     c->tuning.block_traverse = TRAV_BLOCK;
-    c->tuning.blocks_per_cu = 4;
+    c->max_trav_blocks = traversal_blocks_per_cu();
+    c->tuning.blocks_per_cu = (uint32_t)c->max_trav_blocks;
     c->tuning.sort_by_material = 1;
     *out = c;
     return AG_OK;
@@ -172,7 +175,9 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
     if (!c || !t) return fail(AG_E_INVALID, "null argument");
     if (t->paths_per_wave < 1024 || t->paths_per_wave > (1u << 26)) return fail(AG_E_INVALID, "paths_per_wave out of range");
     if (t->block_traverse != TRAV_BLOCK) return fail(AG_E_INVALID, "block_traverse must be 256");
-    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 5) return fail(AG_E_INVALID, "blocks_per_cu must be 1..5");
+    if (t->blocks_per_cu < 1 || (int)t->blocks_per_cu > c->max_trav_blocks)
+        return fail(AG_E_INVALID, "blocks_per_cu must be 1.." + std::to_string(c->max_trav_blocks) +
+                                      " (resident traversal blocks per CU)");
     c->tuning = *t;
     return AG_OK;
 }
@@ -199,6 +204,12 @@ static int ensure_spill(ag_ctx* c) {
         c->spill_threads = threads;
     }
     return AG_OK;
+}
+
+// Counter words per wave, rounded up to whole 128-byte lines; the cursors follow.
+static size_t counter_words(uint32_t max_depth) {
+    size_t n = (size_t)(max_depth + 2) * CNT_STRIDE;
+    return (n + CURSOR_STRIDE - 1) / CURSOR_STRIDE * CURSOR_STRIDE;
 }
 
 static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) {
@@ -246,9 +257,11 @@ static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) 
         if ((rc = c->bucket_buf.ensure((size_t)cap * n_keys * sizeof(uint32_t)))) return rc;
         c->n_keys_alloc = n_keys;
     }
-    if ((rc = c->counter_buf.ensure((size_t)(max_depth + 2) * CNT_STRIDE * sizeof(uint32_t)))) return rc;
+    size_t n_cnt = counter_words(max_depth);
+    if ((rc = c->counter_buf.ensure((n_cnt + (size_t)(max_depth + 2) * CURSORS_PER_DEPTH) * sizeof(uint32_t)))) return rc;
     c->Q.bucket = c->bucket_buf.as<uint32_t>();
     c->Q.counters = c->counter_buf.as<uint32_t>();
+    c->Q.cursors = c->Q.counters + n_cnt;
     c->Q.cap = cap;
     c->Q.n_keys = n_keys;
     if ((rc = c->seg_buf.ensure(MAX_SEGMENTS * sizeof(DSeg)))) return rc;
@@ -716,7 +729,8 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
         uint32_t npaths = 0;
         for (auto& sg : w) npaths = std::max(npaths, sg.begin + sg.npix * sg.ns);
         HIPCHK(hipMemcpyAsync(c->seg_buf.p, w.data(), w.size() * sizeof(DSeg), hipMemcpyHostToDevice, st));
-        launch_zero_u32(st, c->Q.counters, (ps->max_depth + 2) * CNT_STRIDE);
+        launch_zero_u32(st, c->Q.counters,
+                        (uint32_t)(counter_words(ps->max_depth) + (size_t)(ps->max_depth + 2) * CURSORS_PER_DEPTH));
         timed(3, [&] { launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), npaths, views, c->P, c->Q); });
         for (uint32_t d = 0; d < ps->max_depth; d++) {
             timed(0, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>()); });
@@ -989,6 +1003,7 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
     HIPCHK(hipSetDevice(ctx->device));
     int rc = ensure_spill(ctx);
     if (rc) return rc;
+    if ((rc = ctx->cursor_buf.ensure(CURSOR_SET * sizeof(uint32_t)))) return rc;
     hipStream_t st = ctx->stream;
     RaysSoA r{};
     DevBuf in, outb;
@@ -1019,8 +1034,9 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
             float* b = outb.as<float>();
             h = HitsSoA{b, b + n, b + 2 * (size_t)n, reinterpret_cast<uint32_t*>(b + 3 * (size_t)n)};
         }
+        launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
         HIPCHK(hipEventRecord(ctx->ev[2], st));
-        launch_rays_intersect(st, grid, nodes, tris, r, h, n, ctx->spill_buf.as<uint2>());
+        launch_rays_intersect(st, grid, nodes, tris, r, h, n, ctx->cursor_buf.as<uint32_t>(), ctx->spill_buf.as<uint2>());
         HIPCHK(hipEventRecord(ctx->ev[3], st));
         HIPCHK(hipGetLastError());
         if (!device_ptrs) {
@@ -1035,8 +1051,9 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
             if ((rc = outb.ensure(n))) return rc;
             o = outb.as<uint8_t>();
         }
+        launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
         HIPCHK(hipEventRecord(ctx->ev[2], st));
-        launch_rays_occluded(st, grid, nodes, tris, r, o, n, ctx->spill_buf.as<uint2>());
+        launch_rays_occluded(st, grid, nodes, tris, r, o, n, ctx->cursor_buf.as<uint32_t>(), ctx->spill_buf.as<uint2>());
         HIPCHK(hipEventRecord(ctx->ev[3], st));
         HIPCHK(hipGetLastError());
         if (!device_ptrs) HIPCHK(hipMemcpyAsync(occ, o, n, hipMemcpyDeviceToHost, st));

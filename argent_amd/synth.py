@@ -157,6 +157,136 @@ def signet(seed=2, band_nu=1024, band_nv=256, plate_nt=412, plate_np=820, relief
     return pos, tris, vertex_normals(pos, tris)
 
 
+def _clip(poly, n, c):
+    """Sutherland-Hodgman: keep the part of a convex polygon with n.x <= c."""
+    out = []
+    m = len(poly)
+    for i in range(m):
+        a, b = poly[i], poly[(i + 1) % m]
+        da, db = np.dot(n, a) - c, np.dot(n, b) - c
+        if da <= 0.0:
+            out.append(a)
+        if (da < 0.0) != (db < 0.0) and da != db:
+            out.append(a + (b - a) * (da / (da - db)))
+    return out
+
+
+def convex_hull_of_planes(normals, offsets):
+    """The polytope {x : n_i . x <= c_i} as a triangle mesh (one fan per face,
+    faces outward).  Every plane must bound the solid."""
+    normals = np.asarray(normals, np.float64)
+    offsets = np.asarray(offsets, np.float64)
+    big = 4.0 * float(np.max(np.abs(offsets))) + 1.0
+    verts, tris = [], []
+    for i, (n, c) in enumerate(zip(normals, offsets)):
+        n = n / np.linalg.norm(n)
+        a = np.array([1.0, 0.0, 0.0]) if abs(n[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+        t1 = np.cross(n, a)
+        t1 /= np.linalg.norm(t1)
+        t2 = np.cross(n, t1)
+        o = n * c
+        poly = [o + big * (sx * t1 + sy * t2) for sx, sy in ((-1, -1), (1, -1), (1, 1), (-1, 1))]
+        for j, (m, d) in enumerate(zip(normals, offsets)):
+            if j != i and poly:
+                poly = _clip(poly, m / np.linalg.norm(m), d / np.linalg.norm(m))
+        if len(poly) < 3:
+            continue
+        base = len(verts)
+        verts.extend(poly)
+        for k in range(1, len(poly) - 1):
+            tri = [base, base + k, base + k + 1]
+            v0, v1, v2 = (poly[0], poly[k], poly[k + 1])
+            if np.dot(np.cross(v1 - v0, v2 - v0), n) < 0.0:
+                tri = [base, base + k + 1, base + k]
+            tris.append(tri)
+    return np.asarray(verts, np.float32), np.asarray(tris, np.uint32)
+
+
+def brilliant(diameter=1.0, seed=0, jitter_deg=0.4):
+    """Round brilliant of girdle diameter `diameter` (mm), table up (+Y),
+    girdle centred on the origin: the convex hull of 57 planes -- table,
+    8 stars, 8 bezels, 16 upper and 16 lower girdle facets, 8 pavilion
+    mains -- with table 57 %, crown 34.5 deg, pavilion 40.75 deg, girdle
+    2 %, each facet angle jittered by up to `jitter_deg` (seeded)."""
+    rng = np.random.default_rng(seed)
+    r, g = 0.5, 0.01
+    crown, pav = np.deg2rad(34.5), np.deg2rad(40.75)
+    table = 0.285
+    h_table = g + (r - table) * np.tan(crown)
+    normals, offsets = [], []
+
+    def add(az_deg, tilt, through, up=True):
+        j = np.deg2rad(jitter_deg) * (2.0 * rng.random() - 1.0)
+        tilt = tilt + j
+        az = np.deg2rad(az_deg)
+        n = np.array([np.sin(tilt) * np.cos(az), np.cos(tilt) if up else -np.cos(tilt), np.sin(tilt) * np.sin(az)])
+        normals.append(n)
+        offsets.append(float(np.dot(n, through)))
+
+    def rim(az_deg, radius, y):
+        az = np.deg2rad(az_deg)
+        return np.array([radius * np.cos(az), y, radius * np.sin(az)])
+
+    normals.append(np.array([0.0, 1.0, 0.0]))
+    offsets.append(h_table)
+    for k in range(8):
+        a = 45.0 * k
+        add(a, crown, rim(a, r, g))                                        # bezel
+        add(a + 22.5, np.deg2rad(21.5), rim(a + 22.5, table, h_table))     # star
+        for s in (-11.25, 11.25):
+            add(a + 22.5 + s, np.deg2rad(41.0), rim(a + 22.5 + s, r, g))   # upper girdle
+            add(a + 22.5 + s, np.deg2rad(42.0), rim(a + 22.5 + s, r, -g), up=False)  # lower girdle
+        add(a, pav, rim(a, r, -g), up=False)                               # pavilion main
+    pos, tris = convex_hull_of_planes(np.asarray(normals) , np.asarray(offsets))
+    return (pos * np.float32(diameter)).astype(np.float32), tris
+
+
+def signet_stones(seed=2, n_stones=12, d_range=(2.0, 6.0), stone_seed=12, signet_kw=None):
+    """The signet with `n_stones` brilliants set in its head plate: girdle
+    diameters drawn from U[d_range] (largest placed first; a stone that finds
+    no free spot after 400 tries shrinks by 10 %), tables level with the
+    plate's top surface, no two stones overlapping.  Returns positions,
+    triangles, vertex normals (stone normals flat per facet) and per-triangle
+    material (0 metal, 1 + stone index % 2: two gem slots)."""
+    pos, tris, nrm = signet(seed, **(signet_kw or {}))
+    rng = np.random.default_rng(stone_seed)
+    ds = np.sort(rng.uniform(d_range[0], d_range[1], n_stones))[::-1]
+    ax, az, cy, ay = 7.0, 5.5, 9.0 + 1.3 + 1.2, 2.2
+    placed = []
+    P, T, N, M = [pos], [tris], [nrm], [np.zeros(len(tris), np.uint16)]
+    base = len(pos)
+    for i, d in enumerate(ds):
+        for attempt in range(100000):
+            if attempt and attempt % 400 == 0:
+                d *= 0.9
+            x, z = (2.0 * rng.random() - 1.0) * ax, (2.0 * rng.random() - 1.0) * az
+            q = (x / ax) ** 2 + (z / az) ** 2
+            rad = d / 2.0
+            # keep the girdle inside the flatter part of the plate
+            if ((abs(x) + rad) / ax) ** 2 + ((abs(z) + rad) / az) ** 2 > 0.8:
+                continue
+            if any((x - px) ** 2 + (z - pz) ** 2 < ((rad + pr) * 1.05) ** 2 for px, pz, pr in placed):
+                continue
+            break
+        placed.append((x, z, rad))
+        y = cy + ay * np.sqrt(max(1.0 - q, 0.0)) + 0.05
+        sp, st = brilliant(d, seed=stone_seed * 1000 + i)
+        sp = sp + np.array([x, y - 0.16 * d, z], np.float32)
+        P.append(sp.astype(np.float32))
+        T.append(st + np.uint32(base))
+        a, b, c = sp[st[:, 0]], sp[st[:, 1]], sp[st[:, 2]]
+        fn = np.cross(b - a, c - a)
+        fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+        vn = np.zeros_like(sp)
+        for k in range(3):
+            vn[st[:, k]] = fn
+        N.append(vn.astype(np.float32))
+        M.append(np.full(len(st), 1 + i % 2, np.uint16))
+        base += len(sp)
+    return (np.concatenate(P).astype(np.float32), np.concatenate(T).astype(np.uint32),
+            np.concatenate(N).astype(np.float32), np.concatenate(M).astype(np.uint16), [2 * p[2] for p in placed])
+
+
 def sphere_rays(n, center, radius=30.0, jitter_deg=20.0, seed=3):
     """Origins uniform on a sphere; directions toward `center`, rotated by a
     uniform angle in [0, jitter] about a random perpendicular axis."""

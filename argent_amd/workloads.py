@@ -98,5 +98,17 @@ def signet_still():
                 seed=7)
 
 
+def signet_gems_still(hdri_size=(2048, 1024)):
+    """The signet with 12 brilliants (diamond and sapphire) in its plate,
+    lit by a 2048 x 1024 HDRI (log-normal luminance sigma 2 plus three hot
+    spots) together with the white rig's softboxes (light pick 0.5 between
+    the map and the panels), 1080 x 1920, 512 spp, depth 24."""
+    pos, tris, nrm, mat, diam = synth.signet_stones(seed=2)
+    p = Piece(pos, tris, nrm, tri_material=mat)
+    w, h = hdri_size
+    return dict(piece=p, metal="sterling", studio="white", gems=("diamond", "sapphire"), width=1080, height=1920,
+                spp=512, depth=24, f_stop=0.0, seed=7, hdri=synth.hdri(w, h, seed=4), stone_diameters=diam)
+
+
 def path_settings(depth):
     return PathSettings(max_depth=depth, clamp_indirect=24.0, rr_depth=6)

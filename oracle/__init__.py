@@ -54,6 +54,10 @@ def lib():
             ("or_rng_dim", C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32]),
             ("or_camera_ray", C.c_int, [C.POINTER(A.Camera), C.c_float, C.c_float, C.c_float, C.c_float, _f, _f]),
             ("or_bsdf_probe", C.c_int, [C.c_void_p, C.c_int32, _f, C.c_float, _f, _f, _f, _f]),
+            ("or_linear_rgb", C.c_int, [_f, _f]),
+            ("or_env_build", C.c_int, [_f, C.c_int32, C.c_int32, C.c_float, _f, _f, _f, _f]),
+            ("or_env_sample", C.c_int, [_f, _f, C.c_int32, C.c_int32, C.c_float, _f, C.c_uint32, _f, _f]),
+            ("or_env_pdf", C.c_int, [_f, _f, C.c_int32, C.c_int32, C.c_float, _f, C.c_uint32, _f]),
         ]
         for name, res, args in sigs:
             fn = getattr(L, name)
@@ -171,3 +175,46 @@ def camera_matrix(white=None):
 
 def rng_dim(ps, si, dim):
     return lib().or_rng_dim(ps, si, dim)
+
+
+def linear_rgb():
+    """LinearRgb (envmap.rs:15-105): bases (3, 471) and inverse response (3, 3)."""
+    b = np.empty((3, 471), np.float32)
+    m = np.empty((3, 3), np.float32)
+    lib().or_linear_rgb(_p(b), _p(m))
+    return b, m
+
+
+def env_build(rgb, exposure_ev=0.0):
+    """EnvData::load (envmap.rs:141-178) tables for a linear-RGB image (h, w, 3)."""
+    rgb = np.ascontiguousarray(np.asarray(rgb, np.float32))
+    h, w = rgb.shape[:2]
+    weights = np.empty((h, w, 3), np.float32)
+    marg = np.empty(h + 1, np.float32)
+    cond = np.empty((h, w + 1), np.float32)
+    total = np.empty(1, np.float32)
+    lib().or_env_build(_p(rgb), w, h, exposure_ev, _p(weights), _p(marg), _p(cond), _p(total))
+    return weights, marg, cond, float(total[0])
+
+
+def env_sample(marg, cond, u, rotation_deg=0.0):
+    """EnvMap::sample (envmap.rs:224-239) for (n, 2) uniforms: directions, pdfs."""
+    h, w1 = cond.shape
+    u = np.ascontiguousarray(np.asarray(u, np.float32))
+    n = len(u)
+    dirs = np.empty((n, 3), np.float32)
+    pdfs = np.empty(n, np.float32)
+    lib().or_env_sample(_p(np.ascontiguousarray(marg)), _p(np.ascontiguousarray(cond)), w1 - 1, h, rotation_deg,
+                        _p(u), n, _p(dirs), _p(pdfs))
+    return dirs, pdfs
+
+
+def env_pdf(marg, cond, dirs, rotation_deg=0.0):
+    """EnvMap::pdf (envmap.rs:208-221) of (n, 3) directions."""
+    h, w1 = cond.shape
+    dirs = np.ascontiguousarray(np.asarray(dirs, np.float32))
+    pdfs = np.empty(len(dirs), np.float32)
+    lib().or_env_pdf(_p(np.ascontiguousarray(marg)), _p(np.ascontiguousarray(cond)), w1 - 1, h, rotation_deg,
+                     _p(dirs), len(dirs), _p(pdfs))
+    return pdfs
+

@@ -1349,6 +1349,106 @@ struct EnvMap {
     }
 };
 
+// envmap.rs:15-105: RGB -> spectrum as three smooth bases under a 6504 K
+// blackbody, corrected by the inverse of their linear-sRGB responses.
+// spectrum.rs:104-108, 125-136 (Blackbody::new, planck)
+static float planck_nm(float lambda_nm, float t) {
+    if (t <= 0.0f) return 0.0f;
+    const double c = 299792458.0, h = 6.62606957e-34, kb = 1.3806488e-23;
+    double l = (double)lambda_nm * 1e-9;
+    double l2 = l * l;
+    double l5 = l * (l2 * l2);
+    double le = (2.0 * h * c * c) / (l5 * (std::exp((h * c) / (l * kb * (double)t)) - 1.0));
+    return (float)le;
+}
+static float smoothstep_f(float e0, float e1, float x) {
+    float t = std::clamp((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+struct LinRgb {
+    float basis[3][471];
+    float inv[3][3];
+    LinRgb() {
+        const float kelvin = 6504.0f;
+        float peak = 2.897772e-3f / kelvin * 1e9f;
+        float norm = 1.0f / planck_nm(peak, kelvin);
+        for (int i = 0; i < 471; i++) {
+            float l = 360.0f + (float)i;
+            float wb = 1.0f - smoothstep_f(470.0f, 515.0f, l);
+            float wr = smoothstep_f(565.0f, 610.0f, l);
+            float wg = std::fmax(1.0f - wb - wr, 0.0f);
+            float e = planck_nm(l, kelvin) * norm;
+            basis[0][i] = wr * e;
+            basis[1][i] = wg * e;
+            basis[2][i] = wb * e;
+        }
+        float m[3][3];
+        for (int j = 0; j < 3; j++) {
+            float xyz[3] = {0.0f, 0.0f, 0.0f};
+            for (int i = 0; i < 471; i++) {
+                float cb[3];
+                xyz_bar(360.0f + (float)i, cb);
+                for (int k = 0; k < 3; k++) xyz[k] += cb[k] * basis[j][i];
+            }
+            float x = xyz[0] / CIE_Y_INTEGRAL, y = xyz[1] / CIE_Y_INTEGRAL, z = xyz[2] / CIE_Y_INTEGRAL;
+            // cie.rs:74-81
+            float rgb[3] = {3.2404542f * x - 1.5371385f * y - 0.4985314f * z,
+                            -0.969266f * x + 1.8760108f * y + 0.041556f * z,
+                            0.0556434f * x - 0.2040259f * y + 1.0572252f * z};
+            for (int i = 0; i < 3; i++) m[i][j] = rgb[i];
+        }
+        // envmap.rs:94-103
+        float a = m[0][0], b = m[0][1], c = m[0][2], d = m[1][0], e = m[1][1], f = m[1][2], g = m[2][0],
+              hh = m[2][1], ii = m[2][2];
+        float det = a * (e * ii - f * hh) - b * (d * ii - f * g) + c * (d * hh - e * g);
+        float k = 1.0f / det;
+        float r[3][3] = {{(e * ii - f * hh) * k, (c * hh - b * ii) * k, (b * f - c * e) * k},
+                         {(f * g - d * ii) * k, (a * ii - c * g) * k, (c * d - a * f) * k},
+                         {(d * hh - e * g) * k, (b * g - a * hh) * k, (a * e - b * d) * k}};
+        std::memcpy(inv, r, sizeof(inv));
+    }
+    void weights(const float rgb[3], float out[3]) const {
+        for (int i = 0; i < 3; i++) out[i] = inv[i][0] * rgb[0] + inv[i][1] * rgb[1] + inv[i][2] * rgb[2];
+    }
+};
+static const LinRgb& lin_rgb() {
+    static const LinRgb L;
+    return L;
+}
+
+// envmap.rs:141-178 (EnvData::load after decoding): basis weights per texel,
+// the luminance x sin(theta) + 1e-6 sampling function, conditional CDFs per
+// row (w+1 each) and the marginal CDF (h+1).
+static void env_build(const float* rgb, int w, int h, float exposure_ev, float* weights, float* marginal,
+                      float* conditional, float* total) {
+    float k = std::pow(2.0f, exposure_ev);
+    const LinRgb& lin = lin_rgb();
+    std::vector<float> row_sum((size_t)h);
+    for (int y = 0; y < h; y++) {
+        float s = std::sin(PI * ((float)y + 0.5f) / (float)h);
+        size_t base = (size_t)y * (size_t)(w + 1);
+        float acc = 0.0f;
+        conditional[base] = 0.0f;
+        for (int x = 0; x < w; x++) {
+            const float* p = rgb + ((size_t)y * w + x) * 3;
+            float c[3] = {p[0] * k, p[1] * k, p[2] * k};
+            lin.weights(c, weights + ((size_t)y * w + x) * 3);
+            acc += std::fmax(0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2], 0.0f) * s + 1e-6f;
+            conditional[base + x + 1] = acc;
+        }
+        row_sum[y] = acc;
+        for (int x = 0; x <= w; x++) conditional[base + x] /= acc;
+    }
+    float acc = 0.0f;
+    marginal[0] = 0.0f;
+    for (int y = 0; y < h; y++) {
+        acc += row_sum[y];
+        marginal[y + 1] = acc;
+    }
+    *total = acc / ((float)w * (float)h);
+    for (int y = 0; y <= h; y++) marginal[y] /= acc;
+}
+
 // ------------------------------------------------------------------ scene.rs
 enum HitKind { H_PIECE, H_PANEL, H_FLOOR };
 struct Hit {
@@ -1970,3 +2070,42 @@ extern "C" int or_bsdf_probe(const or_scene* s, int32_t m, const float* lam4, fl
     out[17] = b.is_delta() ? 1.0f : 0.0f;
     return 0;
 }
+
+// ------------------------------------------------------------------ environment map probes
+extern "C" int or_linear_rgb(float* basis3x471, float* inv9) {
+    const orc::LinRgb& L = orc::lin_rgb();
+    std::memcpy(basis3x471, L.basis, sizeof(L.basis));
+    std::memcpy(inv9, L.inv, sizeof(L.inv));
+    return 0;
+}
+
+extern "C" int or_env_build(const float* rgb, int32_t w, int32_t h, float exposure_ev, float* weights,
+                            float* marginal, float* conditional, float* total) {
+    if (!rgb || w <= 0 || h <= 0) return 1;
+    orc::env_build(rgb, w, h, exposure_ev, weights, marginal, conditional, total);
+    return 0;
+}
+
+// EnvMap::sample / pdf (envmap.rs:206-239) over the given tables: n draws
+// from (u1, u2) pairs; out: dir (3) and pdf per draw.  With dirs_only, the
+// pdf of the given directions instead.
+extern "C" int or_env_sample(const float* marginal, const float* conditional, int32_t w, int32_t h, float rotation_deg,
+                             const float* u, uint32_t n, float* dirs, float* pdfs) {
+    orc::EnvMap E{w, h, nullptr, marginal, conditional, rotation_deg, nullptr};
+    for (uint32_t i = 0; i < n; i++) {
+        float p = 0.0f;
+        orc::V3 d = E.sample(u[2 * i], u[2 * i + 1], &p);
+        dirs[3 * i] = d.x;
+        dirs[3 * i + 1] = d.y;
+        dirs[3 * i + 2] = d.z;
+        pdfs[i] = p;
+    }
+    return 0;
+}
+extern "C" int or_env_pdf(const float* marginal, const float* conditional, int32_t w, int32_t h, float rotation_deg,
+                          const float* dirs, uint32_t n, float* pdfs) {
+    orc::EnvMap E{w, h, nullptr, marginal, conditional, rotation_deg, nullptr};
+    for (uint32_t i = 0; i < n; i++) pdfs[i] = E.pdf(orc::V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    return 0;
+}
+

@@ -68,6 +68,17 @@ int or_camera_ray(const ag_camera* cam, float sx, float sy, float lu, float lv, 
 int or_bsdf_probe(const or_scene* s, int32_t m, const float* lam4, float mask, const float* u4, const float* wo3,
                   const float* wi3, float* out18);
 
+/* Environment map (envmap.rs): the LinearRgb bases (3 x 471) and inverse
+ * response (3 x 3); EnvData::load's tables from linear RGB (w*h*3);
+ * EnvMap::sample for n (u1, u2) pairs and EnvMap::pdf for n directions. */
+int or_linear_rgb(float* basis3x471, float* inv9);
+int or_env_build(const float* rgb, int32_t w, int32_t h, float exposure_ev, float* weights, float* marginal,
+                 float* conditional, float* total);
+int or_env_sample(const float* marginal, const float* conditional, int32_t w, int32_t h, float rotation_deg,
+                  const float* u, uint32_t n, float* dirs, float* pdfs);
+int or_env_pdf(const float* marginal, const float* conditional, int32_t w, int32_t h, float rotation_deg,
+               const float* dirs, uint32_t n, float* pdfs);
+
 #ifdef __cplusplus
 }
 #endif

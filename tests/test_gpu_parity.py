@@ -249,6 +249,25 @@ def test_signet_white_render_matches_oracle(ctx):
     assert e < 0.01
 
 
+def test_gems_hdri_render_matches_oracle(ctx):
+    """Brilliants (diamond: Sellmeier dispersion; sapphire: absorption bands)
+    set in the signet, lit by an importance-sampled HDRI together with the
+    white rig's softboxes (light pick 0.5), depth 24."""
+    pos, tris, nrm, mat, _ = synth.signet_stones(
+        seed=2, signet_kw=dict(band_nu=256, band_nv=64, plate_nt=100, plate_np=200))
+    piece = workloads.Piece(pos, tris, nrm, tri_material=mat)
+    desc = piece.scene("sterling", 0.07, 0.8, "white", gems=("diamond", "sapphire"))
+    desc.set_env_map(synth.hdri(256, 128, seed=4), rotation_deg=30.0)
+    n_l, _, _, env_pick = desc.light_cdf()
+    assert n_l > 0 and env_pick == 0.5
+    cam, turn = piece.shot("top", 100.0, 0.0, 1.0)
+    gpu, ref = _render_both(ctx, desc, cam, 128, 128, workloads.path_settings(24), 32, 7, turn)
+    e = rel_rmse(gpu[0], ref[0])
+    print(f"relative RMSE {e:.2e}")
+    assert e < 0.01
+    assert rel_rmse(gpu[1], ref[1]) < 0.01 and rel_rmse(gpu[2], ref[2]) < 0.01
+
+
 def sphere(r, n):
     """furnace.rs:19-44"""
     i = np.arange(n + 1, dtype=np.float32)

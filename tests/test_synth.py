@@ -49,3 +49,52 @@ def test_binary_stl_round_trip():
 def test_hdri_shape():
     img = synth.hdri(256, 128, seed=4)
     assert img.shape == (128, 256, 3) and np.all(img > 0) and img.max() > 20 * np.median(img)
+
+
+def test_brilliant_is_a_closed_convex_57_facet_solid():
+    p, t = synth.brilliant(4.0, seed=3)
+    a, b, c = (p[t[:, k]].astype(np.float64) for k in range(3))
+    n = np.cross(b - a, c - a)
+    # one plane per facet: group triangle normals
+    un = n / np.linalg.norm(n, axis=1, keepdims=True)
+    planes = []
+    for k in range(len(un)):
+        if not any(un[k] @ q[:3] > 0.99999 for q in planes):
+            planes.append(np.r_[un[k], un[k] @ a[k]])
+    assert len(planes) == 57
+    centre = p.mean(axis=0)
+    assert np.all(np.einsum("ij,ij->i", n, (a + b + c) / 3 - centre) > 0)  # outward
+    # closed: every edge is shared by exactly two triangles (welded by position)
+    key = {tuple(np.round(v, 5)): i for i, v in enumerate(p)}
+    vid = np.array([key[tuple(np.round(v, 5))] for v in p])
+    edges = {}
+    for tri in vid[t]:
+        for e in ((tri[0], tri[1]), (tri[1], tri[2]), (tri[2], tri[0])):
+            k = tuple(sorted(e))
+            edges[k] = edges.get(k, 0) + 1
+    assert all(v == 2 for v in edges.values())
+    # convex: every vertex on or behind every facet plane
+    for pl in planes:
+        assert np.all(p @ pl[:3] <= pl[3] + 1e-3)
+    # facets pass through the nominal girdle circle at their midlines; where
+    # neighbours meet the outline lies a few per cent outside it
+    girdle = 2 * np.max(np.linalg.norm(p[:, [0, 2]], axis=1))
+    assert 4.0 <= girdle < 4.0 * 1.06
+
+
+def test_signet_stones_do_not_overlap():
+    pos, tris, nrm, mat, diam = synth.signet_stones(
+        seed=2, signet_kw=dict(band_nu=64, band_nv=16, plate_nt=20, plate_np=40))
+    assert len(diam) == 12 and set(np.unique(mat)) == {0, 1, 2}
+    assert len(mat) == len(tris) and np.allclose(np.linalg.norm(nrm, axis=1), 1.0, atol=1e-5)
+    stones = [tris[mat == m] for m in (1, 2)]
+    centres = []
+    for m in (1, 2):
+        sel = tris[mat == m]
+        for k in range(0, len(sel), 216):
+            v = pos[sel[k:k + 216].ravel()]
+            centres.append((v[:, [0, 2]].mean(axis=0), (v[:, 0].max() - v[:, 0].min()) / 2))
+    for i in range(len(centres)):
+        for j in range(i + 1, len(centres)):
+            (ci, ri), (cj, rj) = centres[i], centres[j]
+            assert np.linalg.norm(ci - cj) >= (ri + rj) * 0.99
