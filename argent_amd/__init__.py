@@ -143,6 +143,15 @@ class Context:
         check(A.lib().ag_ctx_stats(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in A.Stats._fields_}
 
+    def depth_stats(self, n=64):
+        """Per bounce depth of the last render: extension and shadow rays, and
+        (timing enabled) ms in the extend, shade and shadow kernels."""
+        e, sh = (C.c_uint64 * n)(), (C.c_uint64 * n)()
+        me, ms, mh = (C.c_double * n)(), (C.c_double * n)(), (C.c_double * n)()
+        check(A.lib().ag_ctx_depth_stats(self.h, n, e, sh, me, ms, mh))
+        return {"extension_rays": list(e), "shadow_rays": list(sh), "ms_extend": list(me), "ms_shade": list(ms),
+                "ms_shadow": list(mh)}
+
 
 class SceneDesc:
     """Host-side scene description (Scene::new inputs, scene.rs:82-86).
@@ -253,12 +262,19 @@ class SceneDesc:
 class Scene:
     """Device scene (ag_scene): BVH, mesh attributes, tables resident in HBM."""
 
-    def __init__(self, ctx, desc):
+    def __init__(self, ctx, desc, share_mesh_with=None):
+        """With `share_mesh_with` (a Scene of the same mesh), only the look --
+        materials, panels, environment, floor -- comes from `desc`; the mesh
+        and BVH on the device are shared (ag_scene_create_shared)."""
         self.ctx = ctx
         self.desc = desc
+        self.base = share_mesh_with
         h = C.c_void_p()
         desc._bind()
-        check(A.lib().ag_scene_create(ctx.h, C.byref(desc.d), C.byref(h)))
+        if share_mesh_with is None:
+            check(A.lib().ag_scene_create(ctx.h, C.byref(desc.d), C.byref(h)))
+        else:
+            check(A.lib().ag_scene_create_shared(ctx.h, share_mesh_with.h, C.byref(desc.d), C.byref(h)))
         self.h = h
 
     def close(self):

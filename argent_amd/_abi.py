@@ -98,7 +98,8 @@ REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_REJECTED = range(4)
 
 class Tuning(C.Structure):
     _fields_ = [("paths_per_wave", C.c_uint32), ("block_traverse", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("sort_by_material", C.c_uint32)]
+                ("blocks_per_cu", C.c_uint32), ("sort_by_material", C.c_uint32),
+                ("refill_min", C.c_uint32), ("batch_blocks_per_cu", C.c_uint32), ("batch_refill_min", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -144,6 +145,7 @@ _SIGS = [
     ("ag_camera_frame", C.c_int, [f32p, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                   C.c_float, C.POINTER(Camera)]),
     ("ag_scene_create", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    ("ag_scene_create_shared", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     ("ag_scene_destroy", None, [C.c_void_p]),
     ("ag_scene_set_turntable", C.c_int, [C.c_void_p, C.c_float]),
     ("ag_scene_set_material", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Material)]),
@@ -182,6 +184,8 @@ _SIGS = [
     ("ag_ctx_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("ag_ctx_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("ag_ctx_enable_timing", C.c_int, [C.c_void_p, C.c_int32]),
+    ("ag_ctx_depth_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

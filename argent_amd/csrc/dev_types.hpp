@@ -18,7 +18,7 @@ constexpr int TRAV_BLOCK = 256;                  // threads per traversal block
 #define AG_STACK_LDS 8
 #endif
 constexpr int STACK_LDS = AG_STACK_LDS;          // stack entries per lane in LDS
-constexpr int STACK_SPILL = 64;                  // further entries per thread in HBM
+constexpr int STACK_SPILL = 96;                  // further entries per thread in HBM
 constexpr int MAX_SEGMENTS = 4096;               // work segments per wave
 
 // Hit codes written by the extension kernel.
@@ -148,5 +148,6 @@ constexpr int CURSOR_SLICES = 8;
 constexpr int CURSOR_STRIDE = 32;  // u32 per cursor (128 B)
 constexpr int CURSOR_SET = CURSOR_SLICES * CURSOR_STRIDE;
 constexpr int CURSORS_PER_DEPTH = 2 * CURSOR_SET;
+constexpr int TALLY_DEPTHS = 64;  // per-depth ray counts kept for statistics
 
 }  // namespace ag

@@ -1,9 +1,11 @@
 // Host BVH build for the device traversal kernels: binned SAH over all three
-// axes (32 bins, parallel above 16K primitives), collapsed to 4-wide nodes
-// by repeatedly opening the child with the largest surface area (the policy
-// of the reference's collapse, bvh.rs:303-378), laid out for 16-byte vector
-// loads.  Topology differs from the reference's BVH; closest-hit results do
-// not depend on it.
+// axes (32 bins, parallel above 16K primitives, a node-traversal cost equal to
+// one triangle test so leaves keep up to four triangles), collapsed to 8-wide
+// nodes by repeatedly opening the child with the largest surface area (the
+// policy of the reference's 4-wide collapse, bvh.rs:303-378), with child
+// boxes quantized to 8 bits per plane against the node's box (conservative),
+// laid out for 16-byte vector loads.  Topology differs from the reference's
+// BVH; closest-hit results do not depend on it.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -125,6 +127,8 @@ struct Builder {
                 }
             }
         }
+        // SAH with a traversal cost of one triangle test: C = A + A_l N_l + A_r N_r
+        best_cost += bounds.area();
         float leaf_cost = bounds.area() * (float)n;
         if (n <= (uint32_t)BVH_MAX_LEAF && (best_axis < 0 || best_cost >= leaf_cost)) {
             make_leaf();
@@ -163,16 +167,44 @@ struct Builder {
 
 }  // namespace
 
+void quantize_axis(float p, float s, float lo, float hi, uint8_t* qlo, uint8_t* qhi) {
+    float fl = std::floor((lo - p) / s), fh = std::ceil((hi - p) / s);
+    int l = (int)std::clamp(fl, 0.0f, 255.0f), h = (int)std::clamp(fh, 0.0f, 255.0f);
+    while (l > 0 && std::fma((float)l, s, p) > lo) l--;
+    while (h < 255 && std::fma((float)h, s, p) < hi) h++;
+    *qlo = (uint8_t)l;
+    *qhi = (uint8_t)h;
+}
+
+// Exponent byte of the smallest power-of-two scale s with p + 255 s >= hi in
+// float fma arithmetic.
+static uint8_t axis_exponent(float p, float hi) {
+    float ext = hi - p;
+    int e = 1;
+    if (ext > 0.0f) {
+        int ex;
+        std::frexp(ext / 255.0f, &ex);  // ext / 255 < 2^ex
+        e = std::clamp(ex + 127, 1, 254);
+    }
+    for (;; e++) {
+        float s;
+        uint32_t bits = (uint32_t)e << 23;
+        std::memcpy(&s, &bits, 4);
+        if (std::fma(255.0f, s, p) >= hi || e >= 254) return (uint8_t)e;
+    }
+}
+
 void build_wide_bvh(const float* pos, const uint32_t* tri, uint32_t n_tris, WideBvh* out) {
     out->nodes.clear();
     out->tris.clear();
-    out->n_nodes = out->n_leaves = out->depth = 0;
+    out->n_nodes = out->n_leaves = out->depth = out->max_stack = 0;
     if (n_tris == 0) {
-        // one node with four empty slots: every query misses
-        out->nodes.assign(32, 0.0f);
-        uint32_t* ch = reinterpret_cast<uint32_t*>(&out->nodes[24]);
-        for (int c = 0; c < 4; c++) ch[c] = BVH_EMPTY;
+        // one node without children: every query misses
+        out->nodes.assign(BVH_NODE_FLOATS, 0.0f);
+        uint32_t w = 127u | 127u << 8 | 127u << 16;
+        std::memcpy(&out->nodes[3], &w, 4);
         out->n_nodes = 1;
+        out->depth = 1;
         return;
     }
     std::vector<Prim> prims(n_tris);
@@ -191,33 +223,28 @@ void build_wide_bvh(const float* pos, const uint32_t* tri, uint32_t n_tris, Wide
         out->lo[a] = bn[0].b.lo[a];
         out->hi[a] = bn[0].b.hi[a];
     }
-    // triangles in leaf order
-    out->tris.resize((size_t)n_tris * 12);
-    for (uint32_t i = 0; i < n_tris; i++) {
-        uint32_t id = prims[i].id;
-        const float* p0 = pos + 3 * (size_t)tri[3 * (size_t)id + 0];
-        const float* p1 = pos + 3 * (size_t)tri[3 * (size_t)id + 1];
-        const float* p2 = pos + 3 * (size_t)tri[3 * (size_t)id + 2];
-        float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
-        float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
-        float* t = &out->tris[(size_t)i * 12];
-        t[0] = p0[0]; t[1] = p0[1]; t[2] = p0[2]; t[3] = e1[0];
-        t[4] = e1[1]; t[5] = e1[2]; t[6] = e2[0]; t[7] = e2[1];
-        t[8] = e2[2];
-        std::memcpy(&t[9], &id, 4);
-        t[10] = 0.0f; t[11] = 0.0f;
-    }
     auto area = [](const BNode& n) {
         float e[3];
         for (int a = 0; a < 3; a++) e[a] = n.b.hi[a] - n.b.lo[a];
         return e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
     };
-    // collapse, depth first
+    auto put_tri = [&](uint32_t id) {
+        const float* p0 = pos + 3 * (size_t)tri[3 * (size_t)id + 0];
+        const float* p1 = pos + 3 * (size_t)tri[3 * (size_t)id + 1];
+        const float* p2 = pos + 3 * (size_t)tri[3 * (size_t)id + 2];
+        float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        float t[12] = {p0[0], p0[1], p0[2], e1[0], e1[1], e1[2], e2[0], e2[1], e2[2], 0.0f, 0.0f, 0.0f};
+        std::memcpy(&t[9], &id, 4);
+        out->tris.insert(out->tris.end(), t, t + 12);
+    };
     std::vector<float>& W = out->nodes;
-    uint32_t max_depth = 0, leaves = 0;
-    std::function<uint32_t(int, uint32_t)> emit = [&](int root, uint32_t depth) -> uint32_t {
-        uint32_t me = (uint32_t)(W.size() / 32);
-        W.resize(W.size() + 32, 0.0f);
+    out->tris.reserve((size_t)n_tris * 12);
+    uint32_t max_depth = 0, leaves = 0, max_stack = 0;
+    // fill node `me` from binary subtree `root`; inner children get consecutive
+    // slots allocated here, then are filled depth first
+    std::function<void(uint32_t, int, uint32_t, uint32_t)> fill = [&](uint32_t me, int root, uint32_t depth,
+                                                                      uint32_t stack) {
         max_depth = std::max(max_depth, depth + 1);
         std::vector<int> kids;
         if (bn[root].left < 0)
@@ -226,7 +253,7 @@ void build_wide_bvh(const float* pos, const uint32_t* tri, uint32_t n_tris, Wide
             kids.push_back(bn[root].left);
             kids.push_back(bn[root].right);
         }
-        while (kids.size() < 4) {
+        while (kids.size() < (size_t)BVH_WIDTH) {
             int best = -1;
             float ba = -1.0f;
             for (size_t i = 0; i < kids.size(); i++) {
@@ -246,38 +273,57 @@ void build_wide_bvh(const float* pos, const uint32_t* tri, uint32_t n_tris, Wide
             kids.push_back(bn[k].left);
             kids.push_back(bn[k].right);
         }
-        uint32_t child[4] = {BVH_EMPTY, BVH_EMPTY, BVH_EMPTY, BVH_EMPTY};
-        float box[6][4];
-        for (int c = 0; c < 4; c++) {
-            for (int a = 0; a < 3; a++) {
-                box[a][c] = std::numeric_limits<float>::infinity();
-                box[3 + a][c] = -std::numeric_limits<float>::infinity();
-            }
+        Box nb;
+        for (int k : kids) nb.grow(bn[k].b);
+        float* nd = &W[(size_t)me * BVH_NODE_FLOATS];
+        uint8_t e[3];
+        for (int a = 0; a < 3; a++) {
+            nd[a] = nb.lo[a];
+            e[a] = axis_exponent(nb.lo[a], nb.hi[a]);
         }
+        uint32_t child_base = (uint32_t)(W.size() / BVH_NODE_FLOATS);
+        uint32_t tri_base = (uint32_t)(out->tris.size() / 12);
+        uint32_t imask = 0, n_inner = 0, toff = 0;
+        uint8_t meta[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        uint8_t q[6][8];
+        std::memset(q, 0, sizeof(q));
         for (size_t c = 0; c < kids.size(); c++) {
             const BNode& n = bn[kids[c]];
             for (int a = 0; a < 3; a++) {
-                box[a][c] = n.b.lo[a];
-                box[3 + a][c] = n.b.hi[a];
+                float s;
+                uint32_t bits = (uint32_t)e[a] << 23;
+                std::memcpy(&s, &bits, 4);
+                quantize_axis(nb.lo[a], s, n.b.lo[a], n.b.hi[a], &q[a][c], &q[3 + a][c]);
             }
-            if (n.left < 0) {
-                child[c] = BVH_LEAF | ((n.count - 1) << BVH_COUNT_SHIFT) | n.first;
+            if (n.left >= 0) {
+                imask |= 1u << c;
+                meta[c] = (uint8_t)n_inner++;
+            } else {
+                meta[c] = (uint8_t)(n.count << 5 | toff);
+                for (uint32_t i = 0; i < n.count; i++) put_tri(prims[n.first + i].id);
+                toff += n.count;
                 leaves++;
             }
         }
-        for (size_t c = 0; c < kids.size(); c++) {
-            if (bn[kids[c]].left >= 0) child[c] = emit(kids[c], depth + 1);
-        }
-        float* nd = &W[(size_t)me * 32];
-        for (int r = 0; r < 6; r++)
-            for (int c = 0; c < 4; c++) nd[r * 4 + c] = box[r][c];
-        std::memcpy(&nd[24], child, 16);
-        return me;
+        uint32_t w = (uint32_t)e[0] | (uint32_t)e[1] << 8 | (uint32_t)e[2] << 16 | imask << 24;
+        std::memcpy(&nd[3], &w, 4);
+        uint32_t hdr[4] = {child_base, tri_base, 0, 0};
+        std::memcpy(&hdr[2], meta, 8);
+        std::memcpy(&nd[4], hdr, 16);
+        std::memcpy(&nd[8], q, 48);
+        // a traversal holds at most `stack` entries here plus all other children
+        max_stack = std::max(max_stack, stack + (uint32_t)kids.size() - 1);
+        W.resize(W.size() + (size_t)n_inner * BVH_NODE_FLOATS, 0.0f);
+        uint32_t r = 0;
+        for (size_t c = 0; c < kids.size(); c++)
+            if (bn[kids[c]].left >= 0) fill(child_base + r++, kids[c], depth + 1, stack + (uint32_t)kids.size() - 1);
     };
-    emit(0, 0);
-    out->n_nodes = (uint32_t)(W.size() / 32);
+    W.assign(BVH_NODE_FLOATS, 0.0f);
+    fill(0, 0, 0, 0);
+    out->n_nodes = (uint32_t)(W.size() / BVH_NODE_FLOATS);
     out->n_leaves = leaves;
     out->depth = max_depth;
+    out->max_stack = max_stack;
 }
 
 }  // namespace ag

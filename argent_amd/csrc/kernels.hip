@@ -53,7 +53,7 @@ AGD uint32_t wave_append_keyed(uint32_t* counters, int key) {
 }
 
 // ------------------------------------------------------------------ BVH traversal
-// Bvh::intersect / Bvh::occluded (bvh.rs:159-245) over the 4-wide device BVH.
+// Bvh::intersect / Bvh::occluded (bvh.rs:159-245) over the compressed 8-wide device BVH.
 // Per-lane stack: STACK_LDS entries in LDS, the rest in a per-thread HBM spill
 // area.  Box tests are widened by a relative 2^-16 on both ends: enough for
 // the box test's own rounding and for the f32 Moller-Trumbore hits that land
@@ -94,23 +94,45 @@ AGD void st_pop(const TravStack& s, int& sp, uint32_t& e, float& t) {
 constexpr float BOX_LO = 1.0f - 1.52587890625e-5f;  // 1 - 2^-16
 constexpr float BOX_HI = 1.0f + 1.52587890625e-5f;
 
-AGD void slab4(const float4* nd, f3 o, f3 inv, float tmax, float tmin_out[4], uint32_t ch[4]) {
-    float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
-    uint4 c = *reinterpret_cast<const uint4*>(&nd[6]);
-    ch[0] = c.x; ch[1] = c.y; ch[2] = c.z; ch[3] = c.w;
-    float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
-    float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+// Node layout (80 B): see the host builder.  Children's boxes are
+// p + q * 2^(e-127) per plane (an exact fma: the quantized box contains the
+// exact one as evaluated here); slab test as bvh.rs:280-299 on each.
+constexpr int NODE_F4 = 5;
+
+AGD void slab8(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float tm[8], uint32_t ch[8]) {
+    float4 n0 = nd[0];
+    uint4 n1 = *reinterpret_cast<const uint4*>(&nd[1]);
+    uint4 qa = *reinterpret_cast<const uint4*>(&nd[2]);
+    uint4 qb = *reinterpret_cast<const uint4*>(&nd[3]);
+    uint4 qc = *reinterpret_cast<const uint4*>(&nd[4]);
+    uint32_t w = __float_as_uint(n0.w);
+    float sx = __uint_as_float((w & 0xFFu) << 23);
+    float sy = __uint_as_float(((w >> 8) & 0xFFu) << 23);
+    float sz = __uint_as_float(((w >> 16) & 0xFFu) << 23);
+    uint32_t imask = w >> 24;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        float tx0 = (lxa[k] - o.x) * inv.x, tx1 = (hxa[k] - o.x) * inv.x;
-        float ty0 = (lya[k] - o.y) * inv.y, ty1 = (hya[k] - o.y) * inv.y;
-        float tz0 = (lza[k] - o.z) * inv.z, tz1 = (hza[k] - o.z) * inv.z;
+    for (int k = 0; k < 8; k++) {
+        const int sh = (k & 3) * 8;
+        const bool hiw = k >= 4;
+        uint32_t m = ((hiw ? n1.w : n1.z) >> sh) & 0xFFu;
+        float lx = fmaf((float)(((hiw ? qa.y : qa.x) >> sh) & 0xFFu), sx, n0.x);
+        float ly = fmaf((float)(((hiw ? qa.w : qa.z) >> sh) & 0xFFu), sy, n0.y);
+        float lz = fmaf((float)(((hiw ? qb.y : qb.x) >> sh) & 0xFFu), sz, n0.z);
+        float hx = fmaf((float)(((hiw ? qb.w : qb.z) >> sh) & 0xFFu), sx, n0.x);
+        float hy = fmaf((float)(((hiw ? qc.y : qc.x) >> sh) & 0xFFu), sy, n0.y);
+        float hz = fmaf((float)(((hiw ? qc.w : qc.z) >> sh) & 0xFFu), sz, n0.z);
+        bool inner = (imask >> k) & 1u;
+        uint32_t cnt = m >> 5;
+        ch[k] = inner ? n1.x + m : (0x80000000u | ((cnt - 1u) << 27) | (n1.y + (m & 31u)));
+        float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
+        float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
+        float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
         float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         float tmx = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
         tmin *= BOX_LO;
         tmx = fminf(tmx * BOX_HI, tmax);
-        bool hit = ch[k] != 0xFFFFFFFFu && tmin <= tmx;
-        tmin_out[k] = hit ? tmin : INF_F;
+        bool hit = (inner || cnt != 0u) && tmin <= tmx;
+        tm[k] = hit ? tmin : INF_F;
     }
 }
 
@@ -140,23 +162,19 @@ AGD float tri_test(const float4* tp, f3 o, f3 d, float* uo, float* vo, uint32_t*
 // is cut into CURSOR_SLICES slices, one per XCD (blocks are dealt to the XCDs
 // round-robin, so block b starts on slice b % 8), each with its own cursor;
 // a wavefront takes GRAB rays from a slice per atomic and hands them to its
-// idle lanes whenever at least REFILL_MIN lanes are idle, moving on to the
+// idle lanes whenever at least refill_min lanes are idle, moving on to the
 // next slice when its own is spent.  Every busy lane advances by one inner
 // node or one leaf per iteration, and the lanes that finished are handed to
 // the job together.  A lane's ray therefore never waits for the slowest ray
 // of a static batch of 64, and the atomics stay few and spread over lines.
 constexpr uint32_t NO_RAY = 0xFFFFFFFFu;
 constexpr uint32_t TRAV_DONE = 0xFFFFFFFEu;  // never an entry: leaf entries stay below 0xA0000000
-#ifndef AG_REFILL_MIN
-#define AG_REFILL_MIN 16
-#endif
 #ifndef AG_GRAB
 #define AG_GRAB 64
 #endif
 #ifndef AG_TRAV_WAVES_PER_EU
-#define AG_TRAV_WAVES_PER_EU 8
+#define AG_TRAV_WAVES_PER_EU 0
 #endif
-constexpr uint32_t REFILL_MIN = AG_REFILL_MIN;  // idle lanes that trigger a refill (64: whole batches)
 constexpr uint32_t GRAB = AG_GRAB;              // rays taken per cursor atomic
 
 AGD uint32_t lane_prefix(uint64_t mask) {
@@ -205,6 +223,7 @@ struct TravLane {
     uint32_t prim;  // closest hit: original triangle or NO_RAY; any hit: 1 once occluded
     uint32_t cur;   // entry to visit next, or TRAV_DONE
     int sp;
+    uint32_t aux;   // the job's own (view index)
 };
 
 AGD void lane_begin(TravLane& L, const float4* nodes, const float4* tris, f3 o, f3 d, float tmax) {
@@ -227,20 +246,23 @@ AGD void lane_begin(TravLane& L, const float4* nodes, const float4* tris, f3 o, 
 // entry not beyond the best hit, or done.
 AGD void step_closest(TravLane& L, const TravStack& st) {
     if (!(L.cur & 0x80000000u)) {
-        float tm[4];
-        uint32_t ch[4];
-        slab4(L.nodes + (size_t)L.cur * 8, L.o, L.inv, L.t, tm, ch);
+        float tm[8];
+        uint32_t ch[8];
+        slab8(L.nodes + (size_t)L.cur * NODE_F4, L.o, L.inv, L.t, tm, ch);
+        // ascending t: the 19-comparator sorting network for 8 keys
 #define CSWAP(i, j)                                       \
     if (tm[j] < tm[i]) {                                  \
         float tt = tm[i]; tm[i] = tm[j]; tm[j] = tt;      \
         uint32_t cc = ch[i]; ch[i] = ch[j]; ch[j] = cc;   \
     }
-        CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
+        CSWAP(0, 2) CSWAP(1, 3) CSWAP(4, 6) CSWAP(5, 7) CSWAP(0, 4) CSWAP(1, 5) CSWAP(2, 6) CSWAP(3, 7)
+        CSWAP(0, 1) CSWAP(2, 3) CSWAP(4, 5) CSWAP(6, 7) CSWAP(2, 4) CSWAP(3, 5) CSWAP(1, 4) CSWAP(3, 6)
+        CSWAP(1, 2) CSWAP(3, 4) CSWAP(5, 6)
 #undef CSWAP
         if (tm[0] != INF_F) {
-            if (tm[3] != INF_F) st_push(st, L.sp, ch[3], tm[3]);
-            if (tm[2] != INF_F) st_push(st, L.sp, ch[2], tm[2]);
-            if (tm[1] != INF_F) st_push(st, L.sp, ch[1], tm[1]);
+#pragma unroll
+            for (int k = 7; k >= 1; k--)
+                if (tm[k] != INF_F) st_push(st, L.sp, ch[k], tm[k]);
             L.cur = ch[0];
             return;
         }
@@ -275,21 +297,24 @@ AGD void step_closest(TravLane& L, const TravStack& st) {
 // triangle hit in (1e-7, t_max).
 AGD void step_any(TravLane& L, const TravStack& st) {
     if (!(L.cur & 0x80000000u)) {
-        float tm[4];
-        uint32_t ch[4];
-        slab4(L.nodes + (size_t)L.cur * 8, L.o, L.inv, L.t, tm, ch);
-        int first = -1;
+        float tm[8];
+        uint32_t ch[8];
+        slab8(L.nodes + (size_t)L.cur * NODE_F4, L.o, L.inv, L.t, tm, ch);
+        bool any = false;
+        uint32_t next = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < 8; k++) {
             if (tm[k] != INF_F) {
-                if (first < 0)
-                    first = k;
-                else
+                if (!any) {
+                    any = true;
+                    next = ch[k];
+                } else {
                     st_push(st, L.sp, ch[k], tm[k]);
+                }
             }
         }
-        if (first >= 0) {
-            L.cur = ch[first];
+        if (any) {
+            L.cur = next;
             return;
         }
     } else {
@@ -319,7 +344,7 @@ AGD void step_any(TravLane& L, const TravStack& st) {
 // is called by every lane of the wavefront, `mine` where ray i just finished;
 // void drain() once by every lane after the last ray.
 template <bool ANY, class Job>
-AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, const TravStack& st) {
+AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t refill_min, const TravStack& st) {
     TravLane L;
     L.cur = TRAV_DONE;
     L.sp = 0;
@@ -330,7 +355,7 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, const TravSt
         if (more) {
             uint64_t idle = __ballot(idx == NO_RAY);
             uint32_t cnt = (uint32_t)__popcll(idle);
-            if (cnt >= REFILL_MIN) {
+            if (cnt >= refill_min) {
                 uint32_t first = 0;
                 uint32_t k = supply_take(R, cnt, &first);
                 if (k == 0) {
@@ -408,19 +433,19 @@ struct BatchJob {
 __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_rays_intersect(const float4* __restrict__ nodes,
                                                                const float4* __restrict__ tris, RaysSoA rays,
                                                                HitsSoA hits, uint32_t n, uint32_t* cursor,
-                                                               uint2* spill) {
+                                                               uint32_t refill, uint2* spill) {
     TRAV_STACK_DECL
     BatchJob J{nodes, tris, rays, hits, nullptr};
-    traverse_persistent<false>(J, n, cursor, st);
+    traverse_persistent<false>(J, n, cursor, refill, st);
 }
 
 __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_rays_occluded(const float4* __restrict__ nodes,
                                                               const float4* __restrict__ tris, RaysSoA rays,
                                                               uint8_t* out, uint32_t n, uint32_t* cursor,
-                                                              uint2* spill) {
+                                                              uint32_t refill, uint2* spill) {
     TRAV_STACK_DECL
     BatchJob J{nodes, tris, rays, HitsSoA{}, out};
-    traverse_persistent<true>(J, n, cursor, st);
+    traverse_persistent<true>(J, n, cursor, refill, st);
 }
 
 // ------------------------------------------------------------------ scene pieces
@@ -745,55 +770,58 @@ struct ExtendJob {
     const float4* rd;
     uint32_t* stage;  // this wavefront's STAGE entries
     uint32_t staged;  // uniform
+    // Panels and floor first (world space, strict <: panels in index order,
+    // then the floor), then the piece with t_max just above their nearest
+    // hit, so the piece wins ties as it does when tested first.  Until the
+    // piece is hit, L.u / L.v hold the panel-or-floor code and distance.
     AGD bool begin(uint32_t i, TravLane& L) {
         float4 a = ro[i], b = rd[i];
-        const DView& V = views[P.view[__float_as_uint(a.w)]];
+        uint32_t vi = P.view[__float_as_uint(a.w)];
+        const DView& V = views[vi];
         const DScene& S = *V.scene;
         f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+        float tmax = INF_F;
+        uint32_t code = HIT_MISS;
+        for (int k = 0; k < S.n_panels; k++) {
+            if (depth == 0 && !S.panel[k].camera_visible) continue;
+            float t;
+            if (panel_intersect(S.panel[k], o, d, tmax, &t)) {
+                tmax = t;
+                code = HIT_PANEL | (uint32_t)k;
+            }
+        }
+        if (S.floor_present) {
+            float t;
+            if (floor_intersect(S, o, d, tmax, &t)) {
+                tmax = t;
+                code = HIT_FLOOR;
+            }
+        }
         if (V.turn_deg != 0.0f) {
             o = rot_y(o, V.tsn, V.tcn);
             d = rot_y(d, V.tsn, V.tcn);
         }
-        lane_begin(L, S.nodes, S.tris, o, d, INF_F);
+        // next float above tmax (> 0 here: panel t > 1e-6, floor t > 1e-5)
+        lane_begin(L, S.nodes, S.tris, o, d, tmax < INF_F ? __uint_as_float(__float_as_uint(tmax) + 1u) : INF_F);
+        L.u = __uint_as_float(code);
+        L.v = tmax;
+        L.aux = vi;
         return true;
     }
     AGD void finish(uint32_t i, const TravLane& L, bool mine) {
         int key = -1;
         if (mine) {
-            float4 a = ro[i], b = rd[i];
-            const DView& V = views[P.view[__float_as_uint(a.w)]];
-            const DScene& S = *V.scene;
-            f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
-            float tmax = INF_F;
-            uint32_t code = HIT_MISS;
+            const DScene& S = *views[L.aux].scene;
+            float4 h;
             if (L.prim != NO_RAY) {
-                tmax = L.t;
-                code = L.prim;
+                h = make_float4(L.t, L.u, L.v, __uint_as_float(L.prim));
+                key = 2 + (int)S.tmat[L.prim];
+            } else {
+                uint32_t code = __float_as_uint(L.u);
+                h = make_float4(L.v, 0.0f, 0.0f, L.u);
+                key = code == HIT_MISS ? 0 : code == HIT_FLOOR ? 2 + S.floor_mat : 1;
             }
-            for (int k = 0; k < S.n_panels; k++) {
-                if (depth == 0 && !S.panel[k].camera_visible) continue;
-                float t;
-                if (panel_intersect(S.panel[k], o, d, tmax, &t)) {
-                    tmax = t;
-                    code = HIT_PANEL | (uint32_t)k;
-                }
-            }
-            if (S.floor_present) {
-                float t;
-                if (floor_intersect(S, o, d, tmax, &t)) {
-                    tmax = t;
-                    code = HIT_FLOOR;
-                }
-            }
-            Q.hit[i] = make_float4(tmax, L.u, L.v, __uint_as_float(code));
-            if (code == HIT_MISS)
-                key = 0;
-            else if (code == HIT_FLOOR)
-                key = 2 + S.floor_mat;
-            else if (code & HIT_PANEL)
-                key = 1;
-            else
-                key = 2 + (int)S.tmat[code];
+            Q.hit[i] = h;
         }
         uint64_t m = __ballot(key >= 0);
         if (key >= 0) stage[staged + lane_prefix(m)] = i | ((uint32_t)key << 26);
@@ -827,12 +855,12 @@ struct ExtendJob {
 };
 
 __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
-                                                       uint2* spill) {
+                                                       uint32_t refill, uint2* spill) {
     TRAV_STACK_DECL
     __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64], 0u};
-    traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, st);
+    traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st);
 }
 
 struct ShadeArgs {
@@ -1094,71 +1122,67 @@ struct ShadowJob {
     const DView* views;
     DPaths P;
     DQueues Q;
-    // kind 0: segment to a light point (t_max = dist * (1 - 1e-4)); else a sky direction
-    AGD void ray(uint32_t i, f3& from, f3& dir, float& tmax, int& ignore, uint32_t& kind, uint32_t& path) const {
+    // The other panels and the floor first (world space); a blocked ray
+    // skips the piece.  For a sky ray that reaches the floor the
+    // transmittance is 1 - presence (scene.rs:131-139).  L.u holds that
+    // transmittance, L.aux the path.
+    AGD bool begin(uint32_t i, TravLane& L) {
         float4 a = Q.sh_o[i], b = Q.sh_d[i];
-        path = __float_as_uint(a.w);
-        from = f3{a.x, a.y, a.z};
-        kind = __float_as_uint(Q.sh_x[i].w);
+        uint32_t path = __float_as_uint(a.w);
+        f3 from{a.x, a.y, a.z};
+        uint32_t kind = __float_as_uint(Q.sh_x[i].w);
+        const DView& V = views[P.view[path]];
+        const DScene& S = *V.scene;
+        f3 dir;
+        float tmax, vis = 1.0f;
+        bool blocked = false;
         if (kind == 0u) {
             f3 dd = f3{b.x, b.y, b.z} - from;
             float dist = length(dd);
             dir = dd / dist;
             tmax = dist * (1.0f - 1e-4f);
-            ignore = (int)__float_as_uint(b.w);
+            int ignore = (int)__float_as_uint(b.w);
+            for (int k = 0; k < S.n_panels && !blocked; k++) {
+                float t;
+                if (k != ignore && panel_intersect(S.panel[k], from, dir, tmax, &t)) blocked = true;
+            }
+            if (!blocked && S.floor_present) {
+                float t;
+                if (floor_intersect(S, from, dir, tmax, &t)) blocked = true;
+            }
         } else {
             dir = f3{b.x, b.y, b.z};
             tmax = INF_F;
-            ignore = -1;
-        }
-    }
-    AGD bool begin(uint32_t i, TravLane& L) {
-        f3 from, dir;
-        float tmax;
-        int ignore;
-        uint32_t kind, path;
-        ray(i, from, dir, tmax, ignore, kind, path);
-        const DView& V = views[P.view[path]];
-        const DScene& S = *V.scene;
-        if (V.turn_deg != 0.0f) {
-            from = rot_y(from, V.tsn, V.tcn);
-            dir = rot_y(dir, V.tsn, V.tcn);
-        }
-        lane_begin(L, S.nodes, S.tris, from, dir, tmax);
-        return true;
-    }
-    AGD void finish(uint32_t i, const TravLane& L, bool mine) {
-        if (!mine || L.prim != NO_RAY) return;  // blocked by the piece
-        f3 from, dir;
-        float tmax;
-        int ignore;
-        uint32_t kind, path;
-        ray(i, from, dir, tmax, ignore, kind, path);
-        const DScene& S = *views[P.view[path]].scene;
-        Spec c = from4(Q.sh_c[i]);
-        float4 x = Q.sh_x[i];
-        if (kind == 0u) {
-            for (int k = 0; k < S.n_panels; k++) {
+            for (int k = 0; k < S.n_panels && !blocked; k++) {
                 float t;
-                if (k != ignore && panel_intersect(S.panel[k], from, dir, tmax, &t)) return;
+                if (panel_intersect(S.panel[k], from, dir, INF_F, &t)) blocked = true;
             }
-            if (S.floor_present) {
-                float t;
-                if (floor_intersect(S, from, dir, tmax, &t)) return;
-            }
-        } else {
-            float vis = 1.0f;
-            for (int k = 0; k < S.n_panels; k++) {
-                float t;
-                if (panel_intersect(S.panel[k], from, dir, INF_F, &t)) return;
-            }
-            if (S.floor_present) {
+            if (!blocked && S.floor_present) {
                 float t;
                 if (floor_intersect(S, from, dir, INF_F, &t)) vis = 1.0f - floor_presence(S, from + dir * t);
             }
-            if (!(vis > 0.0f)) return;
-            c = c * (x.x * vis / x.y);
+            if (!(vis > 0.0f)) blocked = true;
         }
+        f3 oo = from, od = dir;
+        if (V.turn_deg != 0.0f) {
+            oo = rot_y(from, V.tsn, V.tcn);
+            od = rot_y(dir, V.tsn, V.tcn);
+        }
+        lane_begin(L, S.nodes, S.tris, oo, od, tmax);
+        L.u = vis;
+        L.aux = path;
+        if (blocked) {
+            L.prim = 1;
+            return false;
+        }
+        return true;
+    }
+    AGD void finish(uint32_t i, const TravLane& L, bool mine) {
+        if (!mine || L.prim != NO_RAY) return;  // blocked
+        float4 x = Q.sh_x[i];
+        Spec c = from4(Q.sh_c[i]);
+        if (__float_as_uint(x.w) != 0u) c = c * (x.x * L.u / x.y);
+        uint32_t path = L.aux;
         Spec Lr = from4(P.L[path]);
         add_clamped(Lr, c, x.z >= 0.0f, x.z);
         P.L[path] = to4(Lr);
@@ -1167,11 +1191,11 @@ struct ShadowJob {
 };
 
 __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
-                                                       uint2* spill) {
+                                                       uint32_t refill, uint2* spill) {
     TRAV_STACK_DECL
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ShadowJob J{views, P, Q};
-    traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, st);
+    traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st);
 }
 
 // render.rs:96-111: per pixel, samples in order, into the pass accumulators
@@ -1226,12 +1250,19 @@ __global__ __launch_bounds__(256) void k_accumulate(const DSeg* __restrict__ seg
 }
 
 // ray tallies of one wave into 64-bit totals: [extension rays, shadow rays]
+// totals: [0] extension rays, [1] shadow rays, [2 + d] extension rays at
+// depth d, [2 + TALLY_DEPTHS + d] shadow rays at depth d (d < TALLY_DEPTHS)
 __global__ void k_tally(const uint32_t* __restrict__ counters, uint32_t depth_slots, unsigned long long* totals) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         unsigned long long e = 0, s = 0;
         for (uint32_t d = 0; d < depth_slots; d++) {
-            e += counters[d * CNT_STRIDE];
-            s += counters[d * CNT_STRIDE + 1];
+            uint32_t ed = counters[d * CNT_STRIDE], sd = counters[d * CNT_STRIDE + 1];
+            e += ed;
+            s += sd;
+            if (d < (uint32_t)TALLY_DEPTHS) {
+                totals[2 + d] += ed;
+                totals[2 + TALLY_DEPTHS + d] += sd;
+            }
         }
         totals[0] += e;
         totals[1] += s;
@@ -1287,12 +1318,12 @@ void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n) {
 }
 
 void launch_rays_intersect(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, HitsSoA h,
-                           uint32_t n, uint32_t* cursor, uint2* spill) {
-    hipLaunchKernelGGL(k_rays_intersect, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, h, n, cursor, spill);
+                           uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill) {
+    hipLaunchKernelGGL(k_rays_intersect, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, h, n, cursor, refill, spill);
 }
 void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, uint8_t* out,
-                          uint32_t n, uint32_t* cursor, uint2* spill) {
-    hipLaunchKernelGGL(k_rays_occluded, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, out, n, cursor, spill);
+                          uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill) {
+    hipLaunchKernelGGL(k_rays_occluded, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, out, n, cursor, refill, spill);
 }
 int traversal_blocks_per_cu() {
     int lo = 1 << 30;
@@ -1309,16 +1340,18 @@ void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t
                    DQueues Q) {
     hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, s, segs, nseg, npaths, views, P, Q);
 }
-void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint2* spill) {
-    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, spill);
+void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
+                   uint2* spill) {
+    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spill);
 }
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
                   float clamp_indirect, DPaths P, DQueues Q, int depth) {
     ShadeArgs A{views, vmats, max_depth, rr_depth, clamp_indirect};
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
 }
-void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint2* spill) {
-    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, spill);
+void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
+                   uint2* spill) {
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spill);
 }
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
                        DPaths P) {

@@ -27,18 +27,21 @@ struct ProbeIn {
 void launch_bsdf_probe(hipStream_t s, const DScene* S, ProbeIn in, float* out, uint32_t n);
 void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n);
 // `cursor` must hold 0 on entry (the persistent kernels pull rays from it).
+// `refill`: idle lanes (1..64) before a wavefront takes new rays.
 void launch_rays_intersect(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, HitsSoA h,
-                           uint32_t n, uint32_t* cursor, uint2* spill);
+                           uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill);
 void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, uint8_t* out,
-                          uint32_t n, uint32_t* cursor, uint2* spill);
+                          uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill);
 // Resident blocks per CU of the traversal kernels (occupancy limit).
 int traversal_blocks_per_cu();
 void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,
                    DQueues Q);
-void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint2* spill);
+void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
+                   uint2* spill);
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
                   float clamp_indirect, DPaths P, DQueues Q, int depth);
-void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint2* spill);
+void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
+                   uint2* spill);
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
                        DPaths P);
 void launch_tally(hipStream_t s, const uint32_t* counters, uint32_t depth_slots, unsigned long long* totals);

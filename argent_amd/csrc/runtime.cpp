@@ -78,7 +78,7 @@ struct ag_ctx {
         vtab_buf;
     DPaths P{};
     DQueues Q{};
-    int grid_trav = 0, grid_work = 0;
+    int grid_trav = 0, grid_batch = 0, grid_work = 0;
     int max_trav_blocks = 1;  // resident traversal blocks per CU (occupancy)
     size_t spill_threads = 0;
     // timing: HIP event pairs around every stage launch on the context stream
@@ -87,7 +87,13 @@ struct ag_ctx {
     hipEvent_t ev[8] = {};
     std::vector<hipEvent_t> evpool;
     size_t ev_used = 0;
-    std::vector<std::pair<int, size_t>> ev_marks;  // (stage, index of the first event of the pair)
+    struct Mark {
+        int stage, depth;
+        size_t ev;  // index of the first event of the pair
+    };
+    std::vector<Mark> ev_marks;
+    std::vector<unsigned long long> depth_rays;  // [2 * TALLY_DEPTHS]: extension, then shadow
+    std::vector<double> depth_ms;                // [3 * TALLY_DEPTHS]: extend, shade, shadow
     DevBuf tally_buf;
     double last_query_ms = 0.0;
     hipEvent_t next_event() {
@@ -100,16 +106,26 @@ struct ag_ctx {
     }
 };
 
+// The piece (scene.rs:48 Arc<Piece>): BVH and mesh arrays on the device,
+// shared by every scene built over the same mesh.
+struct Geometry {
+    DevBuf nodes, tris, pos, nrm, idx, tmat, cavity;
+    uint32_t n_nodes = 0, n_leaves = 0, depth = 0, n_vertices = 0, n_triangles = 0;
+    uint16_t max_material = 0;
+    bool has_nrm = false, has_cavity = false;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    size_t bytes() const { return nodes.bytes + tris.bytes + pos.bytes + idx.bytes + tmat.bytes + nrm.bytes + cavity.bytes; }
+};
+
 struct ag_scene {
     ag_ctx* ctx = nullptr;
+    std::shared_ptr<Geometry> geo;
     DScene hs{};  // host copy with device pointers
-    DevBuf d_scene, nodes, tris, pos, nrm, idx, tmat, cavity, tab, env_w, env_m, env_c;
+    DevBuf d_scene, tab, env_w, env_m, env_c;
     std::vector<ag_material> mats;
     size_t mat_stride = 0;  // floats per material in the table blob
     size_t mat_base = 0;
-    uint32_t n_nodes = 0, n_leaves = 0, depth = 0;
     size_t bytes = 0;
-    float lo[3], hi[3];
     float turntable_deg = 0.0f;
     int n_keys = 2;
     float cam_m[9];  // white-balanced XYZ -> linear sRGB (color.rs:48-55)
@@ -142,7 +158,13 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     c->tuning.paths_per_wave = 1u << 22;
     c->tuning.block_traverse = TRAV_BLOCK;
     c->max_trav_blocks = traversal_blocks_per_cu();
-    c->tuning.blocks_per_cu = (uint32_t)c->max_trav_blocks;
+    // Render passes: whole coherent batches of 64 rays per wavefront at 4
+    // blocks per CU; ray batches (often incoherent): refill at 32 idle lanes,
+    // full occupancy.
+    c->tuning.blocks_per_cu = (uint32_t)std::min(4, c->max_trav_blocks);
+    c->tuning.refill_min = 64;
+    c->tuning.batch_blocks_per_cu = (uint32_t)c->max_trav_blocks;
+    c->tuning.batch_refill_min = 32;
     c->tuning.sort_by_material = 1;
     *out = c;
     return AG_OK;
@@ -175,9 +197,12 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
     if (!c || !t) return fail(AG_E_INVALID, "null argument");
     if (t->paths_per_wave < 1024 || t->paths_per_wave > (1u << 26)) return fail(AG_E_INVALID, "paths_per_wave out of range");
     if (t->block_traverse != TRAV_BLOCK) return fail(AG_E_INVALID, "block_traverse must be 256");
-    if (t->blocks_per_cu < 1 || (int)t->blocks_per_cu > c->max_trav_blocks)
-        return fail(AG_E_INVALID, "blocks_per_cu must be 1.." + std::to_string(c->max_trav_blocks) +
-                                      " (resident traversal blocks per CU)");
+    if (t->blocks_per_cu < 1 || (int)t->blocks_per_cu > c->max_trav_blocks || t->batch_blocks_per_cu < 1 ||
+        (int)t->batch_blocks_per_cu > c->max_trav_blocks)
+        return fail(AG_E_INVALID, "blocks_per_cu and batch_blocks_per_cu must be 1.." +
+                                      std::to_string(c->max_trav_blocks) + " (resident traversal blocks per CU)");
+    if (t->refill_min < 1 || t->refill_min > 64 || t->batch_refill_min < 1 || t->batch_refill_min > 64)
+        return fail(AG_E_INVALID, "refill_min and batch_refill_min must be 1..64");
     c->tuning = *t;
     return AG_OK;
 }
@@ -185,6 +210,22 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
 extern "C" int ag_ctx_stats(const ag_ctx* c, ag_stats* s) {
     if (!c || !s) return fail(AG_E_INVALID, "null argument");
     *s = c->stats;
+    return AG_OK;
+}
+
+extern "C" int ag_ctx_depth_stats(const ag_ctx* c, int32_t n, uint64_t* ext, uint64_t* sh, double* ms_extend,
+                                  double* ms_shade, double* ms_shadow) {
+    if (!c) return fail(AG_E_INVALID, "null context");
+    if (n < 0 || n > TALLY_DEPTHS) return fail(AG_E_INVALID, "n must be 0..64");
+    for (int d = 0; d < n; d++) {
+        bool have = c->depth_rays.size() == 2 * (size_t)TALLY_DEPTHS;
+        bool timed = c->depth_ms.size() == 3 * (size_t)TALLY_DEPTHS;
+        if (ext) ext[d] = have ? c->depth_rays[d] : 0;
+        if (sh) sh[d] = have ? c->depth_rays[TALLY_DEPTHS + d] : 0;
+        if (ms_extend) ms_extend[d] = timed ? c->depth_ms[d] : 0.0;
+        if (ms_shade) ms_shade[d] = timed ? c->depth_ms[TALLY_DEPTHS + d] : 0.0;
+        if (ms_shadow) ms_shadow[d] = timed ? c->depth_ms[2 * TALLY_DEPTHS + d] : 0.0;
+    }
     return AG_OK;
 }
 
@@ -196,8 +237,9 @@ extern "C" int ag_ctx_enable_timing(ag_ctx* c, int32_t on) {
 
 static int ensure_spill(ag_ctx* c) {
     c->grid_trav = c->n_cu * (int)c->tuning.blocks_per_cu;
+    c->grid_batch = c->n_cu * (int)c->tuning.batch_blocks_per_cu;
     c->grid_work = c->n_cu * 8;
-    size_t threads = (size_t)c->grid_trav * TRAV_BLOCK;
+    size_t threads = (size_t)std::max(c->grid_trav, c->grid_batch) * TRAV_BLOCK;
     if (threads > c->spill_threads || !c->spill_buf.p) {
         int rc = c->spill_buf.ensure(threads * STACK_SPILL * sizeof(uint2));
         if (rc) return rc;
@@ -327,42 +369,56 @@ static int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
     return AG_OK;
 }
 
-extern "C" int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* d, ag_scene** out) {
-    if (!ctx || !out) return fail(AG_E_INVALID, "null argument");
-    int rc = validate_desc(d);
-    if (rc) return rc;
-    HIPCHK(hipSetDevice(ctx->device));
-    std::unique_ptr<ag_scene> s(new ag_scene());
-    s->ctx = ctx;
+static int build_geometry(ag_ctx* ctx, const ag_scene_desc* d, std::shared_ptr<Geometry>* out) {
     hipStream_t st = ctx->stream;
+    auto g = std::make_shared<Geometry>();
     WideBvh bvh;
     build_wide_bvh(d->positions, d->triangles, d->n_triangles, &bvh);
-    if (bvh.depth * 3 + 1 > (uint32_t)(STACK_LDS + STACK_SPILL))
-        return fail(AG_E_INVALID, "BVH too deep for the traversal stack (depth " + std::to_string(bvh.depth) + ")");
-    s->n_nodes = bvh.n_nodes;
-    s->n_leaves = bvh.n_leaves;
-    s->depth = bvh.depth;
+    if (bvh.max_stack + 1 > (uint32_t)(STACK_LDS + STACK_SPILL))
+        return fail(AG_E_INVALID, "BVH too deep for the traversal stack (" + std::to_string(bvh.max_stack) +
+                                      " entries needed)");
+    g->n_nodes = bvh.n_nodes;
+    g->n_leaves = bvh.n_leaves;
+    g->depth = bvh.depth;
+    g->n_vertices = d->n_vertices;
+    g->n_triangles = d->n_triangles;
     for (int a = 0; a < 3; a++) {
-        s->lo[a] = bvh.lo[a];
-        s->hi[a] = bvh.hi[a];
+        g->lo[a] = bvh.lo[a];
+        g->hi[a] = bvh.hi[a];
     }
-    if ((rc = upload(s->nodes, bvh.nodes.data(), bvh.nodes.size() * 4, st))) return rc;
+    int rc;
+    if ((rc = upload(g->nodes, bvh.nodes.data(), bvh.nodes.size() * 4, st))) return rc;
     std::vector<float> dummy(12, 0.0f);
-    if ((rc = upload(s->tris, bvh.tris.empty() ? dummy.data() : bvh.tris.data(),
+    if ((rc = upload(g->tris, bvh.tris.empty() ? dummy.data() : bvh.tris.data(),
                      std::max<size_t>(bvh.tris.size(), 12) * 4, st)))
         return rc;
     size_t nv = std::max<uint32_t>(d->n_vertices, 1), nt = std::max<uint32_t>(d->n_triangles, 1);
     std::vector<float> zeros3(3 * nv, 0.0f);
     std::vector<uint32_t> zidx(3 * nt, 0);
-    if ((rc = upload(s->pos, d->n_vertices ? d->positions : zeros3.data(), 3 * nv * 4, st))) return rc;
-    if ((rc = upload(s->idx, d->n_triangles ? d->triangles : zidx.data(), 3 * nt * 4, st))) return rc;
+    if ((rc = upload(g->pos, d->n_vertices ? d->positions : zeros3.data(), 3 * nv * 4, st))) return rc;
+    if ((rc = upload(g->idx, d->n_triangles ? d->triangles : zidx.data(), 3 * nt * 4, st))) return rc;
     std::vector<uint16_t> tm(nt, 0);
     if (d->tri_material && d->n_triangles) std::memcpy(tm.data(), d->tri_material, d->n_triangles * 2);
-    if ((rc = upload(s->tmat, tm.data(), nt * 2, st))) return rc;
-    if (d->normals && d->n_vertices)
-        if ((rc = upload(s->nrm, d->normals, 3 * nv * 4, st))) return rc;
-    if (d->cavity && d->n_vertices)
-        if ((rc = upload(s->cavity, d->cavity, nv * 4, st))) return rc;
+    for (uint16_t m : tm) g->max_material = std::max(g->max_material, m);
+    if ((rc = upload(g->tmat, tm.data(), nt * 2, st))) return rc;
+    g->has_nrm = d->normals && d->n_vertices;
+    g->has_cavity = d->cavity && d->n_vertices;
+    if (g->has_nrm)
+        if ((rc = upload(g->nrm, d->normals, 3 * nv * 4, st))) return rc;
+    if (g->has_cavity)
+        if ((rc = upload(g->cavity, d->cavity, nv * 4, st))) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    *out = std::move(g);
+    return AG_OK;
+}
+
+// Everything but the mesh: materials, panels, environment, floor, white.
+static int scene_over(ag_ctx* ctx, std::shared_ptr<Geometry> geo, const ag_scene_desc* d, ag_scene** out) {
+    int rc;
+    std::unique_ptr<ag_scene> s(new ag_scene());
+    s->ctx = ctx;
+    s->geo = std::move(geo);
+    hipStream_t st = ctx->stream;
     // table blob: materials, panels, env spectra + basis, energy, CMF
     s->mat_stride = tables_per_material();
     size_t n_tab = AG_MAX_MATERIALS * s->mat_stride + AG_MAX_PANELS * AG_SPEC_N + 6 * AG_SPEC_N + 1024 + 3 * AG_SPEC_N;
@@ -451,24 +507,50 @@ extern "C" int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* d, ag_scene** o
     S.floor_mat = d->floor.material;
     S.floor_f0 = d->floor.fade_start;
     S.floor_f1 = d->floor.fade_end;
-    S.nodes = s->nodes.as<float4>();
-    S.tris = s->tris.as<float4>();
-    S.pos = s->pos.as<float>();
-    S.nrm = (d->normals && d->n_vertices) ? s->nrm.as<float>() : nullptr;
-    S.idx = s->idx.as<uint32_t>();
-    S.tmat = s->tmat.as<uint16_t>();
-    S.cavity = (d->cavity && d->n_vertices) ? s->cavity.as<float>() : nullptr;
-    S.n_tris = d->n_triangles;
+    const Geometry& G = *s->geo;
+    S.nodes = G.nodes.as<float4>();
+    S.tris = G.tris.as<float4>();
+    S.pos = G.pos.as<float>();
+    S.nrm = G.has_nrm ? G.nrm.as<float>() : nullptr;
+    S.idx = G.idx.as<uint32_t>();
+    S.tmat = G.tmat.as<uint16_t>();
+    S.cavity = G.has_cavity ? G.cavity.as<float>() : nullptr;
+    S.n_tris = G.n_triangles;
     S.n_mat = d->n_materials;
     if ((rc = upload(s->d_scene, &S, sizeof(DScene), st))) return rc;
     HIPCHK(hipStreamSynchronize(st));
     s->turntable_deg = d->turntable_deg;
     s->n_keys = 2 + d->n_materials;
     camera_matrix(d->has_white != 0, d->white_xyz, s->cam_m);
-    s->bytes = s->nodes.bytes + s->tris.bytes + s->pos.bytes + s->idx.bytes + s->tmat.bytes + s->nrm.bytes +
-               s->cavity.bytes + s->tab.bytes + s->env_w.bytes + s->env_m.bytes + s->env_c.bytes;
+    s->bytes = s->tab.bytes + s->env_w.bytes + s->env_m.bytes + s->env_c.bytes;
     *out = s.release();
     return AG_OK;
+}
+
+extern "C" int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* d, ag_scene** out) {
+    if (!ctx || !out) return fail(AG_E_INVALID, "null argument");
+    int rc = validate_desc(d);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    std::shared_ptr<Geometry> g;
+    if ((rc = build_geometry(ctx, d, &g))) return rc;
+    return scene_over(ctx, std::move(g), d, out);
+}
+
+extern "C" int ag_scene_create_shared(ag_ctx* ctx, const ag_scene* base, const ag_scene_desc* d, ag_scene** out) {
+    if (!ctx || !base || !out) return fail(AG_E_INVALID, "null argument");
+    if (base->ctx != ctx) return fail(AG_E_INVALID, "base scene belongs to another context");
+    ag_scene_desc look = *d;  // the mesh comes from `base`; its arrays in `d` are ignored
+    look.positions = look.normals = look.cavity = nullptr;
+    look.triangles = nullptr;
+    look.tri_material = nullptr;
+    look.n_vertices = look.n_triangles = 0;
+    int rc = validate_desc(&look);
+    if (rc) return rc;
+    if ((int)base->geo->max_material >= d->n_materials)
+        return fail(AG_E_INVALID, "the mesh uses material slots beyond n_materials");
+    HIPCHK(hipSetDevice(ctx->device));
+    return scene_over(ctx, base->geo, d, out);
 }
 
 extern "C" void ag_scene_destroy(ag_scene* s) {
@@ -503,18 +585,18 @@ extern "C" int ag_scene_set_material(ag_scene* s, int32_t m, const ag_material* 
 extern "C" int ag_scene_bounds(const ag_scene* s, float* lo, float* hi) {
     if (!s) return fail(AG_E_INVALID, "null scene");
     for (int a = 0; a < 3; a++) {
-        lo[a] = s->lo[a];
-        hi[a] = s->hi[a];
+        lo[a] = s->geo->lo[a];
+        hi[a] = s->geo->hi[a];
     }
     return AG_OK;
 }
 
 extern "C" int ag_scene_stats(const ag_scene* s, uint32_t* nodes, uint32_t* leaves, uint32_t* depth, size_t* bytes) {
     if (!s) return fail(AG_E_INVALID, "null scene");
-    if (nodes) *nodes = s->n_nodes;
-    if (leaves) *leaves = s->n_leaves;
-    if (depth) *depth = s->depth;
-    if (bytes) *bytes = s->bytes;
+    if (nodes) *nodes = s->geo->n_nodes;
+    if (leaves) *leaves = s->geo->n_leaves;
+    if (depth) *depth = s->geo->depth;
+    if (bytes) *bytes = s->bytes + s->geo->bytes();
     return AG_OK;
 }
 
@@ -708,9 +790,10 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
     c->stats = ag_stats{};
     c->ev_used = 0;
     c->ev_marks.clear();
-    if ((rc = c->tally_buf.ensure(2 * sizeof(unsigned long long)))) return rc;
-    HIPCHK(hipMemsetAsync(c->tally_buf.p, 0, 2 * sizeof(unsigned long long), st));
-    auto timed = [&](int stage, auto&& launch) {
+    const size_t n_tally = 2 + 2 * (size_t)TALLY_DEPTHS;
+    if ((rc = c->tally_buf.ensure(n_tally * sizeof(unsigned long long)))) return rc;
+    HIPCHK(hipMemsetAsync(c->tally_buf.p, 0, n_tally * sizeof(unsigned long long), st));
+    auto timed = [&](int stage, int depth, auto&& launch) {
         if (!c->timing) {
             launch();
             return;
@@ -721,7 +804,7 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
         (void)hipEventRecord(a, st);
         launch();
         (void)hipEventRecord(b, st);
-        c->ev_marks.push_back({stage, i0});
+        c->ev_marks.push_back({stage, depth, i0});
     };
     auto t0 = std::chrono::steady_clock::now();
     uint32_t launches = 0;
@@ -731,17 +814,17 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
         HIPCHK(hipMemcpyAsync(c->seg_buf.p, w.data(), w.size() * sizeof(DSeg), hipMemcpyHostToDevice, st));
         launch_zero_u32(st, c->Q.counters,
                         (uint32_t)(counter_words(ps->max_depth) + (size_t)(ps->max_depth + 2) * CURSORS_PER_DEPTH));
-        timed(3, [&] { launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), npaths, views, c->P, c->Q); });
+        timed(3, -1, [&] { launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), npaths, views, c->P, c->Q); });
         for (uint32_t d = 0; d < ps->max_depth; d++) {
-            timed(0, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>()); });
-            timed(1, [&] {
+            timed(0, (int)d, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, c->spill_buf.as<uint2>()); });
+            timed(1, (int)d, [&] {
                 launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, ps->rr_depth, ps->clamp_indirect, c->P,
                              c->Q, (int)d);
             });
-            timed(2, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->spill_buf.as<uint2>()); });
+            timed(2, (int)d, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, c->spill_buf.as<uint2>()); });
             launches++;
         }
-        timed(4, [&] { launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P); });
+        timed(4, -1, [&] { launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P); });
         launch_tally(st, c->Q.counters, ps->max_depth, c->tally_buf.as<unsigned long long>());
         c->stats.samples += npaths;
     }
@@ -749,8 +832,8 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
         size_t n = (size_t)j.film->w * j.film->h * 9;
         launch_film_commit(st, j.film->sums.as<float>(), j.film->acc.as<float>(), n);
     }
-    unsigned long long tally[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(tally, c->tally_buf.p, sizeof(tally), hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> tally(n_tally, 0);
+    HIPCHK(hipMemcpyAsync(tally.data(), c->tally_buf.p, n_tally * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipGetLastError());
     auto t1 = std::chrono::steady_clock::now();
@@ -759,14 +842,17 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
     c->stats.shadow_rays = tally[1];
     c->stats.extend_launches = launches;
     c->stats.shadow_launches = launches;
+    c->depth_rays.assign(tally.begin() + 2, tally.end());
+    c->depth_ms.assign(3 * (size_t)TALLY_DEPTHS, 0.0);
     for (auto& m : c->ev_marks) {
         float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, c->evpool[m.second], c->evpool[m.second + 1]);
-        double* dst = m.first == 0 ? &c->stats.ms_extend
-                    : m.first == 1 ? &c->stats.ms_shade
-                    : m.first == 2 ? &c->stats.ms_shadow
-                    : m.first == 3 ? &c->stats.ms_raygen : &c->stats.ms_accum;
+        (void)hipEventElapsedTime(&ms, c->evpool[m.ev], c->evpool[m.ev + 1]);
+        double* dst = m.stage == 0 ? &c->stats.ms_extend
+                    : m.stage == 1 ? &c->stats.ms_shade
+                    : m.stage == 2 ? &c->stats.ms_shadow
+                    : m.stage == 3 ? &c->stats.ms_raygen : &c->stats.ms_accum;
         *dst += ms;
+        if (m.stage <= 2 && m.depth >= 0 && m.depth < TALLY_DEPTHS) c->depth_ms[(size_t)m.stage * TALLY_DEPTHS + m.depth] += ms;
     }
     for (auto& j : jobs) {
         j.film->samples += j.count;
@@ -920,9 +1006,30 @@ extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queu
         s->queue.erase(s->queue.begin());
     }
     if (!s->active.empty()) {
+        // choose: no preview yet first, then least work left, within the step budget
+        std::vector<uint64_t> order = s->active;
+        auto left = [&](uint64_t k) {
+            const auto& q = s->reqs[k];
+            return (uint64_t)(q.r.spp - q.st.samples) * q.r.width * q.r.height;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+            bool pa = s->reqs[a].st.t_first_preview < 0.0, pb = s->reqs[b].st.t_first_preview < 0.0;
+            if (pa != pb) return pa;
+            return left(a) < left(b);
+        });
+        std::vector<uint64_t> chosen;
+        uint64_t room = s->step_samples;
+        for (uint64_t k : order) {
+            const auto& q = s->reqs[k];
+            uint64_t cost = (uint64_t)q.next_chunk * q.r.width * q.r.height;
+            if (!chosen.empty() && cost > room) continue;
+            chosen.push_back(k);
+            room = cost >= room ? 0 : room - cost;
+            if (room == 0) break;
+        }
         std::vector<ag_material> mats;
         std::vector<ViewJob> jobs;
-        for (uint64_t k : s->active) {
+        for (uint64_t k : chosen) {
             auto& q = s->reqs[k];
             ViewJob j;
             j.film = q.film;
@@ -942,22 +1049,23 @@ extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queu
         rc = run_views(ctx, jobs[0].scene, jobs, &s->ps, ctx->vmat_buf.as<DMat>());
         if (rc) return rc;
         double t = s->now();
-        std::vector<uint64_t> still;
-        for (uint64_t k : s->active) {
+        for (uint64_t k : chosen) {
             auto& q = s->reqs[k];
             q.st.samples = q.film->samples;
             if (q.st.t_first_preview < 0.0) q.st.t_first_preview = t;
-            uint32_t left = q.r.spp - q.st.samples;
-            if (left == 0) {
+            uint32_t rest = q.r.spp - q.st.samples;
+            if (rest == 0) {
                 q.st.state = AG_REQ_DONE;
                 q.st.t_done = t;
                 continue;
             }
             uint64_t px = (uint64_t)q.r.width * q.r.height;
             uint32_t cap = (uint32_t)std::max<uint64_t>(1, s->step_samples / px);
-            q.next_chunk = std::min(left, std::max(q.next_chunk, std::min(cap, q.next_chunk * 2)));
-            still.push_back(k);
+            q.next_chunk = std::min(rest, std::max(q.next_chunk, std::min(cap, q.next_chunk * 2)));
         }
+        std::vector<uint64_t> still;
+        for (uint64_t k : s->active)
+            if (s->reqs[k].st.state == AG_REQ_ACTIVE) still.push_back(k);
         s->active = still;
     }
     if (n_active) *n_active = (uint32_t)s->active.size();
@@ -1022,9 +1130,10 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
         r = RaysSoA{b, b + n, b + 2 * (size_t)n, b + 3 * (size_t)n, b + 4 * (size_t)n, b + 5 * (size_t)n,
                     rays->tmax ? b + 6 * (size_t)n : nullptr};
     }
-    const float4* nodes = s->nodes.as<float4>();
-    const float4* tris = s->tris.as<float4>();
-    int grid = ctx->grid_trav;
+    const float4* nodes = s->geo->nodes.as<float4>();
+    const float4* tris = s->geo->tris.as<float4>();
+    int grid = ctx->grid_batch;
+    uint32_t refill = ctx->tuning.batch_refill_min;
     if (hits) {
         HitsSoA h{};
         if (device_ptrs) {
@@ -1036,7 +1145,8 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
         }
         launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
         HIPCHK(hipEventRecord(ctx->ev[2], st));
-        launch_rays_intersect(st, grid, nodes, tris, r, h, n, ctx->cursor_buf.as<uint32_t>(), ctx->spill_buf.as<uint2>());
+        launch_rays_intersect(st, grid, nodes, tris, r, h, n, ctx->cursor_buf.as<uint32_t>(), refill,
+                              ctx->spill_buf.as<uint2>());
         HIPCHK(hipEventRecord(ctx->ev[3], st));
         HIPCHK(hipGetLastError());
         if (!device_ptrs) {
@@ -1053,7 +1163,8 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
         }
         launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
         HIPCHK(hipEventRecord(ctx->ev[2], st));
-        launch_rays_occluded(st, grid, nodes, tris, r, o, n, ctx->cursor_buf.as<uint32_t>(), ctx->spill_buf.as<uint2>());
+        launch_rays_occluded(st, grid, nodes, tris, r, o, n, ctx->cursor_buf.as<uint32_t>(), refill,
+                             ctx->spill_buf.as<uint2>());
         HIPCHK(hipEventRecord(ctx->ev[3], st));
         HIPCHK(hipGetLastError());
         if (!device_ptrs) HIPCHK(hipMemcpyAsync(occ, o, n, hipMemcpyDeviceToHost, st));

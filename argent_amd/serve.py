@@ -118,13 +118,17 @@ def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, d
     from . import workloads
     ps = workloads.path_settings(depth)
     scenes = {}
-    # one device scene per (mesh, studio); alloys are per-request slot-0 overrides
+    bases = {}
+    # one device scene per (mesh, studio), all scenes of a mesh sharing its
+    # BVH; alloys are per-request slot-0 overrides
     for r in trace:
         key = (r["mesh"] % len(pieces), r["studio"])
         if key not in scenes:
             p = pieces[key[0]]
             d = p.scene("sterling", 0.07, 0.8, key[1])
-            scenes[key] = (Scene(ctx, d), p, d)
+            sc = Scene(ctx, d, share_mesh_with=bases.get(key[0]))
+            bases.setdefault(key[0], sc)
+            scenes[key] = (sc, p, d)
     mats = {a: Material.metal(a, 0.07, 0.8) for a in ALLOYS}
     cams = {}
     srv = Server(ctx, budget_bytes, step_samples, ps)
@@ -144,7 +148,8 @@ def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, d
                 cams[ck] = p.shot("hero", 100.0, f_stop, r["width"] / r["height"], intersect=sc.intersect)
             cam, turn = cams[ck]
             i = srv.submit(sc, cam, r["width"], r["height"], r["spp"], r["seed"], turn, mats[r["alloy"]])
-            ids.append((i, r))
+            # submissions happen between steps: keep how long the request waited since its arrival
+            ids.append((i, r, ((time.perf_counter() - t0) - r["t"]) * 1e3))
         active, queued = srv.step()
         steps += 1
         if active == 0 and queued == 0:
@@ -155,26 +160,33 @@ def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, d
             log("wall limit reached")
             break
     wall = time.perf_counter() - t0
+    backlog_active, backlog_queued = active, queued
     ttfp, ttf, done_samples, served = [], [], 0, 0
-    offered = sum(r["width"] * r["height"] * r["spp"] for _, r in ids) + sum(
+    offered = sum(r["width"] * r["height"] * r["spp"] for _, r, _ in ids) + sum(
         r["width"] * r["height"] * r["spp"] for r in pending)
-    for i, r in ids:
+    for i, r, lag in ids:  # latencies from the arrival time
         st = srv.status(i)
         done_samples += st["samples"] * r["width"] * r["height"]
         if st["t_first_preview"] >= 0:
-            ttfp.append(st["t_first_preview"] - st["t_submit"])
+            ttfp.append(st["t_first_preview"] - st["t_submit"] + lag)
         if st["state"] == A.REQ_DONE:
             served += 1
-            ttf.append(st["t_done"] - st["t_submit"])
+            ttf.append(st["t_done"] - st["t_submit"] + lag)
     srv.close()
     for sc, _, _ in scenes.values():
+        if sc.base is not None:
+            sc.close()
+    for sc in bases.values():
         sc.close()
     pct = lambda v, q: float(np.percentile(v, q)) if v else None
     return {
         "requests": len(trace), "submitted": len(ids), "completed": served, "steps": steps, "wall_s": round(wall, 2),
-        "ttfp_ms_p50": pct(ttfp, 50), "ttfp_ms_p99": pct(ttfp, 99),
+        "latency_from": "arrival",
+        "ttfp_ms_p50": pct(ttfp, 50), "ttfp_ms_p99": pct(ttfp, 99), "previewed": len(ttfp),
         "final_ms_p50": pct(ttf, 50), "final_ms_p99": pct(ttf, 99),
         "served_msamples_per_s": round(done_samples / wall / 1e6, 2),
+        "backlog_active": backlog_active, "backlog_queued": backlog_queued,
+        "not_submitted": len(pending),
         "offered_msamples_per_s": round(offered / max(trace[-1]["t"], 1e-9) / 1e6, 2) if trace else 0.0,
     }
 

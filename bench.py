@@ -152,6 +152,63 @@ def reduce_traffic(fetch_dir, write_dir, mib):
     print(json.dumps(res))
 
 
+def turntable(args):
+    """144 views (36 turntable steps of 10 degrees x 4 alloys) of the signet,
+    1080 x 1920, spp log-uniform in [64, 512] (seeded) times --spp-scale:
+    rendered as one batched schedule (ag_render_views: every view's samples
+    share waves) and as one render_pass per view; equal samples, and every
+    view's film must come out bit-identical both ways."""
+    import argent_amd as ag
+    from argent_amd import workloads
+    wl = workloads.signet_still()
+    piece = wl["piece"]
+    w, h, depth = wl["width"], wl["height"], wl["depth"]
+    ps = workloads.path_settings(depth)
+    ctx = ag.Context(0)
+    scene = ag.Scene(ctx, piece.scene("sterling", 0.07, 0.8, "white"))
+    alloys = ["sterling", "gold14k", "rose14k", "bronze"]
+    mats = [ag.Material.metal(a, 0.07, 0.8) for a in alloys]
+    rng = np.random.default_rng(44)
+    cam, turn0 = piece.shot("hero", 100.0, 0.0, w / h)
+    views = []
+    for step in range(36):
+        for m in range(4):
+            spp = int(round(np.exp(rng.uniform(np.log(64), np.log(512))) * args.spp_scale))
+            views.append(dict(camera=cam, turntable_deg=turn0 + 10.0 * step, material0=m, count=max(spp, 1),
+                              seed=1000 + len(views)))
+    total = sum(v["count"] for v in views) * w * h
+    log(f"turntable: 144 views, {total / 1e9:.2f} G samples")
+    films = [ag.Film(ctx, w, h) for _ in views]
+    for v, f in zip(views, films):
+        v["film"] = f
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ag.render_views(ctx, scene, views, ps, materials=mats)
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t0
+    batch = [f.sums() for f in films]
+    for f in films:
+        f.clear()
+    t0 = time.perf_counter()
+    for v in views:
+        ag.render_views(ctx, scene, [v], ps, materials=mats)
+    ctx.synchronize()
+    t_each = time.perf_counter() - t0
+    same = all(np.array_equal(b, f.sums()) for b, f in zip(batch, films))
+    res = {"metric": "Msamples/s, turntable batch (144 views, one schedule) vs one pass per view, 1 GPU",
+           "value": round(total / t_batch / 1e6, 2), "unit": "Msamples/s", "n_gpus": 1,
+           "per_view_msamples_per_s": round(total / t_each / 1e6, 2), "batch_s": round(t_batch, 2),
+           "per_view_s": round(t_each, 2), "bit_identical": bool(same), "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"signet turntable 36 x 10 deg x 4 alloys at 1080x1920, spp log-uniform "
+                                  f"64..512 x {args.spp_scale:g}, depth {depth}, white studio",
+                      "views": 144, "samples": int(total)}}
+    for f in films:
+        f.close()
+    scene.close()
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +217,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-wall", type=float, default=45.0, help="stop serving after this many seconds")
+    ap.add_argument("--turntable", action="store_true", help="the 144-view turntable batch instead of the still")
+    ap.add_argument("--spp-scale", type=float, default=1.0, help="turntable: scale every view's spp")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="run a streaming copy of this size first (counter calibration)")
     ap.add_argument("--reduce-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -167,6 +230,9 @@ def main():
     args = ap.parse_args()
     if args.reduce_traffic:
         reduce_traffic(*args.reduce_traffic, args.copy_calibration or 1024)
+        return
+    if args.turntable:
+        turntable(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "
@@ -260,12 +326,26 @@ def main():
                          "achieved_GBs": round(pipe_bytes / elapsed / 1e9, 2),
                          "frac": round(pipe_bytes / elapsed / 1e9 / PEAK_HBM_GBS, 5)},
     }
+    film.close()
+    scene.close()
+    if args.serve_seconds > 0:
+        from argent_amd import serve
+        trace = serve.make_trace(args.serve_seconds, args.serve_rate)
+        log(f"serving: {len(trace)} requests over {args.serve_seconds:.0f} s, building meshes ...")
+        pieces = serve.build_pieces(8)
+        log("serving meshes (triangles):", [len(p.triangles) for p in pieces])
+        res = serve.run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, depth=12,
+                              max_wall=args.serve_wall, log=log)
+        res["trace"] = (f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g} s; resolution uniform over "
+                        f"512x512, 1080x1920, 2000x2000; spp log-uniform 16..1024; 8 meshes "
+                        f"{min(len(p.triangles) for p in pieces)}..{max(len(p.triangles) for p in pieces)} "
+                        f"triangles; 6 alloys x 5 studios; depth 12; stopped after {args.serve_wall:g} s")
+        result["serving"] = res
+        log("serving:", json.dumps(res))
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(desc, cam, w, h, spp, ps, seed, args.cpu_seconds)
         result["gpu_over_cpu"] = round(msps / result["cpu_baseline"]["value"], 1)
-    film.close()
-    scene.close()
     ctx.close()
     print(json.dumps(result), flush=True)
 

@@ -236,6 +236,12 @@ int ag_desc_light_cdf(const ag_scene_desc* d, int32_t* n_lights, int32_t lights[
 /* Scene::new + Piece::new (scene.rs:21-32, :82-86): copies the arrays to the
  * device and builds the BVH.  The scene owns its device memory. */
 int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* desc, ag_scene** out);
+/* Scene::new(Arc<Piece>, ...) over the piece of `base` (scene.rs:48, :82-86;
+ * jobs.rs:690-696 and shot.rs:206-221 clone the Arc per look): materials,
+ * panels, environment, floor and white come from `desc`, the mesh and BVH are
+ * shared with `base` (its mesh arrays in `desc` are ignored) and stay alive
+ * until the last scene over them is destroyed. */
+int ag_scene_create_shared(ag_ctx* ctx, const ag_scene* base, const ag_scene_desc* desc, ag_scene** out);
 void ag_scene_destroy(ag_scene* s);
 /* Scene::turntable_deg (scene.rs:53-55). */
 int ag_scene_set_turntable(ag_scene* s, float deg);
@@ -316,14 +322,17 @@ int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_material* mater
 /* ------------------------------------------------------------ serving
  * A continuous-batching render server: the role of the studio's live worker
  * and job queue (live.rs:637-823, queue.rs:475-691) for many clients at once.
- * Requests are admitted FIFO while their films fit the HBM budget; every
- * ag_server_step renders the next sample chunk of every active request in one
- * shared wavefront schedule (requests on different scenes share waves), so a
- * request that arrives while others run joins at the next step.  A request's
- * chunk sizes depend only on its own spp and the server's settings (first
- * chunk preview_spp, then doubling up to a cap of step_samples / pixels), so
- * its film is bit-identical whether it is served alone or in a mix.  Films
- * stay resident until released; ag_server_read gives the progressive image. */
+ * Requests are admitted FIFO while their films fit the HBM budget.  Every
+ * ag_server_step renders, in one shared wavefront schedule (requests on
+ * different scenes share waves), the next sample chunk of as many active
+ * requests as fit in step_samples samples: first those still waiting for their
+ * first preview, then those with the least work left (at least one request per
+ * step).  A request arriving while others run thus gets its preview within
+ * about one step.  A request's chunk sizes depend only on its own spp and the
+ * server's settings (first chunk preview_spp, then doubling up to a cap of
+ * step_samples / pixels), so its film is bit-identical whether it is served
+ * alone or in a mix.  Films stay resident until released; ag_server_read
+ * gives the progressive image. */
 enum { AG_REQ_QUEUED = 0, AG_REQ_ACTIVE = 1, AG_REQ_DONE = 2, AG_REQ_REJECTED = 3 };
 typedef struct ag_request {
     const ag_scene* scene;
@@ -345,8 +354,8 @@ int ag_server_create(ag_ctx* ctx, size_t hbm_budget_bytes, uint64_t step_samples
                      ag_server** out);
 void ag_server_destroy(ag_server* s);
 int ag_server_submit(ag_server* s, const ag_request* req, uint64_t* id);
-/* One scheduling round: admit, render one chunk of every active request,
- * retire finished ones.  Returns the counts left active and queued. */
+/* One scheduling round: admit, render one chunk of each request chosen as
+ * above, retire finished ones.  Returns the counts left active and queued. */
 int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queued);
 int ag_server_status(const ag_server* s, uint64_t id, ag_request_status* out);
 /* Progressive image of a request (as ag_film_read), host memory w*h*3. */
@@ -358,8 +367,12 @@ int ag_server_release(ag_server* s, uint64_t id);
 typedef struct ag_tuning {
     uint32_t paths_per_wave;   /* path slots in flight */
     uint32_t block_traverse;   /* threads per block, traversal kernels */
-    uint32_t blocks_per_cu;    /* persistent blocks per CU */
+    uint32_t blocks_per_cu;    /* persistent traversal blocks per CU, render passes */
     uint32_t sort_by_material; /* 1 = shade in material-sorted order */
+    uint32_t refill_min;       /* render passes: idle lanes (1..64) before a wavefront
+                                  takes new rays; 64 = whole coherent batches */
+    uint32_t batch_blocks_per_cu; /* the same two for ag_intersect / ag_occluded */
+    uint32_t batch_refill_min;
 } ag_tuning;
 int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
 int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);
@@ -372,6 +385,11 @@ typedef struct ag_stats {
     uint32_t extend_launches, shadow_launches;
 } ag_stats;
 int ag_ctx_stats(const ag_ctx* ctx, ag_stats* out);
+/* The last render's statistics per bounce depth d < n (n <= 64): extension
+ * and shadow rays traced, and (with timing enabled) milliseconds in the
+ * extend, shade and shadow kernels of that depth; any pointer may be null. */
+int ag_ctx_depth_stats(const ag_ctx* ctx, int32_t n, uint64_t* extension_rays, uint64_t* shadow_rays,
+                       double* ms_extend, double* ms_shade, double* ms_shadow);
 int ag_ctx_enable_timing(ag_ctx* ctx, int32_t on);
 
 #ifdef __cplusplus

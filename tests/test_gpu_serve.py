@@ -76,3 +76,61 @@ def test_budget_admission_and_rejection(ctx):
     assert srv.status(a2)["state"] == A.REQ_DONE
     srv.close()
     sa.close()
+
+
+def test_new_request_previews_before_long_one_finishes(ctx):
+    """With a step budget smaller than the work in flight, a request that
+    arrives while a long one renders gets its first preview at the next step
+    and, having less work left, finishes first; both still match their
+    renders alone."""
+    pa, _ = _pieces()
+    sa = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
+    cam, turn = pa.shot("hero", 100.0, 0.0, 1.0)
+    step = 48 * 48 * 4
+    srv = Server(ctx, 8 << 30, step, workloads.path_settings(8))  # the depth _serve renders with
+    long_id = srv.submit(sa, cam, 48, 48, 64, 1, turn)
+    for _ in range(3):
+        srv.step()
+    assert srv.status(long_id)["state"] == A.REQ_ACTIVE
+    short_id = srv.submit(sa, cam, 24, 24, 8, 2, turn)
+    srv.step()
+    st_s = srv.status(short_id)
+    assert st_s["t_first_preview"] >= 0, "no preview after one step"
+    for _ in range(1000):
+        a, q = srv.step()
+        if a == 0 and q == 0:
+            break
+    st_l, st_s = srv.status(long_id), srv.status(short_id)
+    assert st_l["state"] == A.REQ_DONE and st_s["state"] == A.REQ_DONE
+    assert st_s["t_done"] < st_l["t_done"]
+    img_l, img_s = srv.read(long_id), srv.read(short_id)
+    srv.close()
+    assert np.array_equal(img_l, _serve(ctx, [dict(scene=sa, camera=cam, width=48, height=48, spp=64, seed=1,
+                                                   turntable_deg=turn)], step_samples=step)[0][1])
+    assert np.array_equal(img_s, _serve(ctx, [dict(scene=sa, camera=cam, width=24, height=24, spp=8, seed=2,
+                                                   turntable_deg=turn)], step_samples=step)[0][1])
+    sa.close()
+
+
+def test_scenes_sharing_a_mesh_render_as_their_own(ctx):
+    """ag_scene_create_shared: a second look over the same device mesh and BVH
+    renders exactly as a scene built from scratch; the mesh outlives the scene
+    it was built with."""
+    pa, _ = _pieces()
+    base = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
+    d_noir = pa.scene("gold14k", 0.07, 0.8, "noir")
+    shared = ag.Scene(ctx, d_noir, share_mesh_with=base)
+    own = ag.Scene(ctx, pa.scene("gold14k", 0.07, 0.8, "noir"))
+    base.close()  # the shared scene keeps the mesh alive
+    cam, turn = pa.shot("hero", 100.0, 16.0, 1.0)
+    imgs = []
+    for sc in (shared, own):
+        film = ag.Film(ctx, 40, 40)
+        sc.desc.d.turntable_deg = turn
+        ag.render_views(ctx, sc, [dict(camera=cam, film=film, count=4, seed=9, turntable_deg=turn)],
+                        workloads.path_settings(8))
+        imgs.append(film.sums())
+        film.close()
+    assert np.array_equal(imgs[0], imgs[1])
+    shared.close()
+    own.close()
