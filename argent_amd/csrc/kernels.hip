@@ -197,8 +197,13 @@ AGD uint32_t supply_take(RaySupply& R, uint32_t want, uint32_t* first) {
     while (R.lo == R.hi) {
         if (R.tried >= CURSOR_SLICES) return 0;
         uint32_t b = slice_begin(R.n, R.slice), e = slice_begin(R.n, R.slice + 1);
-        uint32_t got = 0;
-        if (lane_id() == 0) got = atomicAdd(R.cursors + R.slice * CURSOR_STRIDE, GRAB);
+        uint32_t got = e - b;  // spent unless the cursor says otherwise
+        if (lane_id() == 0) {
+            // A cursor only grows: a plain read past the slice proves it spent
+            // without another atomic on a contended line.
+            uint32_t* cur = R.cursors + R.slice * CURSOR_STRIDE;
+            if (__hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e - b) got = atomicAdd(cur, GRAB);
+        }
         got = __shfl(got, 0);
         if (got < e - b) {
             R.lo = b + got;
@@ -350,7 +355,9 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
     L.sp = 0;
     uint32_t idx = NO_RAY;
     RaySupply R{n, cursors, 0u, 0u, blockIdx.x % CURSOR_SLICES, 0u};
-    bool more = n > 0;  // uniform across the wavefront
+    // wavefronts beyond what n rays can occupy (GRAB each) leave at once
+    uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+    bool more = wave < (n + GRAB - 1) / GRAB;  // uniform across the wavefront
     for (;;) {
         if (more) {
             uint64_t idle = __ballot(idx == NO_RAY);
@@ -874,6 +881,9 @@ struct ShadeArgs {
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q, int depth) {
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     uint32_t* cnt_next = Q.counters + (depth + 1) * CNT_STRIDE;
+    // every extension ray sits in exactly one bucket: blocks past the ray
+    // count have nothing to shade (uniform per block, before any barrier)
+    if (blockIdx.x * blockDim.x >= cnt[0]) return;
     __shared__ uint32_t pre[MAX_KEYS + 1];
     if (threadIdx.x == 0) {
         pre[0] = 0;

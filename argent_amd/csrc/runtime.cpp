@@ -155,7 +155,10 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
         return fail(AG_E_DEVICE, "stream creation failed");
     }
     for (auto& e : c->ev) (void)hipEventCreate(&e);
-    c->tuning.paths_per_wave = 1u << 22;
+    // 2^26 path slots (~19 GB of path state, queues and buckets on a 288 GB
+    // MI355X): few waves per render, so the long tail of the last bounces --
+    // a handful of rays keeping the GPU for a dispatch each -- is paid rarely.
+    c->tuning.paths_per_wave = 1u << 26;
     c->tuning.block_traverse = TRAV_BLOCK;
     c->max_trav_blocks = traversal_blocks_per_cu();
     // Render passes: whole coherent batches of 64 rays per wavefront at 4
