@@ -18,6 +18,36 @@ namespace ag {
 // ------------------------------------------------------------------ wave helpers
 AGD uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
+// Runs f(u) for the active lanes once per distinct value u of `key` among
+// them, with u in a wavefront-uniform register: what f reads through u (a
+// view, its scene, panels, materials) comes in by scalar loads instead of one
+// vector load per lane.  One pass when the lanes agree, the usual case.
+template <class F>
+AGD void per_uniform(uint32_t key, F&& f) {
+    uint64_t todo = __ballot(1);
+    while (todo) {
+        int leader = __ffsll((unsigned long long)todo) - 1;
+        uint32_t u = __builtin_amdgcn_readlane(key, leader);
+        bool mine = key == u;
+        // inside, key == u on every active lane: take it with readfirstlane so
+        // the compiler keeps a scalar (it would otherwise reuse the per-lane key)
+        if (mine) f(__builtin_amdgcn_readfirstlane(key));
+        todo &= ~__ballot(mine);
+    }
+}
+
+// Read *p through the constant address space.  With a wavefront-uniform p
+// (inside per_uniform) the compiler then issues scalar loads into SGPRs; the
+// scene and view blocks are read-only while a kernel runs.
+template <class T>
+AGD T uload(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T*)p;
+#else
+    return *p;  // host pass of this file: never executed
+#endif
+}
+
 // Append for every lane with pred set; returns the slot (valid where pred).
 // Must be reached by all lanes of the wavefront.
 AGD uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -359,21 +389,34 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
     uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
     bool more = wave < (n + GRAB - 1) / GRAB;  // uniform across the wavefront
     for (;;) {
-        if (more) {
-            uint64_t idle = __ballot(idx == NO_RAY);
-            uint32_t cnt = (uint32_t)__popcll(idle);
-            if (cnt >= refill_min) {
+        if (more && (uint32_t)__popcll(__ballot(idx == NO_RAY)) >= refill_min) {
+            // Fill the idle lanes with the next rays in queue order; a ray its
+            // job settles without the piece finishes at once and its lane takes
+            // the next one, so traversal batches hold only rays that need it.
+            for (;;) {
+                uint64_t idle = __ballot(idx == NO_RAY);
+                uint32_t cnt = (uint32_t)__popcll(idle);
+                if (cnt == 0) break;
                 uint32_t first = 0;
                 uint32_t k = supply_take(R, cnt, &first);
                 if (k == 0) {
                     more = false;
-                } else if (idx == NO_RAY) {
+                    break;
+                }
+                bool fin = false;
+                if (idx == NO_RAY) {
                     uint32_t r = lane_prefix(idle);
                     if (r < k) {
                         idx = first + r;
-                        if (!J.begin(idx, L)) L.cur = TRAV_DONE;
+                        if (!J.begin(idx, L)) {
+                            L.cur = TRAV_DONE;
+                            fin = true;
+                        }
                     }
                 }
+                if (__ballot(fin) == 0) break;
+                J.finish(idx, L, fin);
+                if (fin) idx = NO_RAY;
             }
         }
         if (__ballot(idx != NO_RAY) == 0) break;  // only once every slice is spent
@@ -461,7 +504,12 @@ AGD bool panel_intersect(const DPanel& P, f3 o, f3 d, float tmax, float* tout) {
     f3 n = ld3(P.n);
     float denom = dot(d, n);
     if (fabsf(denom) < 1e-9f) return false;
-    float t = dot(ld3(P.c) - o, n) / denom;
+    float num = dot(ld3(P.c) - o, n);
+    // t = num / denom > 1e-6 needs num and denom of one sign: reject the
+    // planes behind the ray before paying for the division (same result,
+    // NaN included: a NaN num either fails here or divides to NaN below)
+    if ((num > 0.0f) != (denom > 0.0f) || num == 0.0f) return false;
+    float t = num / denom;
     if (t <= 1e-6f || t >= tmax) return false;
     f3 p = (o + d * t) - ld3(P.c);
     float a = dot(p, ld3(P.hu)) / P.hu2;
@@ -784,51 +832,61 @@ struct ExtendJob {
     AGD bool begin(uint32_t i, TravLane& L) {
         float4 a = ro[i], b = rd[i];
         uint32_t vi = P.view[__float_as_uint(a.w)];
-        const DView& V = views[vi];
-        const DScene& S = *V.scene;
-        f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
-        float tmax = INF_F;
-        uint32_t code = HIT_MISS;
-        for (int k = 0; k < S.n_panels; k++) {
-            if (depth == 0 && !S.panel[k].camera_visible) continue;
-            float t;
-            if (panel_intersect(S.panel[k], o, d, tmax, &t)) {
-                tmax = t;
-                code = HIT_PANEL | (uint32_t)k;
+        per_uniform(vi, [&](uint32_t v) {
+            const DScene* Sp = uload(&views[v].scene);
+            const float turn = uload(&views[v].turn_deg), tsn = uload(&views[v].tsn), tcn = uload(&views[v].tcn);
+            f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+            float tmax = INF_F;
+            uint32_t code = HIT_MISS;
+            const int np = uload(&Sp->n_panels);
+            for (int k = 0; k < np; k++) {
+                const DPanel pk = uload(&Sp->panel[k]);
+                if (depth == 0 && !pk.camera_visible) continue;
+                float t;
+                if (panel_intersect(pk, o, d, tmax, &t)) {
+                    tmax = t;
+                    code = HIT_PANEL | (uint32_t)k;
+                }
             }
-        }
-        if (S.floor_present) {
-            float t;
-            if (floor_intersect(S, o, d, tmax, &t)) {
-                tmax = t;
-                code = HIT_FLOOR;
+            if (uload(&Sp->floor_present)) {
+                const float fy = uload(&Sp->floor_y);
+                if (fabsf(d.y) >= 1e-9f) {  // light.rs:133-162, as floor_intersect
+                    float t = (fy - o.y) / d.y;
+                    if (t > 1e-5f && t < tmax) {
+                        tmax = t;
+                        code = HIT_FLOOR;
+                    }
+                }
             }
-        }
-        if (V.turn_deg != 0.0f) {
-            o = rot_y(o, V.tsn, V.tcn);
-            d = rot_y(d, V.tsn, V.tcn);
-        }
-        // next float above tmax (> 0 here: panel t > 1e-6, floor t > 1e-5)
-        lane_begin(L, S.nodes, S.tris, o, d, tmax < INF_F ? __uint_as_float(__float_as_uint(tmax) + 1u) : INF_F);
-        L.u = __uint_as_float(code);
-        L.v = tmax;
+            if (turn != 0.0f) {
+                o = rot_y(o, tsn, tcn);
+                d = rot_y(d, tsn, tcn);
+            }
+            // next float above tmax (> 0 here: panel t > 1e-6, floor t > 1e-5)
+            float tpiece = tmax < INF_F ? __uint_as_float(__float_as_uint(tmax) + 1u) : INF_F;
+            lane_begin(L, uload(&Sp->nodes), uload(&Sp->tris), o, d, tpiece);
+            L.u = __uint_as_float(code);
+            L.v = tmax;
+        });
         L.aux = vi;
         return true;
     }
     AGD void finish(uint32_t i, const TravLane& L, bool mine) {
         int key = -1;
         if (mine) {
-            const DScene& S = *views[L.aux].scene;
-            float4 h;
-            if (L.prim != NO_RAY) {
-                h = make_float4(L.t, L.u, L.v, __uint_as_float(L.prim));
-                key = 2 + (int)S.tmat[L.prim];
-            } else {
-                uint32_t code = __float_as_uint(L.u);
-                h = make_float4(L.v, 0.0f, 0.0f, L.u);
-                key = code == HIT_MISS ? 0 : code == HIT_FLOOR ? 2 + S.floor_mat : 1;
-            }
-            Q.hit[i] = h;
+            per_uniform(L.aux, [&](uint32_t v) {
+                const DScene* Sp = uload(&views[v].scene);
+                float4 h;
+                if (L.prim != NO_RAY) {
+                    h = make_float4(L.t, L.u, L.v, __uint_as_float(L.prim));
+                    key = 2 + (int)uload(&Sp->tmat)[L.prim];
+                } else {
+                    uint32_t code = __float_as_uint(L.u);
+                    h = make_float4(L.v, 0.0f, 0.0f, L.u);
+                    key = code == HIT_MISS ? 0 : code == HIT_FLOOR ? 2 + uload(&Sp->floor_mat) : 1;
+                }
+                Q.hit[i] = h;
+            });
         }
         uint64_t m = __ballot(key >= 0);
         if (key >= 0) stage[staged + lane_prefix(m)] = i | ((uint32_t)key << 26);
@@ -1141,45 +1199,63 @@ struct ShadowJob {
         uint32_t path = __float_as_uint(a.w);
         f3 from{a.x, a.y, a.z};
         uint32_t kind = __float_as_uint(Q.sh_x[i].w);
-        const DView& V = views[P.view[path]];
-        const DScene& S = *V.scene;
-        f3 dir;
-        float tmax, vis = 1.0f;
         bool blocked = false;
-        if (kind == 0u) {
-            f3 dd = f3{b.x, b.y, b.z} - from;
-            float dist = length(dd);
-            dir = dd / dist;
-            tmax = dist * (1.0f - 1e-4f);
-            int ignore = (int)__float_as_uint(b.w);
-            for (int k = 0; k < S.n_panels && !blocked; k++) {
-                float t;
-                if (k != ignore && panel_intersect(S.panel[k], from, dir, tmax, &t)) blocked = true;
+        per_uniform(P.view[path], [&](uint32_t v) {
+            const DScene* Sp = uload(&views[v].scene);
+            const float turn = uload(&views[v].turn_deg), tsn = uload(&views[v].tsn), tcn = uload(&views[v].tcn);
+            const int np = uload(&Sp->n_panels);
+            const bool has_floor = uload(&Sp->floor_present) != 0;
+            const float fy = uload(&Sp->floor_y);
+            f3 dir;
+            float tmax, vis = 1.0f;
+            if (kind == 0u) {
+                f3 dd = f3{b.x, b.y, b.z} - from;
+                float dist = length(dd);
+                dir = dd / dist;
+                tmax = dist * (1.0f - 1e-4f);
+                int ignore = (int)__float_as_uint(b.w);
+                for (int k = 0; k < np && !blocked; k++) {
+                    const DPanel pk = uload(&Sp->panel[k]);
+                    float t;
+                    if (k != ignore && panel_intersect(pk, from, dir, tmax, &t)) blocked = true;
+                }
+            } else {
+                dir = f3{b.x, b.y, b.z};
+                tmax = INF_F;
+                for (int k = 0; k < np && !blocked; k++) {
+                    const DPanel pk = uload(&Sp->panel[k]);
+                    float t;
+                    if (panel_intersect(pk, from, dir, INF_F, &t)) blocked = true;
+                }
             }
-            if (!blocked && S.floor_present) {
-                float t;
-                if (floor_intersect(S, from, dir, tmax, &t)) blocked = true;
+            if (!blocked && has_floor && fabsf(dir.y) >= 1e-9f) {  // light.rs:133-162, as floor_intersect
+                float t = (fy - from.y) / dir.y;
+                if (t > 1e-5f && t < tmax) {
+                    if (kind == 0u) {
+                        blocked = true;
+                    } else {  // floor_presence, light.rs:148-162
+                        f3 p = from + dir * t;
+                        float r = sqrtf(p.x * p.x + p.z * p.z), f0 = uload(&Sp->floor_f0), f1 = uload(&Sp->floor_f1);
+                        float pres = 1.0f;
+                        if (r >= f1) {
+                            pres = 0.0f;
+                        } else if (r > f0) {
+                            float q = (r - f0) / (f1 - f0);
+                            pres = 1.0f - q * q * (3.0f - 2.0f * q);
+                        }
+                        vis = 1.0f - pres;
+                    }
+                }
             }
-        } else {
-            dir = f3{b.x, b.y, b.z};
-            tmax = INF_F;
-            for (int k = 0; k < S.n_panels && !blocked; k++) {
-                float t;
-                if (panel_intersect(S.panel[k], from, dir, INF_F, &t)) blocked = true;
+            if (kind != 0u && !(vis > 0.0f)) blocked = true;
+            f3 oo = from, od = dir;
+            if (turn != 0.0f) {
+                oo = rot_y(from, tsn, tcn);
+                od = rot_y(dir, tsn, tcn);
             }
-            if (!blocked && S.floor_present) {
-                float t;
-                if (floor_intersect(S, from, dir, INF_F, &t)) vis = 1.0f - floor_presence(S, from + dir * t);
-            }
-            if (!(vis > 0.0f)) blocked = true;
-        }
-        f3 oo = from, od = dir;
-        if (V.turn_deg != 0.0f) {
-            oo = rot_y(from, V.tsn, V.tcn);
-            od = rot_y(dir, V.tsn, V.tcn);
-        }
-        lane_begin(L, S.nodes, S.tris, oo, od, tmax);
-        L.u = vis;
+            lane_begin(L, uload(&Sp->nodes), uload(&Sp->tris), oo, od, tmax);
+            L.u = vis;
+        });
         L.aux = path;
         if (blocked) {
             L.prim = 1;
