@@ -21,7 +21,7 @@ import numpy as np
 from . import _abi as A
 from ._abi import ArgentError, check
 
-__all__ = ["Context", "SceneDesc", "Scene", "Film", "PathSettings", "Material", "render_pass", "render_views",
+__all__ = ["Context", "SceneDesc", "Scene", "Film", "PathSettings", "Material", "render_pass", "render_pass_async", "Pass", "render_views",
            "look_at", "frame", "VIEWS", "ArgentError", "alloy_f0_srgb", "alloy_ior"]
 
 VIEWS = {  # views.rs:17-31: (azimuth, elevation, fill)
@@ -389,6 +389,44 @@ def frame(points, view="hero", focal_mm=100.0, f_stop=0.0, aspect=1.0):
 def render_pass(ctx, scene, cam, film, ps, count, seed):
     """render.rs:67-115 on the GPU: add `count` samples per pixel to `film`."""
     check(A.lib().ag_render_pass(ctx.h, scene.h, C.byref(cam), film.h, C.byref(ps.c()), count, seed))
+
+
+class Pass:
+    """An enqueued render pass (ag_render_pass_async): done() polls, wait()
+    blocks and fills the context statistics."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def done(self):
+        rc = A.lib().ag_event_query(self.h)
+        if rc == A.PENDING:
+            return False
+        check(rc)
+        return True
+
+    def wait(self):
+        check(A.lib().ag_event_wait(self.h))
+
+    def close(self):
+        if self.h:
+            A.lib().ag_event_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render_pass_async(ctx, scene, cam, film, ps, count, seed):
+    """render_pass, enqueued; returns a Pass.  Keep scene, camera and film
+    alive until it is done."""
+    h = C.c_void_p()
+    check(A.lib().ag_render_pass_async(ctx.h, scene.h, C.byref(cam), film.h, C.byref(ps.c()), count, seed,
+                                       C.byref(h)))
+    return Pass(h)
 
 
 def render_views(ctx, scene, views, ps, materials=()):

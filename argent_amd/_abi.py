@@ -9,6 +9,7 @@ import os
 SPEC_N = 471
 GRID_N = 95
 MAX_PANELS = 16
+PENDING = 6  # AG_PENDING
 MAX_MATERIALS = 32
 
 MAT_METAL, MAT_GEM, MAT_GEM_OPAQUE, MAT_PLASTIC, MAT_DIFFUSE, MAT_BLACK, MAT_OPAL = range(7)
@@ -159,6 +160,11 @@ _SIGS = [
     ("ag_film_samples", C.c_uint32, [C.c_void_p]),
     ("ag_render_pass", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_void_p,
                                  C.POINTER(PathSettingsC), C.c_uint32, C.c_uint32]),
+    ("ag_render_pass_async", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_void_p, C.POINTER(PathSettingsC),
+                                       C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("ag_event_query", C.c_int, [C.c_void_p]),
+    ("ag_event_wait", C.c_int, [C.c_void_p]),
+    ("ag_event_destroy", None, [C.c_void_p]),
     ("ag_intersect", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, C.POINTER(HitsSoA)]),
     ("ag_occluded", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, u8p]),
     ("ag_intersect_device", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32,

@@ -735,8 +735,35 @@ void make_dview(DView& v, const ag_scene* sc, const ag_camera* cam, float turn_d
 
 }  // namespace
 
+// A pass enqueued by ag_render_pass_async: completion event, its own ray
+// tallies, and the host sources of its copies, kept until it is done.
+struct ag_event {
+    ag_ctx* ctx = nullptr;
+    hipEvent_t done = nullptr;
+    DevBuf tally;
+    std::vector<std::vector<DSeg>> waves;
+    std::vector<DView> dv;
+    uint64_t samples = 0;
+    uint32_t launches = 0;
+    std::chrono::steady_clock::time_point t0;
+    bool finished = false;
+};
+
+static void set_stats(ag_ctx* c, const std::vector<unsigned long long>& tally, uint32_t launches, uint64_t samples,
+                      double ms_total) {
+    c->stats.ms_total = ms_total;
+    c->stats.samples = samples;
+    c->stats.extension_rays = tally[0];
+    c->stats.shadow_rays = tally[1];
+    c->stats.extend_launches = launches;
+    c->stats.shadow_launches = launches;
+    c->depth_rays.assign(tally.begin() + 2, tally.end());
+}
+
+// With `ev` the pass is only enqueued (no kernel timing); statistics come
+// with ag_event_wait.
 static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, const ag_path_settings* ps,
-                     const DMat* vmats_dev) {
+                     const DMat* vmats_dev, ag_event* ev = nullptr) {
     if (ps->max_depth > 64) return fail(AG_E_INVALID, "max_depth above 64");
     HIPCHK(hipSetDevice(c->device));
     uint32_t cap = c->tuning.paths_per_wave;
@@ -794,10 +821,11 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
     c->ev_used = 0;
     c->ev_marks.clear();
     const size_t n_tally = 2 + 2 * (size_t)TALLY_DEPTHS;
-    if ((rc = c->tally_buf.ensure(n_tally * sizeof(unsigned long long)))) return rc;
-    HIPCHK(hipMemsetAsync(c->tally_buf.p, 0, n_tally * sizeof(unsigned long long), st));
+    DevBuf& tally_buf = ev ? ev->tally : c->tally_buf;
+    if ((rc = tally_buf.ensure(n_tally * sizeof(unsigned long long)))) return rc;
+    HIPCHK(hipMemsetAsync(tally_buf.p, 0, n_tally * sizeof(unsigned long long), st));
     auto timed = [&](int stage, int depth, auto&& launch) {
-        if (!c->timing) {
+        if (!c->timing || ev) {
             launch();
             return;
         }
@@ -828,24 +856,34 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
             launches++;
         }
         timed(4, -1, [&] { launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P); });
-        launch_tally(st, c->Q.counters, ps->max_depth, c->tally_buf.as<unsigned long long>());
+        launch_tally(st, c->Q.counters, ps->max_depth, tally_buf.as<unsigned long long>());
         c->stats.samples += npaths;
     }
     for (auto& j : jobs) {
         size_t n = (size_t)j.film->w * j.film->h * 9;
         launch_film_commit(st, j.film->sums.as<float>(), j.film->acc.as<float>(), n);
     }
+    if (ev) {
+        HIPCHK(hipGetLastError());
+        if (!ev->done) HIPCHK(hipEventCreateWithFlags(&ev->done, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(ev->done, st));
+        ev->waves = std::move(waves);
+        ev->dv = std::move(dv);
+        ev->samples = c->stats.samples;
+        ev->launches = launches;
+        ev->t0 = t0;
+        for (auto& j : jobs) {
+            j.film->samples += j.count;
+            std::memcpy(j.film->cam_m, j.dv.cam_m, sizeof(j.dv.cam_m));
+        }
+        return AG_OK;
+    }
     std::vector<unsigned long long> tally(n_tally, 0);
-    HIPCHK(hipMemcpyAsync(tally.data(), c->tally_buf.p, n_tally * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(tally.data(), tally_buf.p, n_tally * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipGetLastError());
     auto t1 = std::chrono::steady_clock::now();
-    c->stats.ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    c->stats.extension_rays = tally[0];
-    c->stats.shadow_rays = tally[1];
-    c->stats.extend_launches = launches;
-    c->stats.shadow_launches = launches;
-    c->depth_rays.assign(tally.begin() + 2, tally.end());
+    set_stats(c, tally, launches, c->stats.samples, std::chrono::duration<double, std::milli>(t1 - t0).count());
     c->depth_ms.assign(3 * (size_t)TALLY_DEPTHS, 0.0);
     for (auto& m : c->ev_marks) {
         float ms = 0.0f;
@@ -878,6 +916,68 @@ extern "C" int ag_render_pass(ag_ctx* ctx, const ag_scene* scene, const ag_camer
     make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed);
     std::vector<ViewJob> jobs{j};
     return run_views(ctx, scene, jobs, ps, nullptr);
+}
+
+extern "C" int ag_render_pass_async(ag_ctx* ctx, const ag_scene* scene, const ag_camera* cam, ag_film* film,
+                                    const ag_path_settings* ps, uint32_t count, uint32_t seed, ag_event** out) {
+    if (!ctx || !scene || !cam || !film || !ps || !out) return fail(AG_E_INVALID, "null argument");
+    if (film->ctx != ctx || scene->ctx != ctx) return fail(AG_E_INVALID, "objects belong to another context");
+    if ((uint64_t)film->samples + count > 0xFFFFFFFFull) return fail(AG_E_INVALID, "sample index overflow");
+    std::unique_ptr<ag_event> ev(new ag_event());
+    ev->ctx = ctx;
+    if (count > 0) {
+        ViewJob j;
+        j.scene = scene;
+        j.film = film;
+        j.count = count;
+        j.first = film->samples;
+        make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed);
+        std::vector<ViewJob> jobs{j};
+        int rc = run_views(ctx, scene, jobs, ps, nullptr, ev.get());
+        if (rc) return rc;
+    } else {
+        ev->finished = true;
+    }
+    *out = ev.release();
+    return AG_OK;
+}
+
+extern "C" int ag_event_query(const ag_event* ev) {
+    if (!ev) return fail(AG_E_INVALID, "null event");
+    if (ev->finished || !ev->done) return AG_OK;
+    hipError_t e = hipEventQuery(ev->done);
+    if (e == hipSuccess) return AG_OK;
+    if (e == hipErrorNotReady) return AG_PENDING;
+    return fail(AG_E_DEVICE, std::string("event query: ") + hipGetErrorString(e));
+}
+
+extern "C" int ag_event_wait(ag_event* ev) {
+    if (!ev) return fail(AG_E_INVALID, "null event");
+    if (ev->finished) return AG_OK;
+    ag_ctx* c = ev->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n_tally = 2 + 2 * (size_t)TALLY_DEPTHS;
+    std::vector<unsigned long long> tally(n_tally, 0);
+    HIPCHK(hipEventSynchronize(ev->done));
+    HIPCHK(hipMemcpy(tally.data(), ev->tally.p, n_tally * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    auto t1 = std::chrono::steady_clock::now();
+    c->stats = ag_stats{};
+    set_stats(c, tally, ev->launches, ev->samples, std::chrono::duration<double, std::milli>(t1 - ev->t0).count());
+    c->depth_ms.assign(3 * (size_t)TALLY_DEPTHS, 0.0);
+    ev->finished = true;
+    ev->waves.clear();
+    ev->dv.clear();
+    return AG_OK;
+}
+
+extern "C" void ag_event_destroy(ag_event* ev) {
+    if (!ev) return;
+    (void)hipSetDevice(ev->ctx->device);
+    if (ev->done) {
+        (void)hipEventSynchronize(ev->done);
+        (void)hipEventDestroy(ev->done);
+    }
+    delete ev;
 }
 
 static int upload_view_materials(ag_ctx* ctx, const ag_material* materials, int32_t n_materials) {

@@ -143,7 +143,9 @@ def reduce_traffic(fetch_dir, write_dir, mib):
                      "k_extend_kib_raw_mean": round(sum(ext) / len(ext), 1),
                      "k_extend_bytes_mean": round(sum(ext) / len(ext) * scale * 1024.0)}
     total = out["FETCH_SIZE"]["k_extend_bytes_mean"] + out["WRITE_SIZE"]["k_extend_bytes_mean"]
-    res = {"kernel": "k_extend", "workload": "signet still 1080x1920 16spp depth 12 (same wave size as 256 spp)",
+    res = {"kernel": "k_extend",
+           "workload": "signet still 1080x1920 64 spp depth 12: two waves of 2^26 paths, the same per-dispatch "
+                       "ray counts as the waves of the 256-spp still",
            "bytes_per_dispatch": int(total), "fetch": out["FETCH_SIZE"], "write": out["WRITE_SIZE"]}
     path = os.path.join(HERE, "profiles", "k_extend_signet_traffic.json")
     with open(path, "w") as fh:
@@ -277,6 +279,7 @@ def main():
         log(f"step {i}: {s['ms_total']:.0f} ms, {s['extension_rays'] / s['samples']:.3f} segments/sample")
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
+    ds = ctx.depth_stats(depth)  # the last step, per bounce depth
     samples = w * h * spp * args.steps
     msps = samples / elapsed / 1e6
     img = film.color()
@@ -322,6 +325,10 @@ def main():
             "bytes_per_unit": EXTEND_BYTES_PER_RAY, "unit_of_work": "extension ray",
             "launch_ms": round(ext_ms, 4),
         },
+        "per_depth": {"extension_rays": ds["extension_rays"], "shadow_rays": ds["shadow_rays"],
+                      "ms_extend": [round(x, 2) for x in ds["ms_extend"]],
+                      "ms_shade": [round(x, 2) for x in ds["ms_shade"]],
+                      "ms_shadow": [round(x, 2) for x in ds["ms_shadow"]]},
         "pipeline_hbm": {"bytes_alg_per_step": int(pipe_bytes / args.steps),
                          "achieved_GBs": round(pipe_bytes / elapsed / 1e9, 2),
                          "frac": round(pipe_bytes / elapsed / 1e9 / PEAK_HBM_GBS, 5)},

@@ -36,7 +36,8 @@ enum {
     AG_E_DEVICE = 2,    /* HIP runtime error */
     AG_E_NOMEM = 3,     /* allocation failed or over budget */
     AG_E_NOTFOUND = 4,  /* unknown name */
-    AG_E_STATE = 5      /* call not valid in the object's state */
+    AG_E_STATE = 5,     /* call not valid in the object's state */
+    AG_PENDING = 6      /* ag_event_query: the pass has not finished */
 };
 
 /* ---------------------------------------------------------------- materials
@@ -271,6 +272,19 @@ uint32_t ag_film_samples(const ag_film* f);
  * film->samples.  Synchronous. */
 int ag_render_pass(ag_ctx* ctx, const ag_scene* scene, const ag_camera* cam, ag_film* film,
                    const ag_path_settings* ps, uint32_t count, uint32_t seed);
+/* The same pass, enqueued: returns once the work is on the context's stream.
+ * The film's sample count advances at once, so further passes (async or
+ * not) on the context run after it and continue its sample indices; the
+ * film, scene and camera must stay alive until the event is done.  No per-
+ * kernel timing is recorded; ag_event_wait fills the context statistics.
+ * The reference has no asynchronous pass; this is for callers that overlap
+ * host work or other contexts with rendering. */
+typedef struct ag_event ag_event;
+int ag_render_pass_async(ag_ctx* ctx, const ag_scene* scene, const ag_camera* cam, ag_film* film,
+                         const ag_path_settings* ps, uint32_t count, uint32_t seed, ag_event** ev);
+int ag_event_query(const ag_event* ev);  /* AG_OK done, AG_PENDING not yet */
+int ag_event_wait(ag_event* ev);
+void ag_event_destroy(ag_event* ev);
 
 /* Geometry queries over the scene's piece BVH in object space:
  * Bvh::intersect (bvh.rs:159-213) and Bvh::occluded (bvh.rs:216-245).

@@ -372,3 +372,32 @@ def test_progressive_passes_continue_samples(ctx):
     a.close()
     b.close()
     sc.close()
+
+
+def test_async_passes_equal_sync_passes(ctx):
+    """ag_render_pass_async: two enqueued passes continue sample indices like
+    two synchronous ones and give the same film bit for bit; statistics come
+    with wait."""
+    pos, tris, nrm = synth.torus_ring(seed=1, nu=128, nv=32)
+    piece = workloads.Piece(pos, tris, nrm)
+    desc = piece.scene("gold14k", 0.07, 0.8, "noir")
+    cam, turn = piece.shot("hero", 100.0, 16.0, 1.0)
+    desc.d.turntable_deg = turn
+    sc = ag.Scene(ctx, desc)
+    ps = workloads.path_settings(8)
+    f_sync, f_async = ag.Film(ctx, 48, 48), ag.Film(ctx, 48, 48)
+    ag.render_pass(ctx, sc, cam, f_sync, ps, 3, 5)
+    ag.render_pass(ctx, sc, cam, f_sync, ps, 5, 5)
+    p1 = ag.render_pass_async(ctx, sc, cam, f_async, ps, 3, 5)
+    p2 = ag.render_pass_async(ctx, sc, cam, f_async, ps, 5, 5)
+    assert f_async.samples == 8  # counted at enqueue
+    p2.wait()
+    assert p1.done() and p2.done()
+    p1.wait()
+    assert ctx.stats()["samples"] == 48 * 48 * 3 and ctx.stats()["extension_rays"] > 0
+    assert np.array_equal(f_sync.sums(), f_async.sums())
+    p1.close()
+    p2.close()
+    f_sync.close()
+    f_async.close()
+    sc.close()
