@@ -229,10 +229,13 @@ AGD uint32_t supply_take(RaySupply& R, uint32_t want, uint32_t* first) {
         uint32_t b = slice_begin(R.n, R.slice), e = slice_begin(R.n, R.slice + 1);
         uint32_t got = e - b;  // spent unless the cursor says otherwise
         if (lane_id() == 0) {
-            // A cursor only grows: a plain read past the slice proves it spent
-            // without another atomic on a contended line.
+            // Own slice: straight to the atomic.  Another wavefront's slice: a
+            // cursor only grows, so a plain read past the slice proves it spent
+            // without an atomic on a contended line (every wavefront probes the
+            // other slices once the queue runs dry).
             uint32_t* cur = R.cursors + R.slice * CURSOR_STRIDE;
-            if (__hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e - b) got = atomicAdd(cur, GRAB);
+            if (R.tried == 0 || __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e - b)
+                got = atomicAdd(cur, GRAB);
         }
         got = __shfl(got, 0);
         if (got < e - b) {
@@ -933,6 +936,7 @@ struct ShadeArgs {
     const DMat* vmats;
     uint32_t max_depth, rr_depth;
     float clamp_indirect;
+    const float* energy;  // the GGX energy table (32 x 32), staged into LDS per block
 };
 
 // integrator.rs:50-221, one bounce
@@ -943,6 +947,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
     // count have nothing to shade (uniform per block, before any barrier)
     if (blockIdx.x * blockDim.x >= cnt[0]) return;
     __shared__ uint32_t pre[MAX_KEYS + 1];
+    // microfacet.rs:93-140 E(mu, sqrt(alpha)): read by every rough lobe's
+    // multiple-scattering term, several times per hit
+    __shared__ float s_energy[32 * 32];
+    for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x) s_energy[i] = A.energy[i];
     if (threadIdx.x == 0) {
         pre[0] = 0;
         for (int k = 0; k < Q.n_keys; k++) pre[k + 1] = pre[k] + cnt[2 + k];
@@ -955,10 +963,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
     float4* nd = Q.ray_d[(depth + 1) & 1];
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         uint32_t j = base + threadIdx.x;
-        bool emit_ray = false, emit_shadow = false;
+        bool emit_ray = false;
         f3 nro, nrd;
         uint32_t path = 0;
-        float4 sh_o, sh_d, sh_c, sh_x;
         if (j < total) {
             int key = 0;
             while (key + 1 < Q.n_keys && pre[key + 1] <= j) key++;
@@ -1080,7 +1087,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                         f3 dir = env_map_sample(S, u1, u2, &pdf_dir);
                         if (pdf_dir > 0.0f && dot(dir, ngu) > 0.0f) {
                             f3 wi = to_local(fr, dir);
-                            Spec f = bsdf_eval(bs, S.energy, wo, wi);
+                            Spec f = bsdf_eval(bs, s_energy, wo, wi);
                             if (!is_black(f)) {
                                 float p_light = S.env_pick * pdf_dir;
                                 float p_bsdf = bsdf_pdf(bs, wo, wi);
@@ -1088,11 +1095,13 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                                 f3 org = offset_origin(p, ngu, dir);
                                 Spec le = env_eval(S, dir, lam);
                                 Spec base = beta * f * le;
-                                emit_shadow = true;
-                                sh_o = make_float4(org.x, org.y, org.z, __uint_as_float(path));
-                                sh_d = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0xFFFFFFFFu));
-                                sh_c = to4(base);
-                                sh_x = make_float4(fabsf(wi.z) * w, p_light, rough ? cm : -1.0f, __uint_as_float(1u));
+                                // appended here (lanes in this branch only) so the
+                                // record does not stay live through the BSDF sample
+                                uint32_t ss = wave_append(&cnt[1], true);
+                                Q.sh_o[ss] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
+                                Q.sh_d[ss] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0xFFFFFFFFu));
+                                Q.sh_c[ss] = to4(base);
+                                Q.sh_x[ss] = make_float4(fabsf(wi.z) * w, p_light, rough ? cm : -1.0f, __uint_as_float(1u));
                             }
                         }
                     } else if (S.n_lights > 0) {
@@ -1111,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                         float cos_l = dot(-dir, ld3(Pn.n));
                         if (cos_l > 0.0f && dot(dir, ngu) > 0.0f) {
                             f3 wi = to_local(fr, dir);
-                            Spec f = bsdf_eval(bs, S.energy, wo, wi);
+                            Spec f = bsdf_eval(bs, s_energy, wo, wi);
                             if (!is_black(f)) {
                                 float p_light = pick * d2 / (cos_l * Pn.area);
                                 float p_bsdf = bsdf_pdf(bs, wo, wi);
@@ -1119,11 +1128,11 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                                 f3 org = offset_origin(p, ngu, dir);
                                 Spec le = panel_le(Pn, lp, dir, lam);
                                 Spec c = beta * f * le * (fabsf(wi.z) * w / p_light);
-                                emit_shadow = true;
-                                sh_o = make_float4(org.x, org.y, org.z, __uint_as_float(path));
-                                sh_d = make_float4(lp.x, lp.y, lp.z, __uint_as_float((uint32_t)li));
-                                sh_c = to4(c);
-                                sh_x = make_float4(0.0f, 0.0f, rough ? cm : -1.0f, __uint_as_float(0u));
+                                uint32_t ss = wave_append(&cnt[1], true);
+                                Q.sh_o[ss] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
+                                Q.sh_d[ss] = make_float4(lp.x, lp.y, lp.z, __uint_as_float((uint32_t)li));
+                                Q.sh_c[ss] = to4(c);
+                                Q.sh_x[ss] = make_float4(0.0f, 0.0f, rough ? cm : -1.0f, __uint_as_float(0u));
                             }
                         }
                     }
@@ -1131,7 +1140,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                 // continue the path (integrator.rs:180-218)
                 BSample sm;
                 float u0 = rng_dim(ps, si, dim + 4), u1 = rng_dim(ps, si, dim + 5), u2 = rng_dim(ps, si, dim + 6);
-                bool ok = bsdf_sample(bs, S.energy, wo, u0, u1, u2, &sm);
+                bool ok = bsdf_sample(bs, s_energy, wo, u0, u1, u2, &sm);
                 if (ok && (sm.pdf <= 0.0f || is_black(sm.f))) ok = false;
                 f3 wi_w;
                 if (ok) {
@@ -1173,13 +1182,6 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
         if (emit_ray) {
             no[slot] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(path));
             nd[slot] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
-        }
-        uint32_t sslot = wave_append(&cnt[1], emit_shadow);
-        if (emit_shadow) {
-            Q.sh_o[sslot] = sh_o;
-            Q.sh_d[sslot] = sh_d;
-            Q.sh_c[sslot] = sh_c;
-            Q.sh_x[sslot] = sh_x;
         }
     }
 }
@@ -1431,8 +1433,8 @@ void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueue
     hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spill);
 }
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
-                  float clamp_indirect, DPaths P, DQueues Q, int depth) {
-    ShadeArgs A{views, vmats, max_depth, rr_depth, clamp_indirect};
+                  float clamp_indirect, const float* energy, DPaths P, DQueues Q, int depth) {
+    ShadeArgs A{views, vmats, max_depth, rr_depth, clamp_indirect, energy};
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
 }
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,

@@ -39,7 +39,7 @@ void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t
 void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint2* spill);
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
-                  float clamp_indirect, DPaths P, DQueues Q, int depth);
+                  float clamp_indirect, const float* energy, DPaths P, DQueues Q, int depth);
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint2* spill);
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,

@@ -95,6 +95,7 @@ struct ag_ctx {
     std::vector<unsigned long long> depth_rays;  // [2 * TALLY_DEPTHS]: extension, then shadow
     std::vector<double> depth_ms;                // [3 * TALLY_DEPTHS]: extend, shade, shadow
     DevBuf tally_buf;
+    DevBuf energy_buf;  // the library's GGX energy table, staged into LDS by the shading kernel
     double last_query_ms = 0.0;
     hipEvent_t next_event() {
         if (ev_used == evpool.size()) {
@@ -140,6 +141,17 @@ struct ag_film {
 };
 
 // ------------------------------------------------------------------ context
+// The GGX energy table (microfacet.rs:93-140) is a fixed function of GGX;
+// every descriptor must carry the library's (ag_desc_init fills it).
+static const std::vector<float>& library_energy() {
+    static const std::vector<float> E = [] {
+        std::vector<float> e(1024);
+        ag_energy_table(e.data());
+        return e;
+    }();
+    return E;
+}
+
 extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     if (!out) return fail(AG_E_INVALID, "null output");
     int count = 0;
@@ -155,6 +167,15 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
         return fail(AG_E_DEVICE, "stream creation failed");
     }
     for (auto& e : c->ev) (void)hipEventCreate(&e);
+    {
+        const std::vector<float>& E = library_energy();
+        if (c->energy_buf.ensure(E.size() * 4) || hipMemcpy(c->energy_buf.p, E.data(), E.size() * 4,
+                                                            hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return fail(AG_E_DEVICE, "energy table upload failed");
+        }
+    }
     // 2^26 path slots (~19 GB of path state, queues and buckets on a 288 GB
     // MI355X): few waves per render, so the long tail of the last bounces --
     // a handful of rays keeping the GPU for a dispatch each -- is paid rarely.
@@ -345,6 +366,8 @@ static void pack_material(const ag_material& a, float* dst) {
 
 static int validate_desc(const ag_scene_desc* d) {
     if (!d) return fail(AG_E_INVALID, "null descriptor");
+    if (std::memcmp(d->energy, library_energy().data(), 1024 * sizeof(float)) != 0)
+        return fail(AG_E_INVALID, "energy table differs from ag_energy_table()");
     if (d->n_triangles && (!d->positions || !d->triangles)) return fail(AG_E_INVALID, "missing mesh arrays");
     if (d->n_materials < 1 || d->n_materials > AG_MAX_MATERIALS) return fail(AG_E_INVALID, "n_materials must be 1..32");
     if (d->n_panels < 0 || d->n_panels > AG_MAX_PANELS) return fail(AG_E_INVALID, "n_panels must be 0..16");
@@ -849,7 +872,8 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
         for (uint32_t d = 0; d < ps->max_depth; d++) {
             timed(0, (int)d, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, c->spill_buf.as<uint2>()); });
             timed(1, (int)d, [&] {
-                launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, ps->rr_depth, ps->clamp_indirect, c->P,
+                launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, ps->rr_depth, ps->clamp_indirect,
+                             c->energy_buf.as<float>(), c->P,
                              c->Q, (int)d);
             });
             timed(2, (int)d, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, c->spill_buf.as<uint2>()); });

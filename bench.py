@@ -92,6 +92,7 @@ def _cpu_model():
     return "unknown"
 
 
+WORKLOAD_NAMES = {"signet": "signet still", "torus": "torus ring still", "gems": "signet with 12 brilliants still"}
 EXTEND_KERNEL = "ag::k_extend("
 COPY_KERNEL = "neg_kernel"
 
@@ -211,6 +212,54 @@ def turntable(args):
     print(json.dumps(res), flush=True)
 
 
+def ray_batch(args):
+    """The 2^24-ray batch against the signet (origins uniform on a 30 mm
+    sphere, directions toward the centre jittered up to 20 degrees), rays and
+    hits resident in HBM: closest hit and any hit, and the oracle's closest
+    hit on the host's cores over a bounded subset."""
+    import argent_amd as ag
+    from argent_amd import synth
+    pos, tris, nrm = synth.signet(seed=2)
+    c = (pos.min(0) + pos.max(0)) * 0.5
+    n = 1 << 24
+    o, d = synth.sphere_rays(n, c, seed=3)
+    desc = ag.SceneDesc(pos, tris, normals=nrm)
+    desc.add_material(ag.Material.metal("sterling"))
+    ctx = ag.Context(0)
+    sc = ag.Scene(ctx, desc)
+    rays = ag.DeviceRays(ctx, o, d)
+    for _ in range(args.warmup):
+        ag.intersect_device(ctx, sc, rays)
+    ms = [ag.intersect_device(ctx, sc, rays) for _ in range(max(args.steps, 1))]
+    ms_any = [ag.occluded_device(ctx, sc, rays) for _ in range(max(args.steps, 1))]
+    t_ms = sum(ms) / len(ms)
+    achieved = n * 44 / (t_ms * 1e-3) / 1e9
+    res = {"metric": "Grays/s closest hit, 2^24 rays vs the 1.2 M-triangle signet, rays resident in HBM, 1 GPU",
+           "value": round(n / (t_ms * 1e-3) / 1e9, 3), "unit": "Grays/s", "n_gpus": 1, "steps": len(ms),
+           "warmup": args.warmup, "ms_per_step": round(t_ms, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "any_hit_grays_per_s": round(n / (sum(ms_any) / len(ms_any) * 1e-3) / 1e9, 3),
+           "config": {"workload": "2^24 rays, origins on a 30 mm sphere, toward the centre +- 20 degrees",
+                      "rays": n, "triangles": int(len(tris))},
+           "roofline": {"kernel": "k_rays_intersect", "bound": "hbm", "achieved": round(achieved, 1),
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+                        "traffic": None, "bytes_per_unit": 44, "unit_of_work": "ray (28 B in, 16 B hit out)"}}
+    rays.close()
+    sc.close()
+    if not args.no_cpu:
+        import oracle as O
+        osc = O.OracleScene(desc)
+        threads = min(16, os.cpu_count() or 1)
+        m = 1 << 20
+        t0 = time.perf_counter()
+        osc.intersect(o[:m], d[:m], mode=O.MODE_ROBUST, threads=threads)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(m / dt / 1e9, 4), "unit": "Grays/s", "cores": threads, "kind": "port",
+                               "sample": f"the first 2^20 of the rays, {dt:.1f} s", "cpu_model": _cpu_model()}
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,7 +267,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default: the workload's)")
+    ap.add_argument("--workload", choices=["signet", "torus", "gems"], default="signet",
+                    help="signet: the still above (default); torus: the 131 K-triangle ring, 512x512, 64 spp, "
+                         "noir, the reference's CPU-runnable case; gems: the signet with 12 brilliants, an HDRI "
+                         "and the white rig's softboxes, 512 spp, depth 24")
+    ap.add_argument("--rays", action="store_true", help="the 2^24-ray closest/any-hit batch instead of a still")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -236,6 +290,9 @@ def main():
     if args.turntable:
         turntable(args)
         return
+    if args.rays:
+        ray_batch(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "
               f"supported; a still shards by image tiles or sample ranges, a server by requests", file=sys.stderr)
@@ -248,15 +305,20 @@ def main():
     from argent_amd import workloads
 
     t0 = time.time()
-    wl = workloads.signet_still()
+    wl = {"signet": workloads.signet_still, "torus": workloads.torus_still,
+          "gems": workloads.signet_gems_still}[args.workload]()
     piece = wl["piece"]
-    w, h, spp, depth, seed = wl["width"], wl["height"], args.spp, wl["depth"], wl["seed"]
-    desc = piece.scene(wl["metal"], 0.07, 0.8, wl["studio"])
+    w, h, depth, seed = wl["width"], wl["height"], wl["depth"], wl["seed"]
+    spp = args.spp or wl["spp"]
+    desc = piece.scene(wl["metal"], 0.07, 0.8, wl["studio"], gems=wl.get("gems", ()))
+    if "hdri" in wl:
+        desc.set_env_map(wl["hdri"])
     ps = workloads.path_settings(depth)
     ctx = ag.Context(0)
-    cam, turn = piece.shot("hero", 100.0, wl["f_stop"], w / h)
-    desc.d.turntable_deg = turn
     scene = ag.Scene(ctx, desc)
+    cam, turn = piece.shot("hero", 100.0, wl["f_stop"], w / h, intersect=scene.intersect)  # autofocus at f/16
+    scene.set_turntable(turn)
+    desc.d.turntable_deg = turn
     stats = scene.stats()
     log(f"scene: {len(piece.triangles)} triangles, BVH {stats['nodes']} nodes depth {stats['depth']}, "
         f"{stats['bytes'] / 2**20:.0f} MiB on device, built in {time.time() - t0:.1f} s")
@@ -296,7 +358,8 @@ def main():
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("bytes_per_dispatch")
     result = {
-        "metric": "Msamples/s at 1080x1920 256spp vs CPU; served p50/p99 latency, 1 GPU",
+        "metric": ("Msamples/s at 1080x1920 256spp vs CPU; served p50/p99 latency, 1 GPU" if args.workload == "signet"
+                   else "Msamples/s, 1 GPU"),
         "value": round(msps, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -305,12 +368,13 @@ def main():
         "ms_per_step": round(elapsed / args.steps * 1e3, 2),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(msps / 17.7, 3),
+        "vs_baseline": round(msps / 17.7, 3) if args.workload == "signet" and spp == 256 else None,
         "vs_baseline_note": "published CPU still rate: 1920x1080 256 spp in ~30 s on 32 cores = 17.7 Msamples/s "
                             "(README.md:4-6)",
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"signet still {w}x{h} {spp}spp depth {depth}, sterling, white studio, "
+        "config": {"workload": f"{WORKLOAD_NAMES[args.workload]} {w}x{h} {spp}spp depth {depth}, {wl['metal']}, "
+                               f"{wl['studio']} studio{', HDRI + ' + ' and '.join(wl['gems']) if 'gems' in wl else ''}, "
                                f"{len(piece.triangles)} triangles (seeded procedural)",
                    "width": w, "height": h, "spp": spp, "max_depth": depth, "triangles": int(len(piece.triangles))},
         "rays_per_s": round(rays / elapsed / 1e6, 1),
@@ -335,7 +399,7 @@ def main():
     }
     film.close()
     scene.close()
-    if args.serve_seconds > 0:
+    if args.serve_seconds > 0 and args.workload == "signet":
         from argent_amd import serve
         trace = serve.make_trace(args.serve_seconds, args.serve_rate)
         log(f"serving: {len(trace)} requests over {args.serve_seconds:.0f} s, building meshes ...")
