@@ -12,6 +12,17 @@
 
 namespace ag {
 
+// Address spaces for device pointers whose home is known (LDS, global): reads
+// through them are ds_read / global_load, never flat loads, which wait on the
+// LDS and scalar counters as well.  Empty in the host pass.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define AG_LDS __attribute__((address_space(3)))
+#define AG_GLOBAL __attribute__((address_space(1)))
+#else
+#define AG_LDS
+#define AG_GLOBAL
+#endif
+
 constexpr int MAX_KEYS = 2 + AG_MAX_MATERIALS;  // miss, panel, one per material
 constexpr int TRAV_BLOCK = 256;                  // threads per traversal block
 #ifndef AG_STACK_LDS

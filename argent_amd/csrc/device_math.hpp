@@ -148,17 +148,35 @@ AGD uint32_t lk_permute(uint32_t x, uint32_t seed) {
     return x;
 }
 AGD uint32_t nested_scramble(uint32_t x, uint32_t seed) { return __brev(lk_permute(__brev(x), seed)); }
-AGD float rng_dim(uint32_t pix_seed, uint32_t si, uint32_t dim) {
+// Byte tables for Sobol dimensions 1..3 of a set: word [k-1][j][v] is the XOR
+// of the direction numbers SOBOL_DIRS[k][8j + b] over the set bits b of the
+// byte value v, so a sample is four table reads instead of a 32-step loop of
+// dependent loads.  A kernel fills them into LDS once per block.
+constexpr int SOBOL_TAB_WORDS = 3 * 4 * 256;
+AGD void sobol_tables_fill(AG_LDS uint32_t* tab) {
+    for (int e = (int)threadIdx.x; e < SOBOL_TAB_WORDS; e += (int)blockDim.x) {
+        const int k = 1 + e / 1024, j = (e >> 8) & 3, v = e & 255;
+        uint32_t x = 0;
+        for (int b = 0; b < 8; b++)
+            if ((v >> b) & 1) x ^= SOBOL_DIRS[k][8 * j + b];
+        tab[e] = x;
+    }
+}
+AGD float rng_dim(const AG_LDS uint32_t* sob, uint32_t pix_seed, uint32_t si, uint32_t dim) {
     if (dim >= 256u || si >= 65536u) {
         uint32_t h = hash32(pix_seed ^ hash32(si ^ hash32(dim ^ 0x2f0b6e35u)));
         return (float)(h >> 8) * (1.0f / 16777216.0f);
     }
     uint32_t set = dim >> 2, k = dim & 3u;
     uint32_t s = hash32(pix_seed ^ hash32(set * 0x9e3779b9u + 0x632be5abu));
-    uint32_t i = nested_scramble(si, s);
-    uint32_t x = 0;
-    for (int b = 0; b < 32 && i; b++, i >>= 1)
-        if (i & 1u) x ^= SOBOL_DIRS[k][b];
+    const uint32_t i = nested_scramble(si, s);
+    uint32_t x;  // XOR of SOBOL_DIRS[k][b] over the set bits b of i
+    if (k == 0) {
+        x = __brev(i);  // dimension 0: direction number b is 2^(31-b)
+    } else {
+        const AG_LDS uint32_t* t = sob + (k - 1u) * 1024u;
+        x = t[i & 255u] ^ t[256u + ((i >> 8) & 255u)] ^ t[512u + ((i >> 16) & 255u)] ^ t[768u + (i >> 24)];
+    }
     uint32_t hk = hash32(k + 1u);
     x = nested_scramble(x, s ^ (hk + (s << 6) + (s >> 2)));
     return (float)(x >> 8) * (1.0f / 16777216.0f);

@@ -36,7 +36,12 @@ AGD void per_uniform(uint32_t key, F&& f) {
     }
 }
 
-// Read *p through the constant address space.  With a wavefront-uniform p
+// Read *p through the constant address space.  The view, scene and material
+// blocks are read this way wherever a kernel reads them per lane too: the
+// loads are then known not to change while the kernel runs (no reload after
+// the kernel's own stores), and a pointer read this way is known to point to
+// global memory, so what is read through it takes global loads, not flat ones
+// (which wait on the LDS and scalar counters as well and serialise).  With a wavefront-uniform p
 // (inside per_uniform) the compiler then issues scalar loads into SGPRs; the
 // scene and view blocks are read-only while a kernel runs.
 template <class T>
@@ -46,6 +51,46 @@ AGD T uload(const T* p) {
 #else
     return *p;  // host pass of this file: never executed
 #endif
+}
+
+// Field by field through uload: a whole-struct copy through the constant
+// space is lowered to flat loads, and the table pointers inside would lose
+// their global address space.
+AGD DMat load_mat(const DMat* p) {
+    DMat m;
+    m.kind = uload(&p->kind);
+    m.alpha = uload(&p->alpha);
+    m.eta = uload(&p->eta);
+    m.tn = uload(&p->tn);
+    m.tk = uload(&p->tk);
+    m.has_patina = uload(&p->has_patina);
+    m.p_strength = uload(&p->p_strength);
+    m.p_lo = uload(&p->p_lo);
+    m.p_hi = uload(&p->p_hi);
+    m.p_alpha = uload(&p->p_alpha);
+    m.p_eta = uload(&p->p_eta);
+    m.tpatina = uload(&p->tpatina);
+    m.ta = uload(&p->ta);
+    m.tb = uload(&p->tb);
+    m.opal_cell = uload(&p->opal_cell);
+    m.opal_strength = uload(&p->opal_strength);
+    return m;
+}
+AGD DPanel load_panel(const DPanel* p) {
+    DPanel q;
+    for (int i = 0; i < 3; i++) {
+        q.c[i] = uload(&p->c[i]);
+        q.hu[i] = uload(&p->hu[i]);
+        q.hv[i] = uload(&p->hv[i]);
+        q.n[i] = uload(&p->n[i]);
+    }
+    q.area = uload(&p->area);
+    q.hu2 = uload(&p->hu2);
+    q.hv2 = uload(&p->hv2);
+    q.emit = uload(&p->emit);
+    q.camera_visible = uload(&p->camera_visible);
+    q.feather = uload(&p->feather);
+    return q;
 }
 
 // Append for every lane with pred set; returns the slot (valid where pred).
@@ -91,10 +136,13 @@ AGD uint32_t wave_append_keyed(uint32_t* counters, int key) {
 // depend on how the tree groups triangles.  The triangle test is
 // Moller-Trumbore evaluated exactly as the reference does (no contraction),
 // with ties on t resolved to the lower original triangle index.
+// The stack's two homes carry their address spaces in the pointer types, so
+// a pop is an LDS read or a global read: with plain pointers the compiler
+// merges the two branches into one flat load, which waits on both counters.
 struct TravStack {
-    uint32_t* node;  // LDS [STACK_LDS][blockDim]
-    float* tn;       // LDS [STACK_LDS][blockDim]
-    uint2* spill;    // HBM [STACK_SPILL] for this thread
+    AG_LDS uint32_t* node;  // LDS [STACK_LDS][blockDim]
+    AG_LDS float* tn;       // LDS [STACK_LDS][blockDim]
+    AG_GLOBAL uint2* spill; // HBM [STACK_SPILL] for this thread
     uint32_t tid, nthreads;
 };
 
@@ -391,8 +439,18 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
     // wavefronts beyond what n rays can occupy (GRAB each) leave at once
     uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
     bool more = wave < (n + GRAB - 1) / GRAB;  // uniform across the wavefront
+    // A lane whose ray has finished traversing stays parked (idx kept, L.cur
+    // == TRAV_DONE) until the next refill, when all parked lanes are finished
+    // in one call: a job's finish reads memory (a material id, a radiance
+    // slot), and one round trip per refill costs far less than one per step
+    // at which some lane happened to finish.
     for (;;) {
-        if (more && (uint32_t)__popcll(__ballot(idx == NO_RAY)) >= refill_min) {
+        bool parked = idx != NO_RAY && L.cur == TRAV_DONE;
+        if (more && (uint32_t)__popcll(__ballot(idx == NO_RAY || parked)) >= refill_min) {
+            if (__ballot(parked) != 0) {
+                J.finish(idx, L, parked);
+                if (parked) idx = NO_RAY;
+            }
             // Fill the idle lanes with the next rays in queue order; a ray its
             // job settles without the piece finishes at once and its lane takes
             // the next one, so traversal batches hold only rays that need it.
@@ -422,17 +480,22 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
                 if (fin) idx = NO_RAY;
             }
         }
-        if (__ballot(idx != NO_RAY) == 0) break;  // only once every slice is spent
+        if (__ballot(idx != NO_RAY && L.cur != TRAV_DONE) == 0) {
+            // nothing traversing: finish the parked lanes; leave once the
+            // queue is spent, else go round to refill every lane
+            bool last = idx != NO_RAY;
+            if (__ballot(last) != 0) {
+                J.finish(idx, L, last);
+                if (last) idx = NO_RAY;
+            }
+            if (!more) break;
+            continue;
+        }
         if (idx != NO_RAY && L.cur != TRAV_DONE) {
             if (ANY)
                 step_any(L, st);
             else
                 step_closest(L, st);
-        }
-        bool fin = idx != NO_RAY && L.cur == TRAV_DONE;
-        if (__ballot(fin) != 0) {
-            J.finish(idx, L, fin);
-            if (fin) idx = NO_RAY;
         }
     }
     J.drain();
@@ -450,11 +513,11 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
     __shared__ uint32_t s_node[STACK_LDS * TRAV_BLOCK];                     \
     __shared__ float s_tn[STACK_LDS * TRAV_BLOCK];                          \
     TravStack st;                                                           \
-    st.node = s_node;                                                       \
-    st.tn = s_tn;                                                           \
+    st.node = (AG_LDS uint32_t*)s_node;                                     \
+    st.tn = (AG_LDS float*)s_tn;                                            \
     st.tid = threadIdx.x;                                                   \
     st.nthreads = blockDim.x;                                               \
-    st.spill = spill + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * STACK_SPILL;
+    st.spill = (AG_GLOBAL uint2*)(spill + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * STACK_SPILL);
 
 // ------------------------------------------------------------------ ray-batch kernels
 struct BatchJob {
@@ -502,28 +565,45 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_rays_occluded(const f
 }
 
 // ------------------------------------------------------------------ scene pieces
-// light.rs:40-54
-AGD bool panel_intersect(const DPanel& P, f3 o, f3 d, float tmax, float* tout) {
-    f3 n = ld3(P.n);
+// fabsf(x / y) <= 1 with x / y rounded to nearest, as the reference writes it
+// (light.rs:60-62), without dividing unless |x| lies within 2^-20 of y.  For
+// y in [2^-100, 2^100]: |x| <= fl(y (1 - 2^-20)) gives |x| / y < 1, and
+// |x| > fl(y (1 + 2^-20)) gives |x| / y > 1 + 2^-21, whose rounding exceeds 1.
+// A NaN x fails both and divides to NaN, as in the reference.
+AGD bool within_unit(float x, float y) {
+    const float ax = fabsf(x);
+    if (y >= 0x1p-100f && y <= 0x1p100f) {
+        if (ax <= y * (1.0f - 0x1p-20f)) return true;
+        if (ax > y * (1.0f + 0x1p-20f)) return false;
+    }
+    return fabsf(x / y) <= 1.0f;
+}
+
+// light.rs:49-63.  P points into the scene block; with a wavefront-uniform P
+// every field comes in by a scalar load (a whole-struct copy of DPanel is not
+// reliably kept scalar by the compiler and turns into vector loads).
+AGD bool panel_intersect(const DPanel* P, f3 o, f3 d, float tmax, float* tout) {
+    const f3 n{uload(&P->n[0]), uload(&P->n[1]), uload(&P->n[2])};
     float denom = dot(d, n);
     if (fabsf(denom) < 1e-9f) return false;
-    float num = dot(ld3(P.c) - o, n);
+    const f3 c{uload(&P->c[0]), uload(&P->c[1]), uload(&P->c[2])};
+    float num = dot(c - o, n);
     // t = num / denom > 1e-6 needs num and denom of one sign: reject the
     // planes behind the ray before paying for the division (same result,
     // NaN included: a NaN num either fails here or divides to NaN below)
     if ((num > 0.0f) != (denom > 0.0f) || num == 0.0f) return false;
     float t = num / denom;
     if (t <= 1e-6f || t >= tmax) return false;
-    f3 p = (o + d * t) - ld3(P.c);
-    float a = dot(p, ld3(P.hu)) / P.hu2;
-    float b = dot(p, ld3(P.hv)) / P.hv2;
-    if (fabsf(a) <= 1.0f && fabsf(b) <= 1.0f) {
+    f3 p = (o + d * t) - c;
+    const f3 hu{uload(&P->hu[0]), uload(&P->hu[1]), uload(&P->hu[2])};
+    const f3 hv{uload(&P->hv[0]), uload(&P->hv[1]), uload(&P->hv[2])};
+    if (within_unit(dot(p, hu), uload(&P->hu2)) && within_unit(dot(p, hv), uload(&P->hv2))) {
         *tout = t;
         return true;
     }
     return false;
 }
-// light.rs:57-87
+// light.rs:74-94 (Panel::profile)
 AGD float panel_profile(const DPanel& P, f3 p) {
     if (P.feather <= 0.0f) return 1.0f;
     f3 q = p - ld3(P.c);
@@ -541,6 +621,7 @@ AGD float panel_profile(const DPanel& P, f3 p) {
     }
     return fa * fb;
 }
+// light.rs:65-73 (Panel::le)
 AGD Spec panel_le(const DPanel& P, f3 p, f3 dir, const Spec& lam) {
     if (P.emit && dot(dir, ld3(P.n)) < 0.0f) return tab_sample(P.emit, lam) * panel_profile(P, p);
     return splat(0.0f);
@@ -548,7 +629,7 @@ AGD Spec panel_le(const DPanel& P, f3 p, f3 dir, const Spec& lam) {
 // light.rs:133-162
 AGD bool floor_intersect(const DScene& S, f3 o, f3 d, float tmax, float* tout) {
     if (fabsf(d.y) < 1e-9f) return false;
-    float t = (S.floor_y - o.y) / d.y;
+    float t = (uload(&S.floor_y) - o.y) / d.y;
     if (t > 1e-5f && t < tmax) {
         *tout = t;
         return true;
@@ -557,9 +638,10 @@ AGD bool floor_intersect(const DScene& S, f3 o, f3 d, float tmax, float* tout) {
 }
 AGD float floor_presence(const DScene& S, f3 p) {
     float r = sqrtf(p.x * p.x + p.z * p.z);
-    if (r <= S.floor_f0) return 1.0f;
-    if (r >= S.floor_f1) return 0.0f;
-    float t = (r - S.floor_f0) / (S.floor_f1 - S.floor_f0);
+    const float f0 = uload(&S.floor_f0), f1 = uload(&S.floor_f1);
+    if (r <= f0) return 1.0f;
+    if (r >= f1) return 0.0f;
+    float t = (r - f0) / (f1 - f0);
     return 1.0f - t * t * (3.0f - 2.0f * t);
 }
 
@@ -577,7 +659,7 @@ AGD float linrgb_eval(const float* basis, const float* w, float l) {
 }
 // envmap.rs:182-204
 AGD void env_uv(const DScene& S, f3 dir, float* u, float* v) {
-    f3 d = rot_y(dir, S.env_rsn, S.env_rcn);
+    f3 d = rot_y(dir, uload(&S.env_rsn), uload(&S.env_rcn));
     float uu = 0.5f + atan2f(d.x, -d.z) / (2.0f * PI_F);
     *v = acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) / PI_F;
     float r = fmodf(uu, 1.0f);
@@ -588,18 +670,20 @@ AGD f3 env_dir(const DScene& S, float u, float v) {
     float theta = v * PI_F;
     float st = sinf(theta), ct = cosf(theta);
     f3 d{st * sinf(phi), ct, -st * cosf(phi)};
-    return rot_y(d, S.env_rs, S.env_rc);
+    return rot_y(d, uload(&S.env_rs), uload(&S.env_rc));
 }
 AGD float env_map_pdf(const DScene& S, f3 dir) {
     float u, v;
     env_uv(S, dir, &u, &v);
     float st = sinf(PI_F * v);
     if (st <= 1e-6f) return 0.0f;
-    int x = min((int)(u * (float)S.env_w), S.env_w - 1);
-    int y = min((int)(v * (float)S.env_h), S.env_h - 1);
-    const float* row = S.env_cond + (size_t)y * (S.env_w + 1);
-    float pc = (row[x + 1] - row[x]) * (float)S.env_w;
-    float pm = (S.env_marg[y + 1] - S.env_marg[y]) * (float)S.env_h;
+    const int ew = uload(&S.env_w), eh = uload(&S.env_h);
+    const float* marg = uload(&S.env_marg);
+    int x = min((int)(u * (float)ew), ew - 1);
+    int y = min((int)(v * (float)eh), eh - 1);
+    const float* row = uload(&S.env_cond) + (size_t)y * (ew + 1);
+    float pc = (row[x + 1] - row[x]) * (float)ew;
+    float pm = (marg[y + 1] - marg[y]) * (float)eh;
     return pc * pm / (2.0f * PI_F * PI_F * st);
 }
 // number of entries <= u in a sorted array (partition_point(|c| c <= u))
@@ -615,11 +699,12 @@ AGD int count_le(const float* a, int n, float u) {
     return lo;
 }
 AGD f3 env_map_sample(const DScene& S, float u1, float u2, float* pdf) {
-    int h = S.env_h, w = S.env_w;
-    int y = min(max(count_le(S.env_marg, h + 1, u1), 1), h) - 1;
-    const float* cond = S.env_cond + (size_t)y * (w + 1);
+    const int h = uload(&S.env_h), w = uload(&S.env_w);
+    const float* marg = uload(&S.env_marg);
+    int y = min(max(count_le(marg, h + 1, u1), 1), h) - 1;
+    const float* cond = uload(&S.env_cond) + (size_t)y * (w + 1);
     int x = min(max(count_le(cond, w + 1, u2), 1), w) - 1;
-    float fy = (u1 - S.env_marg[y]) / fmaxf(S.env_marg[y + 1] - S.env_marg[y], 1e-12f);
+    float fy = (u1 - marg[y]) / fmaxf(marg[y + 1] - marg[y], 1e-12f);
     float fx = (u2 - cond[x]) / fmaxf(cond[x + 1] - cond[x], 1e-12f);
     float u = ((float)x + fminf(fmaxf(fx, 0.0f), 0.999f)) / (float)w;
     float v = ((float)y + fminf(fmaxf(fy, 0.0f), 0.999f)) / (float)h;
@@ -629,34 +714,37 @@ AGD f3 env_map_sample(const DScene& S, float u1, float u2, float* pdf) {
 }
 // light.rs:110-127 (Env::eval)
 AGD Spec env_eval(const DScene& S, f3 dir, const Spec& lam) {
-    if (S.env_kind == AG_ENV_MAP) {
+    if (uload(&S.env_kind) == AG_ENV_MAP) {
         float u, v;
         env_uv(S, dir, &u, &v);
-        int x = min((int)(u * (float)S.env_w), S.env_w - 1);
-        int y = min((int)(v * (float)S.env_h), S.env_h - 1);
-        const float* w = S.env_weights + ((size_t)y * S.env_w + x) * 3;
+        const int ew = uload(&S.env_w), eh = uload(&S.env_h);
+        int x = min((int)(u * (float)ew), ew - 1);
+        int y = min((int)(v * (float)eh), eh - 1);
+        const float* w = uload(&S.env_weights) + ((size_t)y * ew + x) * 3;
+        const float* basis = uload(&S.env_basis);
         Spec r;
-        for (int i = 0; i < 4; i++) r.v[i] = linrgb_eval(S.env_basis, w, lam.v[i]);
+        for (int i = 0; i < 4; i++) r.v[i] = linrgb_eval(basis, w, lam.v[i]);
         return r;
     }
     float y = fminf(fmaxf(dir.y, -1.0f), 1.0f);
     if (y >= 0.0f) {
-        float t = powf(y, S.env_power);
-        return lerp(t, tab_sample(S.env_hor, lam), tab_sample(S.env_top, lam));
+        float t = powf(y, uload(&S.env_power));
+        return lerp(t, tab_sample(uload(&S.env_hor), lam), tab_sample(uload(&S.env_top), lam));
     }
     float t = powf(-y, 0.5f);
-    return lerp(t, tab_sample(S.env_hor, lam), tab_sample(S.env_bot, lam));
+    return lerp(t, tab_sample(uload(&S.env_hor), lam), tab_sample(uload(&S.env_bot), lam));
 }
 
 AGD float light_pick_pdf(const DScene& S, int panel) {
-    for (int k = 0; k < S.n_lights; k++)
-        if (S.lights[k] == panel) return S.light_cdf[k] - (k == 0 ? 0.0f : S.light_cdf[k - 1]);
+    const int nl = uload(&S.n_lights);
+    for (int k = 0; k < nl; k++)
+        if (uload(&S.lights[k]) == panel) return uload(&S.light_cdf[k]) - (k == 0 ? 0.0f : uload(&S.light_cdf[k - 1]));
     return 0.0f;
 }
 
-AGD const DMat& material_of(const DScene& S, const DMat* vmats, const DView& V, int m) {
-    if (m == 0 && V.mat0 >= 0) return vmats[V.mat0];
-    return S.mat[m];
+AGD DMat material_of(const DScene& S, const DMat* vmats, const DView& V, int m) {
+    const int m0 = uload(&V.mat0);
+    return load_mat(m == 0 && m0 >= 0 ? &vmats[m0] : &S.mat[m]);
 }
 
 // material.rs:96-138
@@ -703,7 +791,7 @@ AGD Bsdf material_bsdf(const DMat& M, const Spec& lam, float mask, float u, f3 p
             f3 dir;
             float spacing, brightness;
             opal_domain(p_obj / M.opal_cell, &dir, &spacing, &brightness);
-            f3 nw = normalize(n_world * 1.2f + rot_y(dir, V.ts, V.tc));
+            f3 nw = normalize(n_world * 1.2f + rot_y(dir, uload(&V.ts), uload(&V.tc)));
             f3 n = to_local(fr, nw);
             if (n.z < 0.0f) n = -n;
             b.kind = B_OPAL;
@@ -766,6 +854,10 @@ AGD int find_segment(const DSeg* segs, int nseg, uint32_t path) {
 // render.rs:67-95 (per sample: pixel seed, jitter, lens, wavelengths, camera ray)
 __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, int nseg, uint32_t npaths,
                                                 const DView* __restrict__ views, DPaths P, DQueues Q) {
+    __shared__ uint32_t s_sobol[SOBOL_TAB_WORDS];
+    const AG_LDS uint32_t* sob = (AG_LDS uint32_t*)s_sobol;
+    sobol_tables_fill((AG_LDS uint32_t*)s_sobol);
+    __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[0] = npaths;
     for (uint32_t path = blockIdx.x * blockDim.x + threadIdx.x; path < npaths; path += gridDim.x * blockDim.x) {
         const DSeg& sg = segs[find_segment(segs, nseg, path)];
@@ -775,11 +867,11 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
         const DView& V = views[sg.view];
         uint32_t x = pix % V.width, y = pix / V.width;
         uint32_t ps = hash32((y * V.width + x) ^ V.seed_h);
-        float jx = tent(rng_dim(ps, si, 0), 1.0f);
-        float jy = tent(rng_dim(ps, si, 1), 1.0f);
-        float lu = rng_dim(ps, si, 2), lv = rng_dim(ps, si, 3);
+        float jx = tent(rng_dim(sob, ps, si, 0), 1.0f);
+        float jy = tent(rng_dim(sob, ps, si, 1), 1.0f);
+        float lu = rng_dim(sob, ps, si, 2), lv = rng_dim(sob, ps, si, 3);
         Spec lam, pdf;
-        sample_wavelengths(rng_dim(ps, si, 4), lam, pdf);
+        sample_wavelengths(rng_dim(sob, ps, si, 4), lam, pdf);
         float sx = (((float)x + 0.5f + jx) * V.aspect_sx) - 1.0f;
         float sy = 1.0f - (((float)y + 0.5f + jy) * V.aspect_sy);
         // camera.rs:57-70
@@ -843,8 +935,8 @@ struct ExtendJob {
             uint32_t code = HIT_MISS;
             const int np = uload(&Sp->n_panels);
             for (int k = 0; k < np; k++) {
-                const DPanel pk = uload(&Sp->panel[k]);
-                if (depth == 0 && !pk.camera_visible) continue;
+                const DPanel* pk = &Sp->panel[k];
+                if (depth == 0 && !uload(&pk->camera_visible)) continue;
                 float t;
                 if (panel_intersect(pk, o, d, tmax, &t)) {
                     tmax = t;
@@ -951,6 +1043,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
     // multiple-scattering term, several times per hit
     __shared__ float s_energy[32 * 32];
     for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x) s_energy[i] = A.energy[i];
+    __shared__ uint32_t s_sobol[SOBOL_TAB_WORDS];
+    const AG_LDS uint32_t* sob = (AG_LDS uint32_t*)s_sobol;
+    sobol_tables_fill((AG_LDS uint32_t*)s_sobol);
     if (threadIdx.x == 0) {
         pre[0] = 0;
         for (int k = 0; k < Q.n_keys; k++) pre[k + 1] = pre[k] + cnt[2 + k];
@@ -980,20 +1075,23 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
             uint32_t flags = P.flags[path];
             uint64_t rng = P.rng[path];
             const uint32_t ps = P.pix_seed[path], si = P.si[path];
-            const DView& V = A.views[P.view[path]];
-            const DScene& S = *V.scene;
+            const DView& V = A.views[P.view[path]];  // fields read below with uload
+            const DScene& S = *uload(&V.scene);
+            const float turn = uload(&V.turn_deg), vts = uload(&V.ts), vtc = uload(&V.tc);
+            const float env_pick = uload(&S.env_pick);
+            const bool env_map = uload(&S.env_kind) == AG_ENV_MAP;
             bool specular = flags & 1u, rough = (flags & 2u) != 0;
             int medium = (int)((flags >> 8) & 255u);
             const bool clamp = rough;
             const float cm = A.clamp_indirect;
             bool alive = true;
             if (code != HIT_MISS && medium != 255) {
-                Spec sig = tab_sample(S.mat[medium].tb, lam);
+                Spec sig = tab_sample(uload(&S.mat[medium].tb), lam);
                 for (int i = 0; i < 4; i++) beta.v[i] = beta.v[i] * expf(-sig.v[i] * t);
             }
             auto env_weight = [&](f3 dir) -> float {
-                if (S.env_kind == AG_ENV_MAP && !specular) {
-                    float pl = S.env_pick * env_map_pdf(S, dir);
+                if (env_map && !specular) {
+                    float pl = env_pick * env_map_pdf(S, dir);
                     return prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl);
                 }
                 return 1.0f;
@@ -1014,10 +1112,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                     p_obj = p;
                     ng = n;
                     ns = n;
-                    mat_id = S.floor_mat;
+                    mat_id = uload(&S.floor_mat);
                 }
             } else if (code & HIT_PANEL) {
-                const DPanel& Pn = S.panel[code & 0xFFFFu];
+                const DPanel Pn = load_panel(&S.panel[code & 0xFFFFu]);
                 Spec le = panel_le(Pn, o + d * t, d, lam);
                 if (!is_black(le)) {
                     if (specular) {
@@ -1025,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                     } else {
                         float cos_l = fmaxf(fabsf(dot(-d, ld3(Pn.n))), 1e-8f);
                         float p_light =
-                            (1.0f - S.env_pick) * light_pick_pdf(S, (int)(code & 0xFFFFu)) * t * t / (cos_l * Pn.area);
+                            (1.0f - env_pick) * light_pick_pdf(S, (int)(code & 0xFFFFu)) * t * t / (cos_l * Pn.area);
                         float w = prev_pdf * prev_pdf / (prev_pdf * prev_pdf + p_light * p_light);
                         add_clamped(L, beta * le * w, clamp, cm);
                     }
@@ -1034,25 +1132,29 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
             } else {
                 // scene.rs:173-201: attributes in object space, then to world
                 uint32_t prim = code;
-                uint32_t i0 = S.idx[3 * prim], i1 = S.idx[3 * prim + 1], i2 = S.idx[3 * prim + 2];
-                f3 pa = ld3(S.pos + 3 * i0), pb = ld3(S.pos + 3 * i1), pc = ld3(S.pos + 3 * i2);
+                const uint32_t* idx = uload(&S.idx);
+                const float* pos = uload(&S.pos);
+                const float* nrm = uload(&S.nrm);
+                const float* cav = uload(&S.cavity);
+                uint32_t i0 = idx[3 * prim], i1 = idx[3 * prim + 1], i2 = idx[3 * prim + 2];
+                f3 pa = ld3(pos + 3 * i0), pb = ld3(pos + 3 * i1), pc = ld3(pos + 3 * i2);
                 f3 ngo = normalize(cross(pb - pa, pc - pa));
                 float hu = hh.y, hv = hh.z;
                 float w = 1.0f - hu - hv;
-                mat_id = S.tmat[prim];
-                const DMat& Mh = material_of(S, A.vmats, V, mat_id);
+                mat_id = uload(&S.tmat)[prim];
+                const DMat Mh = material_of(S, A.vmats, V, mat_id);
                 f3 nso = ngo;
-                if (Mh.kind != AG_MAT_GEM && S.nrm) {
-                    f3 na = ld3(S.nrm + 3 * i0), nb = ld3(S.nrm + 3 * i1), nc = ld3(S.nrm + 3 * i2);
+                if (Mh.kind != AG_MAT_GEM && nrm) {
+                    f3 na = ld3(nrm + 3 * i0), nb = ld3(nrm + 3 * i1), nc = ld3(nrm + 3 * i2);
                     f3 nn = na * w + nb * hu + nc * hv;
                     nso = length_sq(nn) > 1e-12f ? normalize(nn) : ngo;
                 }
-                if (S.cavity) cavity = S.cavity[i0] * w + S.cavity[i1] * hu + S.cavity[i2] * hv;
+                if (cav) cavity = cav[i0] * w + cav[i1] * hu + cav[i2] * hv;
                 p_obj = pa * w + pb * hu + pc * hv;
-                if (V.turn_deg != 0.0f) {
-                    p = rot_y(p_obj, V.ts, V.tc);
-                    ng = rot_y(ngo, V.ts, V.tc);
-                    ns = rot_y(nso, V.ts, V.tc);
+                if (turn != 0.0f) {
+                    p = rot_y(p_obj, vts, vtc);
+                    ng = rot_y(ngo, vts, vtc);
+                    ns = rot_y(nso, vts, vtc);
                 } else {
                     p = p_obj;
                     ng = ngo;
@@ -1060,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                 }
             }
             if (alive) {
-                const DMat& M = material_of(S, A.vmats, V, mat_id);
+                const DMat M = material_of(S, A.vmats, V, mat_id);
                 f3 wo_w = -d;
                 bool transmissive = M.kind == AG_MAT_GEM;
                 f3 nsu = ns, ngu = ng;
@@ -1072,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                 Frame fr = make_frame(nsu);
                 f3 wo = to_local(fr, wo_w);
                 const uint32_t dim = 8u + 8u * (uint32_t)depth;
-                Bsdf bs = material_bsdf(M, lam, cavity, rng_dim(ps, si, dim), p_obj, nsu, fr, V);
+                Bsdf bs = material_bsdf(M, lam, cavity, rng_dim(sob, ps, si, dim), p_obj, nsu, fr, V);
                 if (depth == 0) {
                     float g[3];
                     guide_rgb(M, cavity, g);
@@ -1081,15 +1183,15 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                 }
                 // next-event estimation (integrator.rs:130-178)
                 if (!bsdf_is_delta(bs)) {
-                    float ul = rng_dim(ps, si, dim + 1), u1 = rng_dim(ps, si, dim + 2), u2 = rng_dim(ps, si, dim + 3);
-                    if (S.env_kind == AG_ENV_MAP && ul < S.env_pick) {
+                    float ul = rng_dim(sob, ps, si, dim + 1), u1 = rng_dim(sob, ps, si, dim + 2), u2 = rng_dim(sob, ps, si, dim + 3);
+                    if (env_map && ul < env_pick) {
                         float pdf_dir;
                         f3 dir = env_map_sample(S, u1, u2, &pdf_dir);
                         if (pdf_dir > 0.0f && dot(dir, ngu) > 0.0f) {
                             f3 wi = to_local(fr, dir);
                             Spec f = bsdf_eval(bs, s_energy, wo, wi);
                             if (!is_black(f)) {
-                                float p_light = S.env_pick * pdf_dir;
+                                float p_light = env_pick * pdf_dir;
                                 float p_bsdf = bsdf_pdf(bs, wo, wi);
                                 float w = p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf);
                                 f3 org = offset_origin(p, ngu, dir);
@@ -1104,15 +1206,16 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                                 Q.sh_x[ss] = make_float4(fabsf(wi.z) * w, p_light, rough ? cm : -1.0f, __uint_as_float(1u));
                             }
                         }
-                    } else if (S.n_lights > 0) {
-                        float uu = S.env_pick > 0.0f ? (ul - S.env_pick) / (1.0f - S.env_pick) : ul;
+                    } else if (uload(&S.n_lights) > 0) {
+                        const int nl = uload(&S.n_lights);
+                        float uu = env_pick > 0.0f ? (ul - env_pick) / (1.0f - env_pick) : ul;
                         int k = 0;
-                        while (k < S.n_lights && S.light_cdf[k] < uu) k++;
-                        k = min(k, S.n_lights - 1);
-                        float pick = S.light_cdf[k] - (k == 0 ? 0.0f : S.light_cdf[k - 1]);
-                        int li = S.lights[k];
-                        pick = pick * (1.0f - S.env_pick);
-                        const DPanel& Pn = S.panel[li];
+                        while (k < nl && uload(&S.light_cdf[k]) < uu) k++;
+                        k = min(k, nl - 1);
+                        float pick = uload(&S.light_cdf[k]) - (k == 0 ? 0.0f : uload(&S.light_cdf[k - 1]));
+                        int li = uload(&S.lights[k]);
+                        pick = pick * (1.0f - env_pick);
+                        const DPanel Pn = load_panel(&S.panel[li]);
                         f3 lp = ld3(Pn.c) + ld3(Pn.hu) * (2.0f * u1 - 1.0f) + ld3(Pn.hv) * (2.0f * u2 - 1.0f);
                         f3 to = lp - p;
                         float d2 = length_sq(to);
@@ -1139,7 +1242,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                 }
                 // continue the path (integrator.rs:180-218)
                 BSample sm;
-                float u0 = rng_dim(ps, si, dim + 4), u1 = rng_dim(ps, si, dim + 5), u2 = rng_dim(ps, si, dim + 6);
+                float u0 = rng_dim(sob, ps, si, dim + 4), u1 = rng_dim(sob, ps, si, dim + 5), u2 = rng_dim(sob, ps, si, dim + 6);
                 bool ok = bsdf_sample(bs, s_energy, wo, u0, u1, u2, &sm);
                 if (ok && (sm.pdf <= 0.0f || is_black(sm.f))) ok = false;
                 f3 wi_w;
@@ -1217,17 +1320,15 @@ struct ShadowJob {
                 tmax = dist * (1.0f - 1e-4f);
                 int ignore = (int)__float_as_uint(b.w);
                 for (int k = 0; k < np && !blocked; k++) {
-                    const DPanel pk = uload(&Sp->panel[k]);
                     float t;
-                    if (k != ignore && panel_intersect(pk, from, dir, tmax, &t)) blocked = true;
+                    if (k != ignore && panel_intersect(&Sp->panel[k], from, dir, tmax, &t)) blocked = true;
                 }
             } else {
                 dir = f3{b.x, b.y, b.z};
                 tmax = INF_F;
                 for (int k = 0; k < np && !blocked; k++) {
-                    const DPanel pk = uload(&Sp->panel[k]);
                     float t;
-                    if (panel_intersect(pk, from, dir, INF_F, &t)) blocked = true;
+                    if (panel_intersect(&Sp->panel[k], from, dir, INF_F, &t)) blocked = true;
                 }
             }
             if (!blocked && has_floor && fabsf(dir.y) >= 1e-9f) {  // light.rs:133-162, as floor_intersect
@@ -1235,7 +1336,7 @@ struct ShadowJob {
                 if (t > 1e-5f && t < tmax) {
                     if (kind == 0u) {
                         blocked = true;
-                    } else {  // floor_presence, light.rs:148-162
+                    } else {  // floor_presence, light.rs:150-162
                         f3 p = from + dir * t;
                         float r = sqrtf(p.x * p.x + p.z * p.z), f0 = uload(&Sp->floor_f0), f1 = uload(&Sp->floor_f1);
                         float pres = 1.0f;
@@ -1294,10 +1395,11 @@ __global__ __launch_bounds__(256) void k_accumulate(const DSeg* __restrict__ seg
     for (int g = 0; g < nseg; g++) {
         const DSeg sg = segs[g];
         const DView& V = views[sg.view];
-        const size_t plane = (size_t)V.width * V.height * 3;
+        const size_t plane = (size_t)uload(&V.width) * uload(&V.height) * 3;
+        float* const acc0 = uload(&V.acc);  // a global pointer (see uload)
         for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < sg.npix; q += gridDim.x * blockDim.x) {
             uint32_t pix = sg.p0 + q;
-            float* ac = V.acc + (size_t)pix * 3;
+            float* ac = acc0 + (size_t)pix * 3;
             float* aa = ac + plane;
             float* an = ac + 2 * plane;
             float acc[3] = {ac[0], ac[1], ac[2]}, aacc[3] = {aa[0], aa[1], aa[2]}, nacc[3] = {an[0], an[1], an[2]};
@@ -1413,11 +1515,11 @@ void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const fl
                           uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill) {
     hipLaunchKernelGGL(k_rays_occluded, dim3(grid), dim3(TRAV_BLOCK), 0, s, nodes, tris, r, out, n, cursor, refill, spill);
 }
-int traversal_blocks_per_cu() {
+int traversal_blocks_per_cu(bool batch) {
     int lo = 1 << 30;
-    const void* ks[4] = {(const void*)k_extend, (const void*)k_shadow, (const void*)k_rays_intersect,
-                         (const void*)k_rays_occluded};
-    for (const void* k : ks) {
+    const void* render[2] = {(const void*)k_extend, (const void*)k_shadow};
+    const void* rays[2] = {(const void*)k_rays_intersect, (const void*)k_rays_occluded};
+    for (const void* k : batch ? rays : render) {
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, TRAV_BLOCK, 0) != hipSuccess || b < 1) b = 1;
         lo = std::min(lo, b);

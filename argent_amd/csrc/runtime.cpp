@@ -79,7 +79,8 @@ struct ag_ctx {
     DPaths P{};
     DQueues Q{};
     int grid_trav = 0, grid_batch = 0, grid_work = 0;
-    int max_trav_blocks = 1;  // resident traversal blocks per CU (occupancy)
+    int max_trav_blocks = 1;   // resident render traversal blocks per CU (occupancy)
+    int max_batch_blocks = 1;  // the same for the ray-batch kernels
     size_t spill_threads = 0;
     // timing: HIP event pairs around every stage launch on the context stream
     bool timing = false;
@@ -181,14 +182,17 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     // a handful of rays keeping the GPU for a dispatch each -- is paid rarely.
     c->tuning.paths_per_wave = 1u << 26;
     c->tuning.block_traverse = TRAV_BLOCK;
-    c->max_trav_blocks = traversal_blocks_per_cu();
-    // Render passes: whole coherent batches of 64 rays per wavefront at 4
-    // blocks per CU; ray batches (often incoherent): refill at 32 idle lanes,
-    // full occupancy.
-    c->tuning.blocks_per_cu = (uint32_t)std::min(4, c->max_trav_blocks);
+    c->max_trav_blocks = traversal_blocks_per_cu(false);
+    c->max_batch_blocks = traversal_blocks_per_cu(true);
+    // Render passes: extension rays in whole coherent batches of 64 rays per
+    // wavefront at 5 blocks per CU; shadow rays, whose lengths within a batch
+    // differ widely (some stop at a flag, some cross the piece), refill at 32
+    // idle lanes; ray batches (often incoherent): refill at 32, full occupancy.
+    c->tuning.blocks_per_cu = (uint32_t)std::min(5, c->max_trav_blocks);
     c->tuning.refill_min = 64;
-    c->tuning.batch_blocks_per_cu = (uint32_t)c->max_trav_blocks;
+    c->tuning.batch_blocks_per_cu = (uint32_t)c->max_batch_blocks;
     c->tuning.batch_refill_min = 32;
+    c->tuning.shadow_refill_min = 32;
     c->tuning.sort_by_material = 1;
     *out = c;
     return AG_OK;
@@ -221,12 +225,14 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
     if (!c || !t) return fail(AG_E_INVALID, "null argument");
     if (t->paths_per_wave < 1024 || t->paths_per_wave > (1u << 26)) return fail(AG_E_INVALID, "paths_per_wave out of range");
     if (t->block_traverse != TRAV_BLOCK) return fail(AG_E_INVALID, "block_traverse must be 256");
-    if (t->blocks_per_cu < 1 || (int)t->blocks_per_cu > c->max_trav_blocks || t->batch_blocks_per_cu < 1 ||
-        (int)t->batch_blocks_per_cu > c->max_trav_blocks)
-        return fail(AG_E_INVALID, "blocks_per_cu and batch_blocks_per_cu must be 1.." +
-                                      std::to_string(c->max_trav_blocks) + " (resident traversal blocks per CU)");
-    if (t->refill_min < 1 || t->refill_min > 64 || t->batch_refill_min < 1 || t->batch_refill_min > 64)
-        return fail(AG_E_INVALID, "refill_min and batch_refill_min must be 1..64");
+    if (t->blocks_per_cu < 1 || (int)t->blocks_per_cu > c->max_trav_blocks)
+        return fail(AG_E_INVALID, "blocks_per_cu must be 1.." + std::to_string(c->max_trav_blocks) +
+                                      " (resident render traversal blocks per CU)");
+    if (t->batch_blocks_per_cu < 1 || (int)t->batch_blocks_per_cu > c->max_batch_blocks)
+        return fail(AG_E_INVALID, "batch_blocks_per_cu must be 1.." + std::to_string(c->max_batch_blocks) +
+                                      " (resident ray-batch traversal blocks per CU)");
+    for (uint32_t r : {t->refill_min, t->batch_refill_min, t->shadow_refill_min})
+        if (r < 1 || r > 64) return fail(AG_E_INVALID, "refill_min, batch_refill_min and shadow_refill_min must be 1..64");
     c->tuning = *t;
     return AG_OK;
 }
@@ -876,7 +882,7 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
                              c->energy_buf.as<float>(), c->P,
                              c->Q, (int)d);
             });
-            timed(2, (int)d, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, c->spill_buf.as<uint2>()); });
+            timed(2, (int)d, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.shadow_refill_min, c->spill_buf.as<uint2>()); });
             launches++;
         }
         timed(4, -1, [&] { launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P); });

@@ -383,10 +383,12 @@ typedef struct ag_tuning {
     uint32_t block_traverse;   /* threads per block, traversal kernels */
     uint32_t blocks_per_cu;    /* persistent traversal blocks per CU, render passes */
     uint32_t sort_by_material; /* 1 = shade in material-sorted order */
-    uint32_t refill_min;       /* render passes: idle lanes (1..64) before a wavefront
-                                  takes new rays; 64 = whole coherent batches */
+    uint32_t refill_min;       /* render passes, extension rays: idle lanes (1..64)
+                                  before a wavefront takes new rays; 64 = whole
+                                  coherent batches */
     uint32_t batch_blocks_per_cu; /* the same two for ag_intersect / ag_occluded */
     uint32_t batch_refill_min;
+    uint32_t shadow_refill_min;   /* render passes, shadow rays (1..64) */
 } ag_tuning;
 int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
 int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);
