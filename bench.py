@@ -361,9 +361,9 @@ def main():
     dbar = totals["extension_rays"] / totals["samples"]
     pipe_bytes = samples * (RAYGEN_BYTES_PER_SAMPLE + FINALIZE_BYTES_PER_SAMPLE + dbar * SEGMENT_BYTES)
     rays = totals["extension_rays"] + totals["shadow_rays"]
-    traffic = None
+    traffic = None  # measured for the signet still only
     tf = os.path.join(HERE, "profiles", "k_extend_signet_traffic.json")
-    if os.path.exists(tf):
+    if args.workload == "signet" and os.path.exists(tf):
         traffic = json.load(open(tf)).get("bytes_per_dispatch")
     result = {
         "metric": ("Msamples/s at 1080x1920 256spp vs CPU; served p50/p99 latency, 1 GPU" if args.workload == "signet"
