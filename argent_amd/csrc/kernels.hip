@@ -451,9 +451,11 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
                 J.finish(idx, L, parked);
                 if (parked) idx = NO_RAY;
             }
-            // Fill the idle lanes with the next rays in queue order; a ray its
-            // job settles without the piece finishes at once and its lane takes
-            // the next one, so traversal batches hold only rays that need it.
+            // Fill the idle lanes with the next rays in queue order, taking
+            // again while lanes stay idle (a take returns at most what is left
+            // of the wavefront's current grab); a ray its job settles without
+            // the piece finishes at once and its lane takes the next one, so
+            // traversal batches hold only rays that need it.
             for (;;) {
                 uint64_t idle = __ballot(idx == NO_RAY);
                 uint32_t cnt = (uint32_t)__popcll(idle);
@@ -475,9 +477,12 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
                         }
                     }
                 }
-                if (__ballot(fin) == 0) break;
-                J.finish(idx, L, fin);
-                if (fin) idx = NO_RAY;
+                if (__ballot(fin) != 0) {
+                    J.finish(idx, L, fin);
+                    if (fin) idx = NO_RAY;
+                } else if (k == cnt) {
+                    break;  // every idle lane has a ray to trace
+                }
             }
         }
         if (__ballot(idx != NO_RAY && L.cur != TRAV_DONE) == 0) {
