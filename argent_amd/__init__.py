@@ -119,6 +119,10 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
+    def last_query_ms(self):
+        """Kernel time of the last ray query or cavity AO call (HIP events), ms."""
+        return A.lib().ag_last_query_ms(self.h)
+
     def synchronize(self):
         check(A.lib().ag_ctx_synchronize(self.h))
 
@@ -316,6 +320,16 @@ class Scene:
         out = np.empty(n, np.uint8)
         check(A.lib().ag_occluded(self.ctx.h, self.h, C.byref(rays), n, _ptr(out, C.c_uint8)))
         return out.astype(bool)
+
+    def ambient_occlusion(self, samples=48, distance=1.2):
+        """Cavity AO per vertex on the GPU (ag_ambient_occlusion; argent-mesh
+        cavity.rs:95-128 with the piece's BVH as occluder): float32 [n_vertices]."""
+        root = self
+        while root.base is not None:
+            root = root.base
+        out = np.zeros(len(root.desc.positions), np.float32)
+        check(A.lib().ag_ambient_occlusion(self.ctx.h, self.h, samples, distance, _ptr(out)))
+        return out
 
     def bsdf_probe(self, mat, lam, mask, u, wo, wi):
         mat = np.ascontiguousarray(np.asarray(mat, np.int32))

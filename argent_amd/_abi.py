@@ -168,6 +168,7 @@ _SIGS = [
     ("ag_event_destroy", None, [C.c_void_p]),
     ("ag_intersect", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, C.POINTER(HitsSoA)]),
     ("ag_occluded", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, u8p]),
+    ("ag_ambient_occlusion", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, f32p]),
     ("ag_intersect_device", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32,
                                       C.POINTER(HitsSoA)]),
     ("ag_occluded_device", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, u8p]),

@@ -569,6 +569,58 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_rays_occluded(const f
     traverse_persistent<true>(J, n, cursor, refill, st);
 }
 
+// ------------------------------------------------------------------ cavity ambient occlusion
+// cavity.rs:95-128 (ambient_occlusion) with the piece's own any-hit query
+// (scene.rs:40-44 -> bvh.rs:216-245): ray i is sample k = i % samples of
+// vertex v = i / samples, generated here rather than read from memory: a
+// cosine-weighted direction around the vertex normal, stratified in u1 with a
+// per-vertex jitter and a golden-ratio sequence in u2 rotated per vertex,
+// leaving the vertex at p + n eps, reaching `dist`.  A blocked ray adds one to
+// its vertex's count.
+struct AoJob {
+    const float4* nodes;
+    const float4* tris;
+    const float* pos;
+    const float* nrm;
+    uint32_t samples;
+    float dist, eps;
+    uint32_t* counts;
+    AGD bool begin(uint32_t i, TravLane& L) {
+        const uint32_t v = i / samples, k = i - v * samples;
+        L.aux = v;
+        f3 n{nrm[3 * (size_t)v], nrm[3 * (size_t)v + 1], nrm[3 * (size_t)v + 2]};
+        // Vec3::normalize_or_zero: times the reciprocal length, zero unless finite and positive
+        const float rcp = 1.0f / sqrtf((n.x * n.x + n.y * n.y) + n.z * n.z);
+        if (!(isfinite(rcp) && rcp > 0.0f)) {
+            L.prim = NO_RAY;  // no normal: never occluded
+            return false;
+        }
+        n = n * rcp;
+        const Frame fr = make_frame(n);  // Vec3::any_orthonormal_pair (Duff et al. 2017)
+        const f3 o = f3{pos[3 * (size_t)v], pos[3 * (size_t)v + 1], pos[3 * (size_t)v + 2]} + n * eps;
+        const uint32_t h = hash32(v ^ 0x5bd1e995u);
+        const float rot = (float)(h >> 8) * (1.0f / 16777216.0f);
+        const float jit = (float)(hash32(h) >> 8) * (1.0f / 16777216.0f);
+        const float u1 = fminf(((float)k + jit) / (float)samples, 0.999999f);
+        const float x2 = (float)k * 0.618034f + rot;
+        const float u2 = x2 - truncf(x2);
+        const float r = sqrtf(u1), phi = 6.28318530717958647692f * u2;
+        const f3 d = (fr.t * (r * cosf(phi)) + fr.b * (r * sinf(phi))) + n * sqrtf(1.0f - u1);
+        lane_begin(L, nodes, tris, o, d, dist);
+        return true;
+    }
+    AGD void finish(uint32_t i, const TravLane& L, bool mine) {
+        if (mine && L.prim != NO_RAY) atomicAdd(&counts[L.aux], 1u);
+    }
+    AGD void drain() {}
+};
+
+__global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_ao(AoJob J, uint32_t n, uint32_t* cursor, uint32_t refill,
+                                                            uint2* spill) {
+    TRAV_STACK_DECL
+    traverse_persistent<true>(J, n, cursor, refill, st);
+}
+
 // ------------------------------------------------------------------ scene pieces
 // fabsf(x / y) <= 1 with x / y rounded to nearest, as the reference writes it
 // (light.rs:60-62), without dividing unless |x| lies within 2^-20 of y.  For
@@ -1523,13 +1575,21 @@ void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const fl
 int traversal_blocks_per_cu(bool batch) {
     int lo = 1 << 30;
     const void* render[2] = {(const void*)k_extend, (const void*)k_shadow};
-    const void* rays[2] = {(const void*)k_rays_intersect, (const void*)k_rays_occluded};
-    for (const void* k : batch ? rays : render) {
+    const void* rays[3] = {(const void*)k_rays_intersect, (const void*)k_rays_occluded, (const void*)k_ao};
+    const void* const* ks = batch ? rays : render;
+    for (int i = 0; i < (batch ? 3 : 2); i++) {
+        const void* k = ks[i];
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, TRAV_BLOCK, 0) != hipSuccess || b < 1) b = 1;
         lo = std::min(lo, b);
     }
     return lo;
+}
+void launch_ao(hipStream_t s, int grid, const float4* nodes, const float4* tris, const float* pos, const float* nrm,
+               uint32_t samples, float dist, float eps, uint32_t* counts, uint32_t n, uint32_t* cursor,
+               uint32_t refill, uint2* spill) {
+    AoJob J{nodes, tris, pos, nrm, samples, dist, eps, counts};
+    hipLaunchKernelGGL(k_ao, dim3(grid), dim3(TRAV_BLOCK), 0, s, J, n, cursor, refill, spill);
 }
 void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,
                    DQueues Q) {

@@ -32,6 +32,10 @@ void launch_rays_intersect(hipStream_t s, int grid, const float4* nodes, const f
                            uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill);
 void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, uint8_t* out,
                           uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill);
+// cavity ambient occlusion: n = n_vertices * samples rays, hit counts added to counts[vertex]
+void launch_ao(hipStream_t s, int grid, const float4* nodes, const float4* tris, const float* pos, const float* nrm,
+               uint32_t samples, float dist, float eps, uint32_t* counts, uint32_t n, uint32_t* cursor,
+               uint32_t refill, uint2* spill);
 // Resident blocks per CU of the traversal kernels (occupancy limit).
 int traversal_blocks_per_cu(bool batch);  // resident blocks per CU: ray-batch or render kernels
 void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -115,7 +116,8 @@ struct Geometry {
     uint32_t n_nodes = 0, n_leaves = 0, depth = 0, n_vertices = 0, n_triangles = 0;
     uint16_t max_material = 0;
     bool has_nrm = false, has_cavity = false;
-    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};        // the BVH's bounds
+    float vlo[3] = {0, 0, 0}, vhi[3] = {0, 0, 0};      // all vertex positions (Mesh::bounds)
     size_t bytes() const { return nodes.bytes + tris.bytes + pos.bytes + idx.bytes + tmat.bytes + nrm.bytes + cavity.bytes; }
 };
 
@@ -417,6 +419,13 @@ static int build_geometry(ag_ctx* ctx, const ag_scene_desc* d, std::shared_ptr<G
     for (int a = 0; a < 3; a++) {
         g->lo[a] = bvh.lo[a];
         g->hi[a] = bvh.hi[a];
+        // argent-mesh lib.rs:122-134: zeros for no vertices, else min/max over all of them
+        g->vlo[a] = d->n_vertices ? FLT_MAX : 0.0f;
+        g->vhi[a] = d->n_vertices ? -FLT_MAX : 0.0f;
+        for (uint32_t v = 0; v < d->n_vertices; v++) {
+            g->vlo[a] = std::fmin(g->vlo[a], d->positions[3 * (size_t)v + a]);
+            g->vhi[a] = std::fmax(g->vhi[a], d->positions[3 * (size_t)v + a]);
+        }
     }
     int rc;
     if ((rc = upload(g->nodes, bvh.nodes.data(), bvh.nodes.size() * 4, st))) return rc;
@@ -1305,6 +1314,46 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
     HIPCHK(hipStreamSynchronize(st));
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->last_query_ms = ms;
+    return AG_OK;
+}
+
+extern "C" int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t samples, float distance, float* out) {
+    if (!ctx || !s || !out) return fail(AG_E_INVALID, "null argument");
+    const Geometry& g = *s->geo;
+    const uint32_t nv = g.n_vertices;
+    if (nv == 0) return AG_OK;
+    if (g.n_triangles == 0) {  // cavity.rs:54-56 returns zeros before tracing
+        std::fill(out, out + nv, 0.0f);
+        return AG_OK;
+    }
+    if (!g.has_nrm) return fail(AG_E_INVALID, "ambient occlusion needs the scene's vertex normals");
+    const uint32_t n = std::max(samples, 1u);
+    if ((uint64_t)nv * n > 0xFFFFFFFFull) return fail(AG_E_INVALID, "n_vertices * samples exceeds 2^32 - 1 rays");
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_spill(ctx);
+    if (rc) return rc;
+    if ((rc = ctx->cursor_buf.ensure(CURSOR_SET * sizeof(uint32_t)))) return rc;
+    // cavity.rs:87-90 and :97: eps from the diagonal of the vertex bounds, in f32
+    float dx = g.vhi[0] - g.vlo[0], dy = g.vhi[1] - g.vlo[1], dz = g.vhi[2] - g.vlo[2];
+    const float diag = std::sqrt((dx * dx + dy * dy) + dz * dz);
+    const float eps = std::fmax(2e-4f * diag, 1e-3f);
+    DevBuf counts;
+    if ((rc = counts.ensure((size_t)nv * sizeof(uint32_t)))) return rc;
+    hipStream_t st = ctx->stream;
+    launch_zero_u32(st, counts.as<uint32_t>(), nv);
+    launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    launch_ao(st, ctx->grid_batch, g.nodes.as<float4>(), g.tris.as<float4>(), g.pos.as<float>(), g.nrm.as<float>(), n,
+              distance, eps, counts.as<uint32_t>(), nv * n, ctx->cursor_buf.as<uint32_t>(), ctx->tuning.batch_refill_min,
+              ctx->spill_buf.as<uint2>());
+    HIPCHK(hipEventRecord(ctx->ev[3], st));
+    HIPCHK(hipGetLastError());
+    std::vector<uint32_t> h(nv);
+    HIPCHK(hipMemcpyAsync(h.data(), counts.p, (size_t)nv * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->last_query_ms = ms;
+    for (uint32_t v = 0; v < nv; v++) out[v] = (float)h[v] / (float)n;  // cavity.rs:126
     return AG_OK;
 }
 

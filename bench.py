@@ -268,6 +268,55 @@ def ray_batch(args):
     print(json.dumps(res), flush=True)
 
 
+def cavity_ao(args):
+    """Scene prep: cavity ambient occlusion of the signet (600,804 vertices x
+    48 rays, argent-mesh cavity.rs:95-128) with the reach the reference sizes
+    by the piece (shot.rs:132-137: 0.09 x the half-diagonal, clamped to
+    [0.4, 2.5] mm), on the GPU and by the oracle on the host's cores."""
+    import argent_amd as ag
+    from argent_amd import synth
+    pos, tris, nrm = synth.signet(seed=2)
+    desc = ag.SceneDesc(pos, tris, normals=nrm)
+    desc.add_material(ag.Material.metal("sterling"))
+    radius = max(float(np.linalg.norm((pos.max(0) - pos.min(0)) * 0.5)), 1e-3)
+    dist = float(np.clip(radius * 0.09, 0.4, 2.5))
+    samples = 48
+    ctx = ag.Context(0)
+    sc = ag.Scene(ctx, desc)
+    for _ in range(args.warmup):
+        sc.ambient_occlusion(samples, dist)
+    ms = []
+    for _ in range(max(args.steps, 1)):
+        sc.ambient_occlusion(samples, dist)
+        ms.append(ctx.last_query_ms())
+    t_ms = sum(ms) / len(ms)
+    rays = len(pos) * samples
+    res = {"metric": "Mrays/s, cavity ambient occlusion (48 rays per vertex) of the 1.2 M-triangle signet, 1 GPU",
+           "value": round(rays / (t_ms * 1e-3) / 1e6, 1), "unit": "Mrays/s", "n_gpus": 1, "steps": len(ms),
+           "warmup": args.warmup, "ms_per_step": round(t_ms, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"signet AO, {len(pos)} vertices x {samples} rays, reach {dist:.3f} mm",
+                      "vertices": int(len(pos)), "rays": int(rays), "distance_mm": round(dist, 4)}}
+    sc.close()
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        osc = O.OracleScene(desc)
+        passes, t0 = 0, time.perf_counter()
+        while True:  # the whole mesh, again until about --cpu-seconds of work
+            osc.ambient_occlusion(len(pos), samples, dist, threads=threads)
+            passes += 1
+            dt = time.perf_counter() - t0
+            if dt >= args.cpu_seconds or passes >= 64:
+                break
+        res["cpu_baseline"] = {"value": round(passes * rays / dt / 1e6, 2), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port", "sample": f"the whole mesh, {passes} passes, {dt:.1f} s",
+                               "cpu_model": _cpu_model()}
+        res["gpu_over_cpu"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +330,7 @@ def main():
                          "noir, the reference's CPU-runnable case; gems: the signet with 12 brilliants, an HDRI "
                          "and the white rig's softboxes, 512 spp, depth 24")
     ap.add_argument("--rays", action="store_true", help="the 2^24-ray closest/any-hit batch instead of a still")
+    ap.add_argument("--ao", action="store_true", help="cavity ambient occlusion of the signet (scene prep) instead")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -300,6 +350,9 @@ def main():
         return
     if args.rays:
         ray_batch(args)
+        return
+    if args.ao:
+        cavity_ao(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "

@@ -298,6 +298,17 @@ int ag_intersect_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_
 int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_dev, uint32_t n,
                        uint8_t* out_dev);
 
+/* Cavity ambient occlusion of the scene's piece: ambient_occlusion
+ * (argent-mesh cavity.rs:95-128) with the piece as its Occluder (argent-core
+ * scene.rs:40-44, i.e. Bvh::occluded).  For each vertex, the fraction of
+ * `samples` (at least 1) cosine-weighted rays about its normal, leaving
+ * p + n * max(2e-4 * |diagonal of the vertex bounds|, 1e-3), that are blocked
+ * within `distance` (object space).  Needs the scene's vertex normals; a zero
+ * normal gives 0.  out: n_vertices floats (host).  This is the ray-traced step
+ * of the reference's patina mask (cavity.rs:52-79); its concavity and
+ * smoothing steps are mesh processing on the host. */
+int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t samples, float distance, float* out);
+
 /* Material::bsdf + Bsdf::{eval,pdf,sample} (material.rs:96-138, bsdf.rs:53-239)
  * evaluated on the device for n given inputs, for checking the shading code:
  * material index, 4 wavelengths, cavity mask, u = [layer, u0, u1, u2], wo and

@@ -42,6 +42,7 @@ def lib():
             ("or_scene_stats", C.c_int, [C.c_void_p, _u32, _u32]),
             ("or_intersect", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _f, _f, _f, _u32]),
             ("or_occluded", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _u8]),
+            ("or_ambient_occlusion", C.c_int, [C.c_void_p, C.c_uint32, C.c_float, C.c_int32, C.c_int32, _f]),
             ("or_brute_intersect", C.c_int, [_f, _u32, C.c_uint32, *rays, C.c_uint32, C.c_int32, _f, _f, _f, _u32]),
             ("or_render_pass", C.c_int, [C.c_void_p, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, _f, C.c_uint32,
                                          C.POINTER(A.PathSettingsC), C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32,
@@ -117,6 +118,14 @@ class OracleScene:
         out = np.empty(n, np.uint8)
         lib().or_occluded(self.h, *[_p(c) for c in cols], _p(tm), n, mode, threads, _p(out, C.c_uint8))
         return out.astype(bool)
+
+    def ambient_occlusion(self, n_vertices, samples=48, distance=1.2, mode=MODE_ROBUST, threads=0):
+        """Per-vertex cavity AO (argent-mesh cavity.rs:95-128), float32 [n_vertices]."""
+        out = np.zeros(n_vertices, np.float32)
+        rc = lib().or_ambient_occlusion(self.h, samples, distance, mode, threads, _p(out))
+        if rc:
+            raise ValueError("or_ambient_occlusion: the scene has no vertex normals" if rc == 2 else f"rc {rc}")
+        return out
 
     def render_pass(self, cam, width, height, sums, first, ps, count, seed, threads=0, rows=None):
         """sums: float32 (3, h, w, 3) array updated in place.  Returns (segments, shadows, samples)."""

@@ -1927,6 +1927,67 @@ extern "C" int or_occluded(const or_scene* s, const float* ox, const float* oy, 
     return 0;
 }
 
+// argent-mesh cavity.rs:95-128 (ambient_occlusion), the occluder being the
+// piece's BVH as the renderer supplies it (argent-core scene.rs:40-44:
+// Bvh::occluded, bvh.rs:216-245).  Vertex normals are the scene's own (the
+// reference's vertex_normals, cavity.rs:80-86, returns mesh.normals when
+// present); bounds as Mesh::bounds (argent-mesh lib.rs:122-134).
+extern "C" int or_ambient_occlusion(const or_scene* s, uint32_t samples, float distance, int32_t mode,
+                                    int32_t threads, float* out) {
+    if (!s) return 1;
+    const Scene& S = s->sc;
+    const uint32_t nv = (uint32_t)(S.positions.size() / 3);
+    if (nv == 0) return 0;
+    if (S.triangles.empty()) {  // cavity.rs:54-56
+        std::fill(out, out + nv, 0.0f);
+        return 0;
+    }
+    if (S.normals.size() != S.positions.size()) return 2;
+    float lo[3] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
+                   std::numeric_limits<float>::max()};
+    float hi[3] = {std::numeric_limits<float>::lowest(), std::numeric_limits<float>::lowest(),
+                   std::numeric_limits<float>::lowest()};
+    for (uint32_t v = 0; v < nv; v++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::fmin(lo[a], S.positions[3 * (size_t)v + a]);
+            hi[a] = std::fmax(hi[a], S.positions[3 * (size_t)v + a]);
+        }
+    // cavity.rs:87-90 (diag: powi(2) is x * x) and :97
+    const float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
+    const float diag = std::sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+    const float eps = std::fmax(2e-4f * diag, 1e-3f);
+    const uint32_t n = std::max(samples, 1u);
+    const bool robust = mode == OR_MODE_ROBUST;
+    parallel_for(nv, threads, [&](uint32_t a, uint32_t b) {
+        for (uint32_t v = a; v < b; v++) {
+            V3 nrm = ld(&S.normals[3 * (size_t)v]);
+            // Vec3::normalize_or_zero: times the reciprocal length when finite and positive
+            const float rcp = 1.0f / nrm.length();
+            if (!(std::isfinite(rcp) && rcp > 0.0f)) {
+                out[v] = 0.0f;
+                continue;
+            }
+            const V3 nn = nrm * rcp;
+            const Frame F(nn);  // Vec3::any_orthonormal_pair
+            const V3 o = ld(&S.positions[3 * (size_t)v]) + nn * eps;
+            const uint32_t h = hash32(v ^ 0x5bd1e995u);
+            const float rot = (float)(h >> 8) * (1.0f / 16777216.0f);   // argent-mesh util.rs:255-257
+            const float jit = (float)(hash32(h) >> 8) * (1.0f / 16777216.0f);
+            uint32_t hits = 0;
+            for (uint32_t k = 0; k < n; k++) {
+                const float u1 = std::fmin(((float)k + jit) / (float)n, 0.999999f);
+                const float x2 = (float)k * 0.618034f + rot;
+                const float u2 = x2 - std::trunc(x2);  // f32::fract
+                const float r = std::sqrt(u1), phi = 6.28318530717958647692f * u2;
+                const V3 d = (F.t * (r * std::cos(phi)) + F.b * (r * std::sin(phi))) + nn * std::sqrt(1.0f - u1);
+                if (S.bvh.occluded(Ray{o, d}, distance, robust)) hits++;
+            }
+            out[v] = (float)hits / (float)n;
+        }
+    });
+    return 0;
+}
+
 extern "C" int or_brute_intersect(const float* pos, const uint32_t* tri, uint32_t nt, const float* ox, const float* oy,
                                   const float* oz, const float* dx, const float* dy, const float* dz, const float* tmax,
                                   uint32_t n, int32_t threads, float* t, float* u, float* v, uint32_t* prim) {
