@@ -22,7 +22,8 @@ from . import _abi as A
 from ._abi import ArgentError, check
 
 __all__ = ["Context", "SceneDesc", "Scene", "Film", "PathSettings", "Material", "render_pass", "render_pass_async", "Pass", "render_views",
-           "look_at", "frame", "VIEWS", "ArgentError", "alloy_f0_srgb", "alloy_ior"]
+           "look_at", "frame", "VIEWS", "ArgentError", "alloy_f0_srgb", "alloy_ior", "grade", "develop_image",
+           "denoise_image"]
 
 VIEWS = {  # views.rs:17-31: (azimuth, elevation, fill)
     "hero": (34.0, 26.0, 0.72), "front": (0.0, 6.0, 0.70), "face": (0.0, 6.0, 0.70),
@@ -382,6 +383,46 @@ class Film:
 
     def sums(self):
         return np.stack([self._read(k, sums=True) for k in (A.FILM_COLOR, A.FILM_ALBEDO, A.FILM_NORMAL)])
+
+    def denoise(self):
+        """À-trous of the film's colour with its albedo and normal guides, on the GPU (ag_film_denoise)."""
+        out = np.empty((self.height, self.width, 3), np.float32)
+        check(A.lib().ag_film_denoise(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def develop(self, tonemap="agx_punchy", exposure_ev=0.0, saturation=1.0, denoise=False):
+        """The live preview's development on the GPU (ag_film_develop): (h, w, 3) uint8."""
+        out = np.empty((self.height, self.width, 3), np.uint8)
+        g = grade(tonemap, exposure_ev, saturation)
+        check(A.lib().ag_film_develop(self.ctx.h, self.h, C.byref(g), 1 if denoise else 0, _ptr(out, C.c_uint8)))
+        return out
+
+
+def grade(tonemap="agx_punchy", exposure_ev=0.0, saturation=1.0):
+    """ag_grade; tonemap: agx, agx_punchy (the reference's default), agx_golden, aces, khronos, reinhard, clamp."""
+    return A.Grade(exposure_ev, A.TONEMAPS[tonemap] if isinstance(tonemap, str) else int(tonemap), saturation)
+
+
+def develop_image(ctx, img, tonemap="agx_punchy", exposure_ev=0.0, saturation=1.0):
+    """develop (tonemap.rs:370-388) on the GPU: (h, w, 3) float32 -> (h, w, 3) uint8."""
+    img = np.ascontiguousarray(np.asarray(img, np.float32))
+    h, w = img.shape[:2]
+    out = np.empty((h, w, 3), np.uint8)
+    g = grade(tonemap, exposure_ev, saturation)
+    check(A.lib().ag_develop_image(ctx.h, _ptr(img), w, h, C.byref(g), _ptr(out, C.c_uint8)))
+    return out
+
+
+def denoise_image(ctx, color, albedo=None, normal=None):
+    """The à-trous denoiser (denoise.rs:660-709) on the GPU: (h, w, 3) float32 images."""
+    color = np.ascontiguousarray(np.asarray(color, np.float32))
+    h, w = color.shape[:2]
+    a = None if albedo is None else np.ascontiguousarray(np.asarray(albedo, np.float32))
+    n = None if normal is None else np.ascontiguousarray(np.asarray(normal, np.float32))
+    out = np.empty_like(color)
+    check(A.lib().ag_denoise_image(ctx.h, _ptr(color), _ptr(a) if a is not None else None,
+                                   _ptr(n) if n is not None else None, w, h, _ptr(out)))
+    return out
 
 
 def look_at(pos, target, up_hint=(0.0, 1.0, 0.0), focal_mm=100.0, f_stop=0.0, aspect=1.0):

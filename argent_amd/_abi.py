@@ -69,6 +69,14 @@ class PathSettingsC(C.Structure):
     _fields_ = [("max_depth", C.c_uint32), ("clamp_indirect", C.c_float), ("rr_depth", C.c_uint32)]
 
 
+class Grade(C.Structure):
+    """ag_grade (tonemap.rs:53-58)."""
+    _fields_ = [("exposure_ev", C.c_float), ("tonemap", C.c_int32), ("saturation", C.c_float)]
+
+
+TONEMAPS = {"agx": 0, "agx_punchy": 1, "agx_golden": 2, "aces": 3, "khronos": 4, "reinhard": 5, "clamp": 6}
+
+
 class RaysSoA(C.Structure):
     _fields_ = [("ox", f32p), ("oy", f32p), ("oz", f32p), ("dx", f32p), ("dy", f32p), ("dz", f32p),
                 ("tmax", f32p)]
@@ -157,6 +165,10 @@ _SIGS = [
     ("ag_film_destroy", None, [C.c_void_p]),
     ("ag_film_clear", C.c_int, [C.c_void_p]),
     ("ag_film_read", C.c_int, [C.c_void_p, C.c_int32, f32p]),
+    ("ag_develop_image", C.c_int, [C.c_void_p, f32p, C.c_uint32, C.c_uint32, C.POINTER(Grade), u8p]),
+    ("ag_denoise_image", C.c_int, [C.c_void_p, f32p, f32p, f32p, C.c_uint32, C.c_uint32, f32p]),
+    ("ag_film_denoise", C.c_int, [C.c_void_p, C.c_void_p, f32p]),
+    ("ag_film_develop", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Grade), C.c_int32, u8p]),
     ("ag_film_read_sums", C.c_int, [C.c_void_p, C.c_int32, f32p]),
     ("ag_film_samples", C.c_uint32, [C.c_void_p]),
     ("ag_render_pass", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_void_p,

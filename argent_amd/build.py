@@ -14,8 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libargent_hip.so")
 
-HIP_SOURCES = ["kernels.hip"]
-CPP_SOURCES = ["runtime.cpp", "host_build.cpp", "host_spectra.cpp", "host_bvh.cpp"]
+HIP_SOURCES = ["kernels.hip", "post.hip"]
+CPP_SOURCES = ["runtime.cpp", "host_build.cpp", "host_spectra.cpp", "host_bvh.cpp", "host_post.cpp"]
 
 
 def hipcc():
@@ -50,7 +50,7 @@ def build(verbose=False, jobs=8):
         if src.endswith(".hip"):
             cmd = [hc, "-x", "hip", *common, *dev, "-c", src, "-o", obj]
         else:
-            cmd = [hc, "-x", "hip", *common, *dev, "-c", src, "-o", obj] if src == "runtime.cpp" else \
+            cmd = [hc, "-x", "hip", *common, *dev, "-c", src, "-o", obj] if src in ("runtime.cpp", "host_post.cpp") else \
                   [hc, *common, "-c", src, "-o", obj]
         procs.append((src, subprocess.Popen(cmd, cwd=CSRC, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
                                             text=True)))

@@ -9,6 +9,13 @@
 namespace ag {
 
 int fail(int code, const std::string& msg);
+
+struct DevGrade;  // post.hpp
+// The tables of a tone-mapping grade (tonemap.rs:284-313), host_post.cpp:
+// blob = curves (n_curves x curve_len floats) then the encode table
+// (encode_len floats); `g` gets everything but the device pointers.
+int build_grade(float exposure_ev, int32_t op, float saturation, std::vector<float>* blob, DevGrade* g, int* n_curves,
+                size_t* curve_len, size_t* encode_len);
 void light_cdf(const ag_scene_desc* d, int* n_lights, int* lights, float* cdf, float* env_pick);
 
 // Compressed 8-wide BVH in the device layout.  Node: 5 x 16 B = 80 B

@@ -17,6 +17,7 @@
 #include "host_internal.hpp"
 #include "host_spectra.hpp"
 #include "kernels.hpp"
+#include "post.hpp"
 #include "tables/spectral_data.h"
 
 using namespace ag;
@@ -97,6 +98,7 @@ struct ag_ctx {
     std::vector<unsigned long long> depth_rays;  // [2 * TALLY_DEPTHS]: extension, then shadow
     std::vector<double> depth_ms;                // [3 * TALLY_DEPTHS]: extend, shade, shadow
     DevBuf tally_buf;
+    DevBuf post_buf;  // preview development: images, à-trous work, grade tables, 8-bit output
     DevBuf energy_buf;  // the library's GGX energy table, staged into LDS by the shading kernel
     double last_query_ms = 0.0;
     hipEvent_t next_event() {
@@ -1355,6 +1357,140 @@ extern "C" int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t sam
     if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->last_query_ms = ms;
     for (uint32_t v = 0; v < nv; v++) out[v] = (float)h[v] / (float)n;  // cavity.rs:126
     return AG_OK;
+}
+
+// ------------------------------------------------------------------ preview development
+namespace {
+constexpr size_t POST_TABLE_FLOATS = 1 << 16;  // grade tables: at most 3 AgX curves of ~9.3 K + 4097
+struct PostWork {
+    float4* img[4];    // colour, albedo, normal, result
+    AtrousWork W;
+    float* stage[3];   // (r, g, b) staging for host images
+    uint8_t* u8;
+    float* tables;
+};
+int post_work(ag_ctx* c, size_t n, PostWork* P) {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t f4 = up(n * 16), f1 = up(n * 4), f3 = up(n * 12), b3 = up(n * 3);
+    int rc = c->post_buf.ensure(9 * f4 + 6 * f1 + 3 * f3 + b3 + POST_TABLE_FLOATS * 4);
+    if (rc) return rc;
+    char* p = c->post_buf.as<char>();
+    auto take = [&](size_t b) { char* q = p; p += b; return q; };
+    for (auto& im : P->img) im = reinterpret_cast<float4*>(take(f4));
+    P->W.demod = reinterpret_cast<float4*>(take(f4));
+    P->W.ga = reinterpret_cast<float4*>(take(f4));
+    P->W.gn = reinterpret_cast<float4*>(take(f4));
+    P->W.sig[0] = reinterpret_cast<float4*>(take(f4));
+    P->W.sig[1] = reinterpret_cast<float4*>(take(f4));
+    P->W.l[0] = reinterpret_cast<float*>(take(f1));
+    P->W.l[1] = reinterpret_cast<float*>(take(f1));
+    P->W.var[0] = reinterpret_cast<float*>(take(f1));
+    P->W.var[1] = reinterpret_cast<float*>(take(f1));
+    P->W.t0 = reinterpret_cast<float*>(take(f1));
+    P->W.t1 = reinterpret_cast<float*>(take(f1));
+    for (auto& st : P->stage) st = reinterpret_cast<float*>(take(f3));
+    P->u8 = reinterpret_cast<uint8_t*>(take(b3));
+    P->tables = reinterpret_cast<float*>(take(POST_TABLE_FLOATS * 4));
+    return AG_OK;
+}
+// Tables of `g` onto the device; DevGrade ready for launch_develop.
+int post_grade(ag_ctx* c, const ag_grade* grade, const PostWork& P, DevGrade* g) {
+    std::vector<float> blob;
+    int nc = 0;
+    size_t clen = 0, elen = 0;
+    int rc = build_grade(grade->exposure_ev, grade->tonemap, grade->saturation, &blob, g, &nc, &clen, &elen);
+    if (rc) return rc;
+    if (blob.size() > POST_TABLE_FLOATS) return fail(AG_E_INVALID, "grade tables larger than the workspace");
+    HIPCHK(hipMemcpyAsync(P.tables, blob.data(), blob.size() * 4, hipMemcpyHostToDevice, c->stream));
+    for (int k = 0; k < 3; k++) g->curve[k] = nc ? P.tables + (size_t)std::min(k, nc - 1) * clen : P.tables;
+    g->encode = P.tables + (size_t)nc * clen;
+    return AG_OK;
+}
+int post_upload(ag_ctx* c, const float* rgb, size_t n, float* stage, float4* img) {
+    HIPCHK(hipMemcpyAsync(stage, rgb, n * 12, hipMemcpyHostToDevice, c->stream));
+    launch_pack4(c->stream, stage, n, img);
+    return AG_OK;
+}
+int post_film(ag_ctx* c, const ag_film* f, const PostWork& P) {
+    launch_film_images(c->stream, f->sums.as<float>(), (size_t)f->w * f->h, 1.0f / (float)std::max(f->samples, 1u),
+                       P.img[0], P.img[1], P.img[2]);
+    return AG_OK;
+}
+int post_finish_u8(ag_ctx* c, const PostWork& P, size_t n, uint8_t* out) {
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, P.u8, n * 3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AG_OK;
+}
+int post_finish_f(ag_ctx* c, const PostWork& P, size_t n, float* out) {
+    launch_unpack4(c->stream, P.img[3], n, P.stage[0]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, P.stage[0], n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AG_OK;
+}
+}  // namespace
+
+extern "C" int ag_develop_image(ag_ctx* ctx, const float* rgb, uint32_t w, uint32_t h, const ag_grade* grade,
+                                uint8_t* out) {
+    if (!ctx || !grade || ((!rgb || !out) && (size_t)w * h)) return fail(AG_E_INVALID, "null argument");
+    const size_t n = (size_t)w * h;
+    if (n == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    PostWork P;
+    DevGrade g;
+    int rc;
+    if ((rc = post_work(ctx, n, &P)) || (rc = post_grade(ctx, grade, P, &g)) ||
+        (rc = post_upload(ctx, rgb, n, P.stage[0], P.img[0])))
+        return rc;
+    launch_develop(ctx->stream, P.img[0], (int)w, (int)h, g, P.u8);
+    return post_finish_u8(ctx, P, n, out);
+}
+
+extern "C" int ag_denoise_image(ag_ctx* ctx, const float* color, const float* albedo, const float* normal, uint32_t w,
+                                uint32_t h, float* out) {
+    if (!ctx || ((!color || !out) && (size_t)w * h)) return fail(AG_E_INVALID, "null argument");
+    const size_t n = (size_t)w * h;
+    if (n == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    PostWork P;
+    int rc;
+    if ((rc = post_work(ctx, n, &P)) || (rc = post_upload(ctx, color, n, P.stage[0], P.img[0])) ||
+        (albedo && (rc = post_upload(ctx, albedo, n, P.stage[1], P.img[1]))) ||
+        (normal && (rc = post_upload(ctx, normal, n, P.stage[2], P.img[2]))))
+        return rc;
+    launch_atrous(ctx->stream, (int)w, (int)h, P.img[0], albedo ? P.img[1] : nullptr, normal ? P.img[2] : nullptr, P.W,
+                  P.img[3]);
+    return post_finish_f(ctx, P, n, out);
+}
+
+extern "C" int ag_film_denoise(ag_ctx* ctx, const ag_film* film, float* out) {
+    if (!ctx || !film || !out) return fail(AG_E_INVALID, "null argument");
+    if (film->ctx != ctx) return fail(AG_E_INVALID, "film belongs to another context");
+    const size_t n = (size_t)film->w * film->h;
+    if (n == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    PostWork P;
+    int rc;
+    if ((rc = post_work(ctx, n, &P)) || (rc = post_film(ctx, film, P))) return rc;
+    launch_atrous(ctx->stream, (int)film->w, (int)film->h, P.img[0], P.img[1], P.img[2], P.W, P.img[3]);
+    return post_finish_f(ctx, P, n, out);
+}
+
+extern "C" int ag_film_develop(ag_ctx* ctx, const ag_film* film, const ag_grade* grade, int32_t denoise,
+                               uint8_t* out) {
+    if (!ctx || !film || !grade || !out) return fail(AG_E_INVALID, "null argument");
+    if (film->ctx != ctx) return fail(AG_E_INVALID, "film belongs to another context");
+    const size_t n = (size_t)film->w * film->h;
+    if (n == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    PostWork P;
+    DevGrade g;
+    int rc;
+    if ((rc = post_work(ctx, n, &P)) || (rc = post_grade(ctx, grade, P, &g)) || (rc = post_film(ctx, film, P))) return rc;
+    if (denoise) launch_atrous(ctx->stream, (int)film->w, (int)film->h, P.img[0], P.img[1], P.img[2], P.W, P.img[3]);
+    launch_develop(ctx->stream, denoise ? P.img[3] : P.img[0], (int)film->w, (int)film->h, g, P.u8);
+    return post_finish_u8(ctx, P, n, out);
 }
 
 extern "C" int ag_device_alloc(ag_ctx* ctx, size_t bytes, void** out) {

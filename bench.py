@@ -317,6 +317,58 @@ def cavity_ao(args):
     print(json.dumps(res), flush=True)
 
 
+def preview_develop(args):
+    """The live preview's development (live.rs:429-496) of a 1080x1920 signet
+    film (16 spp): à-trous with the albedo and normal guides, then develop
+    (AgX Punchy, dithered 8-bit), on the GPU from the device film to host
+    bytes, and by the oracle on the host (one thread) from the same film."""
+    import argent_amd as ag
+    from argent_amd import workloads
+    wl = workloads.signet_still()
+    w, h = wl["width"], wl["height"]
+    ctx = ag.Context(0)
+    desc = wl["piece"].scene(wl["metal"], 0.07, 0.8, wl["studio"])
+    cam, turn = wl["piece"].shot("hero", 100.0, wl["f_stop"], w / h)
+    desc.d.turntable_deg = turn
+    sc = ag.Scene(ctx, desc)
+    film = ag.Film(ctx, w, h)
+    ag.render_pass(ctx, sc, cam, film, workloads.path_settings(wl["depth"]), 16, wl["seed"])
+    for _ in range(args.warmup):
+        film.develop("agx_punchy", denoise=True)
+    t_dn, t_dev = [], []
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        film.develop("agx_punchy", denoise=True)
+        t_dn.append((time.perf_counter() - t0) * 1e3)
+        t0 = time.perf_counter()
+        film.develop("agx_punchy")
+        t_dev.append((time.perf_counter() - t0) * 1e3)
+    ms_dn, ms_dev = sorted(t_dn)[len(t_dn) // 2], sorted(t_dev)[len(t_dev) // 2]
+    mp = w * h / 1e6
+    res = {"metric": "ms per preview frame, à-trous + develop of a 1080x1920 film to 8-bit, 1 GPU",
+           "value": round(ms_dn, 2), "unit": "ms", "n_gpus": 1, "steps": len(t_dn), "warmup": args.warmup,
+           "ms_per_step": round(ms_dn, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic", "develop_only_ms": round(ms_dev, 2),
+           "mpix_per_s": round(mp / (ms_dn * 1e-3), 1),
+           "config": {"workload": "signet still 1080x1920 at 16 spp, à-trous (5 passes, albedo + normal guides), "
+                                  "AgX Punchy, TPDF dither, 8-bit; film on the device to bytes on the host",
+                      "width": w, "height": h},
+           "timing": "wall clock per call, median, including the 6.2 MB copy to the host"}
+    if not args.no_cpu:
+        import oracle as O
+        color, albedo, normal = film.color(), film.albedo(), film.normal()
+        t0 = time.perf_counter()
+        O.develop(O.atrous(color, albedo, normal), "agx_punchy")
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(dt * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "the same frame once", "cpu_model": _cpu_model()}
+        res["cpu_over_gpu"] = round(dt * 1e3 / ms_dn, 1)
+    film.close()
+    sc.close()
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +383,7 @@ def main():
                          "and the white rig's softboxes, 512 spp, depth 24")
     ap.add_argument("--rays", action="store_true", help="the 2^24-ray closest/any-hit batch instead of a still")
     ap.add_argument("--ao", action="store_true", help="cavity ambient occlusion of the signet (scene prep) instead")
+    ap.add_argument("--develop", action="store_true", help="preview development (à-trous + tone map) instead")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -353,6 +406,9 @@ def main():
         return
     if args.ao:
         cavity_ao(args)
+        return
+    if args.develop:
+        preview_develop(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "

@@ -309,6 +309,39 @@ int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_d
  * smoothing steps are mesh processing on the host. */
 int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t samples, float distance, float* out);
 
+/* Preview development (argent-post), on the device.
+ * Tone operators and looks of tonemap.rs:41-67 (Grade): */
+enum {
+    AG_TONEMAP_AGX = 0,        /* ToneMap::AgX(AgxLook::Base) */
+    AG_TONEMAP_AGX_PUNCHY = 1, /* AgX Punchy, the reference's default grade */
+    AG_TONEMAP_AGX_GOLDEN = 2,
+    AG_TONEMAP_ACES = 3,       /* AcesFitted */
+    AG_TONEMAP_KHRONOS = 4,    /* KhronosNeutral */
+    AG_TONEMAP_REINHARD = 5,
+    AG_TONEMAP_CLAMP = 6
+};
+typedef struct ag_grade {  /* tonemap.rs:53-58 */
+    float exposure_ev;
+    int32_t tonemap;       /* AG_TONEMAP_* */
+    float saturation;      /* 1 = unchanged */
+} ag_grade;
+/* develop (tonemap.rs:370-388): exposure x 2^ev, tone operator, saturation,
+ * sRGB encode (through the reference's per-grade tables), TPDF dither, 8 bit.
+ * rgb: w*h*3 floats, scene-linear (host); out: w*h*3 bytes (host), row-major. */
+int ag_develop_image(ag_ctx* ctx, const float* rgb, uint32_t w, uint32_t h, const ag_grade* grade, uint8_t* out);
+/* The built-in à-trous denoiser (denoise.rs:660-709, the DenoiseBackend::Atrous
+ * path of denoise_with): 5 edge-avoiding passes, albedo demodulation and
+ * albedo/normal edge stops when those are given (NULL: not used).  Host
+ * arrays of w*h*3 floats. */
+int ag_denoise_image(ag_ctx* ctx, const float* color, const float* albedo, const float* normal, uint32_t w,
+                     uint32_t h, float* out);
+/* The same from a film on the device (Film::color/albedo/normal,
+ * render.rs:35-64), without a host round trip of the linear images: the live
+ * preview's development (live.rs:429-496: à-trous with both guides when
+ * `denoise` is nonzero, then develop).  out: w*h*3 floats / bytes (host). */
+int ag_film_denoise(ag_ctx* ctx, const ag_film* film, float* out);
+int ag_film_develop(ag_ctx* ctx, const ag_film* film, const ag_grade* grade, int32_t denoise, uint8_t* out);
+
 /* Material::bsdf + Bsdf::{eval,pdf,sample} (material.rs:96-138, bsdf.rs:53-239)
  * evaluated on the device for n given inputs, for checking the shading code:
  * material index, 4 wavelengths, cavity mask, u = [layer, u0, u1, u2], wo and

@@ -43,6 +43,11 @@ def lib():
             ("or_intersect", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _f, _f, _f, _u32]),
             ("or_occluded", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _u8]),
             ("or_ambient_occlusion", C.c_int, [C.c_void_p, C.c_uint32, C.c_float, C.c_int32, C.c_int32, _f]),
+            ("or_tonemap_pixel", C.c_int, [_f, C.c_uint32, C.c_float, C.c_int32, C.c_float, _f]),
+            ("or_develop_encoded", C.c_int, [_f, C.c_uint32, C.c_float, C.c_int32, C.c_float, _f]),
+            ("or_srgb_encode", C.c_float, [C.c_float]),
+            ("or_develop", C.c_int, [_f, C.c_uint32, C.c_uint32, C.c_float, C.c_int32, C.c_float, _u8]),
+            ("or_atrous", C.c_int, [C.c_uint32, C.c_uint32, _f, _f, _f, _f]),
             ("or_brute_intersect", C.c_int, [_f, _u32, C.c_uint32, *rays, C.c_uint32, C.c_int32, _f, _f, _f, _u32]),
             ("or_render_pass", C.c_int, [C.c_void_p, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, _f, C.c_uint32,
                                          C.POINTER(A.PathSettingsC), C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32,
@@ -227,3 +232,54 @@ def env_pdf(marg, cond, dirs, rotation_deg=0.0):
                      _p(dirs), len(dirs), _p(pdfs))
     return pdfs
 
+
+
+# ------------------------------------------------------------------ preview development (post.cpp)
+TONEMAPS = {"agx": 0, "agx_punchy": 1, "agx_golden": 2, "aces": 3, "khronos": 4, "reinhard": 5, "clamp": 6}
+
+
+def _img(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, 3))
+
+
+def tonemap_pixel(rgb, op, ev=0.0, sat=1.0):
+    """tonemap.rs:207-218 (exact, display-linear), rgb (n, 3)."""
+    x = _img(rgb)
+    out = np.empty_like(x)
+    if lib().or_tonemap_pixel(_p(x), len(x), ev, TONEMAPS.get(op, op), sat, _p(out)):
+        raise ValueError(op)
+    return out
+
+
+def develop_encoded(rgb, op, ev=0.0, sat=1.0):
+    """tonemap.rs:262-338 (the table fast path): sRGB-encoded 0..1, rgb (n, 3)."""
+    x = _img(rgb)
+    out = np.empty_like(x)
+    if lib().or_develop_encoded(_p(x), len(x), ev, TONEMAPS.get(op, op), sat, _p(out)):
+        raise ValueError(op)
+    return out
+
+
+def srgb_encode(v):
+    return np.array([lib().or_srgb_encode(float(x)) for x in np.ravel(v)], np.float32).reshape(np.shape(v))
+
+
+def develop(img, op, ev=0.0, sat=1.0):
+    """tonemap.rs:370-388: (h, w, 3) float32 -> (h, w, 3) uint8."""
+    img = np.ascontiguousarray(np.asarray(img, np.float32))
+    h, w = img.shape[:2]
+    out = np.empty((h, w, 3), np.uint8)
+    if lib().or_develop(_p(img), w, h, ev, TONEMAPS.get(op, op), sat, _p(out, C.c_uint8)):
+        raise ValueError(op)
+    return out
+
+
+def atrous(color, albedo=None, normal=None):
+    """denoise.rs:660-709: (h, w, 3) float32 images -> (h, w, 3)."""
+    color = np.ascontiguousarray(np.asarray(color, np.float32))
+    h, w = color.shape[:2]
+    a = None if albedo is None else np.ascontiguousarray(np.asarray(albedo, np.float32))
+    n = None if normal is None else np.ascontiguousarray(np.asarray(normal, np.float32))
+    out = np.empty_like(color)
+    lib().or_atrous(w, h, _p(color), _p(a) if a is not None else None, _p(n) if n is not None else None, _p(out))
+    return out

@@ -82,6 +82,14 @@ int or_env_sample(const float* marginal, const float* conditional, int32_t w, in
 int or_env_pdf(const float* marginal, const float* conditional, int32_t w, int32_t h, float rotation_deg,
                const float* dirs, uint32_t n, float* pdfs);
 
+// Preview development (post.cpp): argent-post tonemap.rs and denoise.rs.
+// op: 0-2 AgX Base/Punchy/Golden, 3 ACES fitted, 4 Khronos Neutral, 5 Reinhard, 6 Clamp.
+int or_tonemap_pixel(const float* rgb, uint32_t n, float ev, int32_t op, float sat, float* out);
+int or_develop_encoded(const float* rgb, uint32_t n, float ev, int32_t op, float sat, float* out);
+float or_srgb_encode(float x);
+int or_develop(const float* rgb, uint32_t w, uint32_t h, float ev, int32_t op, float sat, uint8_t* out);
+int or_atrous(uint32_t w, uint32_t h, const float* color, const float* albedo, const float* normal, float* out);
+
 #ifdef __cplusplus
 }
 #endif
