@@ -39,7 +39,10 @@ constexpr uint32_t BVH_COUNT_SHIFT = 27;
 constexpr uint32_t BVH_INDEX_MASK = (1u << 27) - 1u;
 constexpr int BVH_MAX_LEAF = 4;
 constexpr int BVH_WIDTH = 8;
-constexpr int BVH_NODE_FLOATS = 20;
+#ifndef AG_NODE_FLOATS
+#define AG_NODE_FLOATS 20  // 80-byte nodes; the traversal kernels read the first 20 floats
+#endif
+constexpr int BVH_NODE_FLOATS = AG_NODE_FLOATS;
 
 struct WideBvh {
     std::vector<float> nodes;  // BVH_NODE_FLOATS per node

@@ -175,7 +175,10 @@ constexpr float BOX_HI = 1.0f + 1.52587890625e-5f;
 // Node layout (80 B): see the host builder.  Children's boxes are
 // p + q * 2^(e-127) per plane (an exact fma: the quantized box contains the
 // exact one as evaluated here); slab test as bvh.rs:280-299 on each.
-constexpr int NODE_F4 = 5;
+#ifndef AG_NODE_FLOATS
+#define AG_NODE_FLOATS 20
+#endif
+constexpr int NODE_F4 = AG_NODE_FLOATS / 4;  // node stride in float4 (the node itself is 5 float4)
 
 AGD void slab8(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float tm[8], uint32_t ch[8]) {
     float4 n0 = nd[0];

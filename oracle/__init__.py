@@ -42,6 +42,7 @@ def lib():
             ("or_scene_stats", C.c_int, [C.c_void_p, _u32, _u32]),
             ("or_intersect", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _f, _f, _f, _u32]),
             ("or_occluded", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _u8]),
+            ("or_intersect_stats", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, _u64]),
             ("or_ambient_occlusion", C.c_int, [C.c_void_p, C.c_uint32, C.c_float, C.c_int32, C.c_int32, _f]),
             ("or_tonemap_pixel", C.c_int, [_f, C.c_uint32, C.c_float, C.c_int32, C.c_float, _f]),
             ("or_develop_encoded", C.c_int, [_f, C.c_uint32, C.c_float, C.c_int32, C.c_float, _f]),
@@ -117,6 +118,13 @@ class OracleScene:
         lib().or_intersect(self.h, *[_p(c) for c in cols], _p(tm), n, mode, threads, _p(t), _p(u), _p(v),
                            _p(prim, C.c_uint32))
         return t, u, v, prim
+
+    def intersect_stats(self, o, d, tmax=None, mode=MODE_ROBUST):
+        """(Node4 visits, triangle tests) of the reference's closest-hit query over these rays."""
+        cols, tm, n = _rays(o, d, tmax)
+        out = np.zeros(2, np.uint64)
+        lib().or_intersect_stats(self.h, *[_p(c) for c in cols], _p(tm), n, mode, _p(out, C.c_uint64))
+        return int(out[0]), int(out[1])
 
     def occluded(self, o, d, tmax=None, mode=MODE_ROBUST, threads=0):
         cols, tm, n = _rays(o, d, tmax)

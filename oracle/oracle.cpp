@@ -1108,7 +1108,8 @@ struct Bvh {
     }
 
     // bvh.rs:159-213
-    bool intersect(const Ray& ray, float t_max, bool robust, TriHit* out) const {
+    // counts (optional): [0] += Node4 visits, [1] += triangle tests
+    bool intersect(const Ray& ray, float t_max, bool robust, TriHit* out, uint64_t* counts = nullptr) const {
         if (tris.empty()) return false;
         V3 inv(1.0f / ray.d.x, 1.0f / ray.d.y, 1.0f / ray.d.z);
         float best_t = t_max;
@@ -1128,6 +1129,7 @@ struct Bvh {
             if (e.entry & LEAF) {
                 if (e.entry == EMPTY) continue;
                 uint32_t start = e.entry & ~LEAF;
+                if (counts) counts[1] += e.count;
                 for (uint32_t i = start; i < start + e.count; i++) {
                     float t, u, v;
                     if (!tri_intersect(tris[i], ray, &t, &u, &v)) continue;
@@ -1146,6 +1148,7 @@ struct Bvh {
                 continue;
             }
             const Node4& n = wide[e.entry];
+            if (counts) counts[0]++;
             float tmin[4];
             uint32_t mask = slab4(n, ray, inv, best_t, robust, tmin);
             std::pair<int, float> order[4];
@@ -1911,6 +1914,23 @@ extern "C" int or_intersect(const or_scene* s, const float* ox, const float* oy,
             }
         }
     });
+    return 0;
+}
+
+// Work of the reference's closest-hit query (bvh.rs:159-213) on its own
+// 4-wide tree: totals[0] = Node4 visits, totals[1] = triangle tests.
+extern "C" int or_intersect_stats(const or_scene* s, const float* ox, const float* oy, const float* oz,
+                                  const float* dx, const float* dy, const float* dz, const float* tmax, uint32_t n,
+                                  int32_t mode, uint64_t* totals) {
+    if (!s) return 1;
+    uint64_t c[2] = {0, 0};
+    for (uint32_t i = 0; i < n; i++) {
+        Ray r{V3(ox[i], oy[i], oz[i]), V3(dx[i], dy[i], dz[i])};
+        TriHit h;
+        s->sc.bvh.intersect(r, tmax ? tmax[i] : INF, mode == OR_MODE_ROBUST, &h, c);
+    }
+    totals[0] = c[0];
+    totals[1] = c[1];
     return 0;
 }
 

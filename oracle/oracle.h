@@ -39,6 +39,9 @@ int or_scene_stats(const or_scene* s, uint32_t* binary_nodes, uint32_t* wide_nod
 int or_intersect(const or_scene* s, const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
                  const float* dz, const float* tmax, uint32_t n, int32_t mode, int32_t threads, float* t, float* u,
                  float* v, uint32_t* prim);
+// reference-tree work of closest-hit queries: totals = {Node4 visits, triangle tests}
+int or_intersect_stats(const or_scene* s, const float* ox, const float* oy, const float* oz, const float* dx,
+                       const float* dy, const float* dz, const float* tmax, uint32_t n, int32_t mode, uint64_t* totals);
 // cavity AO per vertex (argent-mesh cavity.rs:95-128); 2 when the scene has no normals
 int or_ambient_occlusion(const or_scene* s, uint32_t samples, float distance, int32_t mode, int32_t threads,
                          float* out);
