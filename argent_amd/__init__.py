@@ -21,7 +21,7 @@ import numpy as np
 from . import _abi as A
 from ._abi import ArgentError, check
 
-__all__ = ["Context", "SceneDesc", "Scene", "Film", "PathSettings", "Material", "render_pass", "render_pass_async", "Pass", "render_views",
+__all__ = ["Mesh", "load_stl", "Context", "SceneDesc", "Scene", "Film", "PathSettings", "Material", "render_pass", "render_pass_async", "Pass", "render_views",
            "look_at", "frame", "VIEWS", "ArgentError", "alloy_f0_srgb", "alloy_ior", "grade", "develop_image",
            "denoise_image"]
 
@@ -403,6 +403,79 @@ def grade(tonemap="agx_punchy", exposure_ev=0.0, saturation=1.0):
     return A.Grade(exposure_ev, A.TONEMAPS[tonemap] if isinstance(tonemap, str) else int(tonemap), saturation)
 
 
+def _weld_dict(w):
+    return {k: getattr(w, k) for k, _ in A.WeldReport._fields_}
+
+
+class Mesh:
+    """A device-resident triangle mesh (argent-mesh lib.rs:33-39, without parts)
+    with the reference's welding (weld.rs:38-62) and compaction (weld.rs:66-91)
+    run on the GPU."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+
+    @classmethod
+    def from_arrays(cls, ctx, positions, triangles):
+        p = np.ascontiguousarray(np.asarray(positions, np.float32).reshape(-1, 3))
+        t = np.ascontiguousarray(np.asarray(triangles, np.uint32).reshape(-1, 3))
+        h = C.c_void_p()
+        check(A.lib().ag_mesh_create(ctx.h, _ptr(p), len(p), _ptr(t, C.c_uint32), len(t), C.byref(h)))
+        return cls(ctx, h)
+
+    def close(self):
+        if self.h:
+            A.lib().ag_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def counts(self):
+        nv, nt = C.c_uint32(), C.c_uint32()
+        check(A.lib().ag_mesh_info(self.h, C.byref(nv), C.byref(nt)))
+        return nv.value, nt.value
+
+    def read(self):
+        """(positions (nv, 3) float32, triangles (nt, 3) uint32) in host memory."""
+        nv, nt = self.counts
+        p = np.empty((nv, 3), np.float32)
+        t = np.empty((nt, 3), np.uint32)
+        check(A.lib().ag_mesh_read(self.h, _ptr(p), _ptr(t, C.c_uint32)))
+        return p, t
+
+    def weld(self, rel_tolerance=1e-6):
+        """weld.rs:38-62; returns the WeldReport as a dict."""
+        r = A.WeldReport()
+        check(A.lib().ag_mesh_weld(self.h, rel_tolerance, C.byref(r)))
+        return _weld_dict(r)
+
+    def compact(self):
+        """weld.rs:66-91; returns the number of triangles dropped."""
+        d = C.c_uint64()
+        check(A.lib().ag_mesh_compact(self.h, C.byref(d)))
+        return d.value
+
+
+def load_stl(ctx, data, weld=True, weld_tolerance=1e-6, unit_scale=1.0, crease_angle_deg=50.0, device_ptr=None):
+    """load_bytes (io.rs:87-126) for a binary STL: returns (Mesh, report dict).
+    `data` is the file's bytes in host memory; for bytes already in device
+    memory pass their address as device_ptr and their length as data."""
+    o = A.LoadOptions(int(bool(weld)), weld_tolerance, crease_angle_deg, unit_scale)
+    r = A.LoadReport()
+    h = C.c_void_p()
+    if device_ptr is not None:
+        check(A.lib().ag_mesh_load_stl(ctx.h, C.c_void_p(device_ptr), int(data), 1, C.byref(o), C.byref(h),
+                                       C.byref(r)))
+    else:
+        buf = bytes(data)
+        check(A.lib().ag_mesh_load_stl(ctx.h, buf, len(buf), 0, C.byref(o), C.byref(h), C.byref(r)))
+    rep = {k: getattr(r, k) for k, _ in A.LoadReport._fields_ if k != "weld"}
+    rep["weld"] = _weld_dict(r.weld) if r.welded else None
+    return Mesh(ctx, h), rep
+
+
 def develop_image(ctx, img, tonemap="agx_punchy", exposure_ev=0.0, saturation=1.0):
     """develop (tonemap.rs:370-388) on the GPU: (h, w, 3) float32 -> (h, w, 3) uint8."""
     img = np.ascontiguousarray(np.asarray(img, np.float32))
@@ -499,6 +572,27 @@ def render_views(ctx, scene, views, ps, materials=()):
         arr[i].scene = v["scene"].h if v.get("scene") is not None else None
     mats = (A.Material * max(1, len(materials)))(*materials) if materials else None
     check(A.lib().ag_render_views(ctx.h, scene.h, mats, len(materials), arr, len(views), C.byref(ps.c())))
+
+
+class DeviceBytes:
+    """Bytes copied into device memory (e.g. a file for ag_mesh_load_stl)."""
+
+    def __init__(self, ctx, data):
+        self.ctx, self.n = ctx, len(data)
+        p = C.c_void_p()
+        check(A.lib().ag_device_alloc(ctx.h, max(self.n, 4), C.byref(p)))
+        self.p = p
+        buf = (C.c_char * self.n).from_buffer_copy(bytes(data))
+        check(A.lib().ag_memcpy_to_device(ctx.h, p, C.cast(buf, C.c_void_p), self.n))
+
+    @property
+    def ptr(self):
+        return self.p.value
+
+    def close(self):
+        if self.p:
+            A.lib().ag_device_free(self.ctx.h, self.p)
+            self.p = None
 
 
 class DeviceRays:

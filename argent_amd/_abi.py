@@ -77,6 +77,26 @@ class Grade(C.Structure):
 TONEMAPS = {"agx": 0, "agx_punchy": 1, "agx_golden": 2, "aces": 3, "khronos": 4, "reinhard": 5, "clamp": 6}
 
 
+class LoadOptions(C.Structure):
+    """ag_load_options (argent-mesh lib.rs:53-74)."""
+    _fields_ = [("weld", C.c_int32), ("weld_tolerance", C.c_float), ("crease_angle_deg", C.c_float),
+                ("unit_scale", C.c_float)]
+
+
+class WeldReport(C.Structure):
+    """ag_weld_report (weld.rs:21-33)."""
+    _fields_ = [("input_vertices", C.c_uint64), ("output_vertices", C.c_uint64), ("exact_merges", C.c_uint64),
+                ("tolerance_merges", C.c_uint64), ("dropped_triangles", C.c_uint64), ("tolerance_mm", C.c_float)]
+
+
+class LoadReport(C.Structure):
+    """ag_load_report (io.rs:21-40, binary STL)."""
+    _fields_ = [("file_bytes", C.c_uint64), ("raw_triangles", C.c_uint64), ("raw_vertices", C.c_uint64),
+                ("invalid_triangles", C.c_uint64), ("trailing_bytes", C.c_uint64),
+                ("declared_triangles", C.c_uint64), ("welded", C.c_int32), ("weld", WeldReport),
+                ("upload_ms", C.c_float), ("parse_ms", C.c_float), ("weld_ms", C.c_float)]
+
+
 class RaysSoA(C.Structure):
     _fields_ = [("ox", f32p), ("oy", f32p), ("oz", f32p), ("dx", f32p), ("dy", f32p), ("dz", f32p),
                 ("tmax", f32p)]
@@ -204,6 +224,15 @@ _SIGS = [
     ("ag_ctx_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("ag_ctx_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("ag_ctx_enable_timing", C.c_int, [C.c_void_p, C.c_int32]),
+    ("ag_mesh_load_stl", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int32, C.POINTER(LoadOptions),
+                                   C.POINTER(C.c_void_p), C.POINTER(LoadReport)]),
+    ("ag_mesh_create", C.c_int, [C.c_void_p, f32p, C.c_uint32, u32p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("ag_mesh_destroy", None, [C.c_void_p]),
+    ("ag_mesh_info", C.c_int, [C.c_void_p, u32p, u32p]),
+    ("ag_mesh_read", C.c_int, [C.c_void_p, f32p, u32p]),
+    ("ag_mesh_device_arrays", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("ag_mesh_weld", C.c_int, [C.c_void_p, C.c_float, C.POINTER(WeldReport)]),
+    ("ag_mesh_compact", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("ag_ctx_depth_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]

@@ -6,9 +6,18 @@
 
 #include "../../include/argent.h"
 
+struct ihipStream_t;  // hipStream_t is ihipStream_t*
+
 namespace ag {
 
 int fail(int code, const std::string& msg);
+
+// The context's device and HIP stream (runtime.cpp).
+::ihipStream_t* ctx_stream(const ag_ctx* c);
+int ctx_device(const ag_ctx* c);
+// A device workspace of at least `bytes`, owned by the context and reused by
+// later calls (grow-only; its contents do not survive a call that grows it).
+int ctx_workspace(ag_ctx* c, size_t bytes, void** p);
 
 struct DevGrade;  // post.hpp
 // The tables of a tone-mapping grade (tonemap.rs:284-313), host_post.cpp:

@@ -98,6 +98,7 @@ struct ag_ctx {
     std::vector<unsigned long long> depth_rays;  // [2 * TALLY_DEPTHS]: extension, then shadow
     std::vector<double> depth_ms;                // [3 * TALLY_DEPTHS]: extend, shade, shadow
     DevBuf tally_buf;
+    DevBuf mesh_ws;  // mesh ingest temporaries (mesh.hip)
     DevBuf post_buf;  // preview development: images, à-trous work, grade tables, 8-bit output
     DevBuf energy_buf;  // the library's GGX energy table, staged into LDS by the shading kernel
     double last_query_ms = 0.0;
@@ -156,6 +157,16 @@ static const std::vector<float>& library_energy() {
     }();
     return E;
 }
+
+namespace ag {
+ihipStream_t* ctx_stream(const ag_ctx* c) { return c->stream; }
+int ctx_device(const ag_ctx* c) { return c->device; }
+int ctx_workspace(ag_ctx* c, size_t bytes, void** p) {
+    const int rc = c->mesh_ws.ensure(bytes);
+    *p = c->mesh_ws.p;
+    return rc;
+}
+}  // namespace ag
 
 extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     if (!out) return fail(AG_E_INVALID, "null output");

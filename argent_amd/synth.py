@@ -43,6 +43,38 @@ def torus_profile(nu, nv, profile):
     return pos, tris
 
 
+def torus(big_r, r, nu, nv):
+    """synth.rs:147-149: a plain torus of revolution."""
+    def prof(u, v):
+        vv = np.broadcast_to(v, (u.shape[0], v.shape[1]))
+        return np.float32(big_r) + np.float32(r) * np.cos(vv), np.float32(r) * np.sin(vv)
+    return torus_profile(nu, nv, prof)
+
+
+def cuboid(size):
+    """synth.rs:9-35: an axis-aligned box centred on the origin, 8 vertices and
+    12 outward-wound triangles."""
+    hx, hy, hz = (np.float32(s) * np.float32(0.5) for s in size)
+    pos = np.array([[-hx, -hy, -hz], [hx, -hy, -hz], [hx, hy, -hz], [-hx, hy, -hz],
+                    [-hx, -hy, hz], [hx, -hy, hz], [hx, hy, hz], [-hx, hy, hz]], np.float32)
+    tris = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                     [3, 7, 6], [3, 6, 2], [0, 4, 7], [0, 7, 3], [1, 2, 6], [1, 6, 5]], np.uint32)
+    return pos, tris
+
+
+def unweld(pos, tris):
+    """synth.rs:257-268: every triangle gets its own three vertices."""
+    p = np.asarray(pos, np.float32)[np.asarray(tris).reshape(-1)]
+    return p, np.arange(len(p), dtype=np.uint32).reshape(-1, 3)
+
+
+def signed_volume(pos, tris):
+    """Enclosed volume by the divergence theorem (f64)."""
+    p = np.asarray(pos, np.float64)
+    a, b, c = p[tris[:, 0]], p[tris[:, 1]], p[tris[:, 2]]
+    return float(np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0)
+
+
 def vertex_normals(pos, tris):
     """Area-weighted vertex normals (unit, float32)."""
     p = pos.astype(np.float64)

@@ -342,6 +342,65 @@ int ag_denoise_image(ag_ctx* ctx, const float* color, const float* albedo, const
 int ag_film_denoise(ag_ctx* ctx, const ag_film* film, float* out);
 int ag_film_develop(ag_ctx* ctx, const ag_film* film, const ag_grade* grade, int32_t denoise, uint8_t* out);
 
+/* ------------------------------------------------------------ mesh ingest
+ * A device-resident triangle mesh: positions (n_vertices x 3 floats) and
+ * triangles (n_triangles x 3 indices), argent-mesh lib.rs:33-39 (Mesh) without
+ * parts.  Welding and compaction run on the device and give the reference's
+ * output bit for bit. */
+typedef struct ag_mesh ag_mesh;
+
+/* LoadOptions (argent-mesh lib.rs:53-74); defaults weld 1, weld_tolerance
+ * 1e-6, crease_angle_deg 50, unit_scale 1. */
+typedef struct ag_load_options {
+    int32_t weld;
+    float weld_tolerance;    /* fraction of the bounding-box diagonal */
+    float crease_angle_deg;  /* reserved for crease normals */
+    float unit_scale;
+} ag_load_options;
+
+/* WeldReport (weld.rs:21-33). */
+typedef struct ag_weld_report {
+    uint64_t input_vertices, output_vertices, exact_merges, tolerance_merges, dropped_triangles;
+    float tolerance_mm;
+} ag_weld_report;
+
+/* LoadReport (io.rs:21-40) for a binary STL: record counts, the truncation
+ * and padding io/stl.rs:38-50 warns about, the invalid triangles io.rs:206-224
+ * drops, and device time of each stage (upload_ms is host wall time of the
+ * copy when the bytes come from host memory). */
+typedef struct ag_load_report {
+    uint64_t file_bytes, raw_triangles, raw_vertices, invalid_triangles, trailing_bytes, declared_triangles;
+    int32_t welded;
+    ag_weld_report weld;
+    float upload_ms, parse_ms, weld_ms;
+} ag_load_report;
+
+/* load_bytes (io.rs:87-126) for a binary STL (io/stl.rs:34-62): parse, drop
+ * triangles with non-finite corners, scale by unit_scale, then weld
+ * (weld.rs:38-62) or, with weld == 0, compact (weld.rs:66-91).  bytes is host
+ * memory, or device memory (4-byte aligned) when bytes_on_device is nonzero.
+ * Format detection (io.rs:149-180) is the caller's: ASCII STL, OBJ, PLY and
+ * 3MF are not read here.  Errors: AG_E_INVALID for a file shorter than its
+ * header, with no triangles, or with only degenerate ones (io.rs:101-103,
+ * :121-123). */
+int ag_mesh_load_stl(ag_ctx* ctx, const void* bytes, size_t len, int32_t bytes_on_device,
+                     const ag_load_options* opts, ag_mesh** out, ag_load_report* report);
+/* A device mesh from host arrays. */
+int ag_mesh_create(ag_ctx* ctx, const float* positions, uint32_t n_vertices, const uint32_t* triangles,
+                   uint32_t n_triangles, ag_mesh** out);
+void ag_mesh_destroy(ag_mesh* m);
+int ag_mesh_info(const ag_mesh* m, uint32_t* n_vertices, uint32_t* n_triangles);
+/* Copies the mesh to host arrays (either may be NULL). */
+int ag_mesh_read(const ag_mesh* m, float* positions, uint32_t* triangles);
+/* The mesh's device arrays, valid until the next call that changes the mesh. */
+int ag_mesh_device_arrays(const ag_mesh* m, const float** positions, const uint32_t** triangles);
+/* weld (weld.rs:38-62): merges bit-identical vertices, then vertices within
+ * rel_tolerance x the bbox diagonal; drops collapsed triangles and
+ * unreferenced vertices.  A triangle index out of range is AG_E_INVALID. */
+int ag_mesh_weld(ag_mesh* m, float rel_tolerance, ag_weld_report* report);
+/* compact (weld.rs:66-91): drops collapsed triangles and unreferenced vertices. */
+int ag_mesh_compact(ag_mesh* m, uint64_t* dropped_triangles);
+
 /* Material::bsdf + Bsdf::{eval,pdf,sample} (material.rs:96-138, bsdf.rs:53-239)
  * evaluated on the device for n given inputs, for checking the shading code:
  * material index, 4 wavelengths, cavity mask, u = [layer, u0, u1, u2], wo and

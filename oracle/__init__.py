@@ -65,6 +65,11 @@ def lib():
             ("or_env_build", C.c_int, [_f, C.c_int32, C.c_int32, C.c_float, _f, _f, _f, _f]),
             ("or_env_sample", C.c_int, [_f, _f, C.c_int32, C.c_int32, C.c_float, _f, C.c_uint32, _f, _f]),
             ("or_env_pdf", C.c_int, [_f, _f, C.c_int32, C.c_int32, C.c_float, _f, C.c_uint32, _f]),
+            ("or_stl_count", C.c_int64, [C.c_char_p, C.c_size_t]),
+            ("or_weld", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f, _u32, _u32, _u32, _u64, _f]),
+            ("or_compact", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, _u32, _u32, _u32, _u64]),
+            ("or_load_stl", C.c_int, [C.c_char_p, C.c_size_t, C.c_int32, C.c_float, C.c_float, _f, _u32, _u32,
+                                      _u32, _u64, _f]),
         ]
         for name, res, args in sigs:
             fn = getattr(L, name)
@@ -291,3 +296,67 @@ def atrous(color, albedo=None, normal=None):
     out = np.empty_like(color)
     lib().or_atrous(w, h, _p(color), _p(a) if a is not None else None, _p(n) if n is not None else None, _p(out))
     return out
+
+
+WELD_FIELDS = ("input_vertices", "output_vertices", "exact_merges", "tolerance_merges", "dropped_triangles")
+
+
+def _mesh_in(pos, tris):
+    pos = np.ascontiguousarray(np.asarray(pos, np.float32).reshape(-1, 3))
+    tris = np.ascontiguousarray(np.asarray(tris, np.uint32).reshape(-1, 3))
+    return pos, tris
+
+
+def weld(pos, tris, rel_tolerance=1e-6):
+    """weld.rs:38-62: (positions, triangles, report dict)."""
+    pos, tris = _mesh_in(pos, tris)
+    op = np.empty_like(pos)
+    ot = np.empty_like(tris)
+    nv, nt = C.c_uint32(), C.c_uint32()
+    rep = np.zeros(5, np.uint64)
+    tol = C.c_float()
+    if lib().or_weld(_p(pos), len(pos), _p(tris, C.c_uint32), len(tris), rel_tolerance, _p(op), C.byref(nv),
+                     _p(ot, C.c_uint32), C.byref(nt), _p(rep, C.c_uint64), C.byref(tol)):
+        raise IndexError("vertex index out of range")
+    r = dict(zip(WELD_FIELDS, (int(x) for x in rep)))
+    r["tolerance_mm"] = tol.value
+    return op[:nv.value].copy(), ot[:nt.value].copy(), r
+
+
+def compact(pos, tris):
+    """weld.rs:66-91: (positions, triangles, dropped triangles)."""
+    pos, tris = _mesh_in(pos, tris)
+    op = np.empty_like(pos)
+    ot = np.empty_like(tris)
+    nv, nt = C.c_uint32(), C.c_uint32()
+    dropped = C.c_uint64()
+    if lib().or_compact(_p(pos), len(pos), _p(tris, C.c_uint32), len(tris), _p(op), C.byref(nv),
+                        _p(ot, C.c_uint32), C.byref(nt), C.byref(dropped)):
+        raise IndexError("vertex index out of range")
+    return op[:nv.value].copy(), ot[:nt.value].copy(), dropped.value
+
+
+LOAD_ERRORS = {1: "binary STL shorter than its 84-byte header", 2: "file contains no triangles",
+               3: "file has only degenerate triangles"}
+
+
+def load_stl(data, weld=True, weld_tolerance=1e-6, unit_scale=1.0):
+    """io.rs:96-122 for a binary STL: (positions, triangles, report dict)."""
+    data = bytes(data)
+    n = lib().or_stl_count(data, len(data))
+    cap = max(int(n), 1)
+    op = np.empty((3 * cap, 3), np.float32)
+    ot = np.empty((cap, 3), np.uint32)
+    nv, nt = C.c_uint32(), C.c_uint32()
+    rep = np.zeros(10, np.uint64)
+    tol = C.c_float()
+    rc = lib().or_load_stl(data, len(data), int(bool(weld)), weld_tolerance, unit_scale, _p(op), C.byref(nv),
+                           _p(ot, C.c_uint32), C.byref(nt), _p(rep, C.c_uint64), C.byref(tol))
+    if rc:
+        raise ValueError(LOAD_ERRORS[rc])
+    r = dict(zip(("raw_triangles", "raw_vertices", "invalid_triangles", "trailing_bytes", "declared_triangles"),
+                 (int(x) for x in rep[:5])))
+    r["weld"] = dict(zip(WELD_FIELDS, (int(x) for x in rep[5:]))) if weld else None
+    if weld:
+        r["weld"]["tolerance_mm"] = tol.value
+    return op[:nv.value].copy(), ot[:nt.value].copy(), r

@@ -93,6 +93,18 @@ float or_srgb_encode(float x);
 int or_develop(const float* rgb, uint32_t w, uint32_t h, float ev, int32_t op, float sat, uint8_t* out);
 int or_atrous(uint32_t w, uint32_t h, const float* color, const float* albedo, const float* normal, float* out);
 
+// Mesh ingest (mesh.cpp): argent-mesh io/stl.rs, io.rs, weld.rs.
+// rep (weld): input vertices, output vertices, exact merges, tolerance merges,
+// dropped triangles.  or_load_stl's rep: raw triangles, raw vertices, invalid
+// triangles dropped, trailing bytes, declared count, then the weld's five.
+int64_t or_stl_count(const uint8_t* b, size_t len);
+int or_weld(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, float rel_tolerance, float* out_pos,
+            uint32_t* out_nv, uint32_t* out_tris, uint32_t* out_nt, uint64_t* rep5, float* tol_mm);
+int or_compact(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out_pos, uint32_t* out_nv,
+               uint32_t* out_tris, uint32_t* out_nt, uint64_t* dropped);
+int or_load_stl(const uint8_t* b, size_t len, int32_t do_weld, float rel_tolerance, float unit_scale, float* out_pos,
+                uint32_t* out_nv, uint32_t* out_tris, uint32_t* out_nt, uint64_t* rep10, float* tol_mm);
+
 #ifdef __cplusplus
 }
 #endif
