@@ -15,13 +15,14 @@ Every compute call runs on the GPU through the native library; there is no
 CPU fallback in this package.
 """
 import ctypes as C
+import time
 
 import numpy as np
 
 from . import _abi as A
 from ._abi import ArgentError, check
 
-__all__ = ["Mesh", "load_stl", "Context", "SceneDesc", "Scene", "Film", "PathSettings", "Material", "render_pass", "render_pass_async", "Pass", "render_views",
+__all__ = ["Live", "Mesh", "load_stl", "Context", "SceneDesc", "Scene", "Film", "PathSettings", "Material", "render_pass", "render_pass_async", "Pass", "render_views",
            "look_at", "frame", "VIEWS", "ArgentError", "alloy_f0_srgb", "alloy_ior", "grade", "develop_image",
            "denoise_image"]
 
@@ -403,6 +404,53 @@ def grade(tonemap="agx_punchy", exposure_ev=0.0, saturation=1.0):
     return A.Grade(exposure_ev, A.TONEMAPS[tonemap] if isinstance(tonemap, str) else int(tonemap), saturation)
 
 
+class Live:
+    """The live view's render worker (argent serve live.rs:637-830) on the GPU:
+    half-resolution first look, ~70 ms refinement passes, paced frames,
+    à-trous + develop on the device, settle test.  `now_ms` defaults to the
+    monotonic clock."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(A.lib().ag_live_create(ctx.h, C.byref(h)))
+        self.h = h
+        self.size = (2, 2)
+
+    def close(self):
+        if self.h:
+            A.lib().ag_live_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def _now(now_ms):
+        return time.monotonic() * 1e3 if now_ms is None else float(now_ms)
+
+    def set(self, scene, cam, width=540, height=960, max_spp=256, denoise=True, tonemap="agx_punchy",
+            exposure_ev=0.0, saturation=1.0, ps=None, now_ms=None):
+        """A change of scene, camera or size: a new generation."""
+        p = A.LiveParams(width, height, max_spp, 1 if denoise else 0, grade(tonemap, exposure_ev, saturation),
+                         (ps or PathSettings()).c())
+        self.size = (int(width), int(height))
+        self._scene = scene
+        check(A.lib().ag_live_set(self.h, scene.h, C.byref(cam), C.byref(p), self._now(now_ms)))
+
+    def set_look(self, tonemap="agx_punchy", exposure_ev=0.0, saturation=1.0, denoise=True, now_ms=None):
+        """A look-only change: the last frame is re-developed."""
+        g = grade(tonemap, exposure_ev, saturation)
+        check(A.lib().ag_live_set_look(self.h, C.byref(g), 1 if denoise else 0, self._now(now_ms)))
+
+    def step(self, now_ms=None):
+        """One worker iteration: (frame dict, (h, w, 3) uint8 image or None)."""
+        w, h = self.size
+        out = np.empty(w * h * 3, np.uint8)
+        f = A.LiveFrame()
+        check(A.lib().ag_live_step(self.h, self._now(now_ms), C.byref(f), _ptr(out, C.c_uint8)))
+        d = {k: getattr(f, k) for k, _ in A.LiveFrame._fields_}
+        img = out[:f.width * f.height * 3].reshape(f.height, f.width, 3).copy() if f.published else None
+        return d, img
+
+
 def _weld_dict(w):
     return {k: getattr(w, k) for k, _ in A.WeldReport._fields_}
 
@@ -484,6 +532,18 @@ def develop_image(ctx, img, tonemap="agx_punchy", exposure_ev=0.0, saturation=1.
     g = grade(tonemap, exposure_ev, saturation)
     check(A.lib().ag_develop_image(ctx.h, _ptr(img), w, h, C.byref(g), _ptr(out, C.c_uint8)))
     return out
+
+
+def relative_change(ctx, a, b):
+    """live.rs:208-218 on the GPU: mean |dY| / mean Y over every 4th pixel of two (h, w, 3) images."""
+    a = np.ascontiguousarray(np.asarray(a, np.float32))
+    b = np.ascontiguousarray(np.asarray(b, np.float32))
+    if a.shape != b.shape:
+        raise ValueError("images differ in size")
+    h, w = a.shape[:2]
+    out = C.c_double()
+    check(A.lib().ag_relative_change(ctx.h, _ptr(a), _ptr(b), w, h, C.byref(out)))
+    return out.value
 
 
 def denoise_image(ctx, color, albedo=None, normal=None):

@@ -77,6 +77,21 @@ class Grade(C.Structure):
 TONEMAPS = {"agx": 0, "agx_punchy": 1, "agx_golden": 2, "aces": 3, "khronos": 4, "reinhard": 5, "clamp": 6}
 
 
+class LiveParams(C.Structure):
+    """ag_live_params (params.rs: preview size, max_spp, denoise, grade, path settings)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_spp", C.c_uint32), ("denoise", C.c_int32),
+                ("grade", Grade), ("ps", PathSettingsC)]
+
+
+class LiveFrame(C.Structure):
+    """ag_live_frame."""
+    _fields_ = [("published", C.c_int32), ("idle", C.c_int32), ("generation", C.c_uint64), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("spp", C.c_uint32), ("max_spp", C.c_uint32), ("half", C.c_int32),
+                ("converged", C.c_int32), ("settled", C.c_int32), ("denoised", C.c_int32),
+                ("redeveloped", C.c_int32), ("pass_spp", C.c_uint32), ("pass_ms", C.c_float),
+                ("ms_per_spp", C.c_float), ("relative_change", C.c_float)]
+
+
 class LoadOptions(C.Structure):
     """ag_load_options (argent-mesh lib.rs:53-74)."""
     _fields_ = [("weld", C.c_int32), ("weld_tolerance", C.c_float), ("crease_angle_deg", C.c_float),
@@ -233,6 +248,12 @@ _SIGS = [
     ("ag_mesh_device_arrays", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("ag_mesh_weld", C.c_int, [C.c_void_p, C.c_float, C.POINTER(WeldReport)]),
     ("ag_mesh_compact", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("ag_live_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("ag_live_destroy", None, [C.c_void_p]),
+    ("ag_live_set", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(LiveParams), C.c_double]),
+    ("ag_live_set_look", C.c_int, [C.c_void_p, C.POINTER(Grade), C.c_int32, C.c_double]),
+    ("ag_live_step", C.c_int, [C.c_void_p, C.c_double, C.POINTER(LiveFrame), u8p]),
+    ("ag_relative_change", C.c_int, [C.c_void_p, f32p, f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     ("ag_ctx_depth_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]

@@ -329,6 +329,60 @@ void launch_atrous(hipStream_t s, int w, int h, const float4* color, const float
     }
     hipLaunchKernelGGL(k_remod, dim3(g), dim3(256), 0, s, W.sig[cur], albedo ? W.demod : nullptr, n, out);
 }
+// live.rs:208-218 (relative_change): over every 4th pixel, the sums of
+// |lum(a) - lum(b)| and |lum(b)| in f64, lum = Rec. 709 luma of f32 RGB.
+// Each block writes its two partial sums; k_relchange_fold adds them.
+constexpr int RELCHANGE_BLOCKS = 512;
+
+__device__ inline void block_sum2(double a, double b, double* out2) {
+    __shared__ double red[256 / 64][2];
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        b += __shfl_xor(b, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6][0] = a;
+        red[threadIdx.x >> 6][1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0;
+        for (int w = 0; w < 256 / 64; w++) {
+            sa += red[w][0];
+            sb += red[w][1];
+        }
+        out2[0] = sa;
+        out2[1] = sb;
+    }
+}
+
+__global__ void k_relchange(const float4* __restrict__ a, const float4* __restrict__ b, size_t npix, double* part) {
+    double diff = 0.0, base = 0.0;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; 4 * q < npix; q += (size_t)gridDim.x * blockDim.x) {
+        const float4 pa = a[4 * q], pb = b[4 * q];
+        const double la = 0.2126 * (double)pa.x + 0.7152 * (double)pa.y + 0.0722 * (double)pa.z;
+        const double lb = 0.2126 * (double)pb.x + 0.7152 * (double)pb.y + 0.0722 * (double)pb.z;
+        diff += fabs(la - lb);
+        base += fabs(lb);
+    }
+    block_sum2(diff, base, part + 2 * blockIdx.x);
+}
+
+__global__ void k_relchange_fold(const double* part, int nb, double* out2) {
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    block_sum2(a, b, out2);
+}
+
+void launch_relative_change(hipStream_t s, const float4* a, const float4* b, size_t npix, double* part, double* out2) {
+    const int nb = std::max(1, std::min(RELCHANGE_BLOCKS, (int)((npix / 4 + 255) / 256)));
+    hipLaunchKernelGGL(k_relchange, dim3(nb), dim3(256), 0, s, a, b, npix, part);
+    hipLaunchKernelGGL(k_relchange_fold, dim3(1), dim3(256), 0, s, part, nb, out2);
+}
+
 void launch_develop(hipStream_t s, const float4* img, int w, int h, const DevGrade& g, uint8_t* out) {
     hipLaunchKernelGGL(k_develop, dim3(grid_for((size_t)w * h)), dim3(256), 0, s, img, w, h, g, out);
 }

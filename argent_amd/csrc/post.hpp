@@ -39,5 +39,8 @@ void launch_film_images(hipStream_t s, const float* sums, size_t npix, float k, 
 void launch_atrous(hipStream_t s, int w, int h, const float4* color, const float4* albedo, const float4* normal,
                    const AtrousWork& W, float4* out);
 void launch_develop(hipStream_t s, const float4* img, int w, int h, const DevGrade& g, uint8_t* out);
+// out2 = (sum |lum a - lum b|, sum |lum b|) over every 4th pixel, f64;
+// part holds 2 doubles per block (at most 512 blocks).
+void launch_relative_change(hipStream_t s, const float4* a, const float4* b, size_t npix, double* part, double* out2);
 
 }  // namespace ag

@@ -1504,6 +1504,339 @@ extern "C" int ag_film_develop(ag_ctx* ctx, const ag_film* film, const ag_grade*
     return post_finish_u8(ctx, P, n, out);
 }
 
+// ------------------------------------------------------------------ live progressive loop
+// The render worker of the live view (argent serve live.rs:637-830) over the
+// device film: a half-resolution first look at 2 spp, then refinement passes
+// sized toward 70 ms, frames published at paced intervals, each denoised
+// (à-trous) and developed on the device, and a settle test that ends
+// refinement once successive denoised frames differ by less than 0.5 %.
+// The caller's thread drives it (one ag_live_step per worker iteration) and
+// passes its clock, so the pacing is reproducible.
+namespace {
+constexpr uint32_t LIVE_SEED = 1;                 // live.rs:39
+constexpr double MOTION_INTERVAL_MS = 83.0;       // live.rs:41
+constexpr double REFINE_INTERVAL_MS = 166.0;      // live.rs:43
+constexpr double REFINE_SPP = 32.0;               // live.rs:45
+constexpr double REFINE_MAX_MS = 1500.0;          // live.rs:46
+constexpr double MOTION_WINDOW_MS = 350.0;        // live.rs:48
+constexpr double SETTLE_DELTA = 0.005;            // live.rs:54
+constexpr uint32_t SETTLE_MIN_SPP = 64;           // live.rs:55
+constexpr double PASS_TARGET_MS = 70.0;           // live.rs:57
+constexpr double DENOISE_SHARE = 0.35;            // live.rs:63
+constexpr double DENOISE_MAX_MS = 400.0;          // live.rs:65
+}  // namespace
+
+struct ag_live {
+    ag_ctx* ctx = nullptr;
+    const ag_scene* scene = nullptr;
+    ag_camera cam{};
+    ag_live_params p{};
+    ag_film* film = nullptr;
+    ag_film* half = nullptr;
+    uint64_t gen = 0, dgen = 0;          // requested
+    uint64_t cur_gen = 0, cur_dgen = 0;  // rendered
+    double changed_at = 0.0;
+    bool half_done = false, final_done = true;
+    bool has_last_post = false;
+    double last_post = 0.0, ms_per_spp = 0.0, gen_started = -1e30;
+    uint64_t published_gen = 0;
+    bool burst = false;
+    bool settled = false;  // refinement stopped early for settled_max
+    uint32_t settled_max = 0;
+    double dn_ms_per_mpx = 0.0;
+    // the last published linear image (live.rs:493-502: Linear)
+    DevBuf last_img, part;
+    bool has_last = false, last_half = false, last_denoised = false;
+    uint64_t last_gen = 0;
+    uint32_t last_w = 0, last_h = 0, last_spp = 0;
+    ~ag_live() {
+        if (film) ag_film_destroy(film);
+        if (half) ag_film_destroy(half);
+    }
+};
+
+namespace {
+double live_denoise_ms(const ag_live* L, uint32_t pixels) { return L->dn_ms_per_mpx * pixels / 1e6; }
+// live.rs:413-419
+double live_paced(const ag_live* L, double base, uint32_t pixels) {
+    const double ms = live_denoise_ms(L, pixels);
+    if (!L->p.denoise || ms > DENOISE_MAX_MS) return base;
+    return std::max(base, ms / DENOISE_SHARE);
+}
+// live.rs:421-424
+double live_refine_interval(const ag_live* L, uint32_t spp, uint32_t pixels) {
+    const double base = std::min(REFINE_INTERVAL_MS * std::max(std::sqrt(spp / REFINE_SPP), 1.0), REFINE_MAX_MS);
+    return live_paced(L, base, pixels);
+}
+bool live_due(const ag_live* L, double now, double every) { return !L->has_last_post || now - L->last_post >= every; }
+
+// Develops the stored linear image with the current grade into out.
+int live_emit(ag_live* L, const PostWork& P, uint8_t* out, ag_live_frame* fr) {
+    ag_ctx* c = L->ctx;
+    DevGrade g;
+    int rc = post_grade(c, &L->p.grade, P, &g);
+    if (rc) return rc;
+    const size_t n = (size_t)L->last_w * L->last_h;
+    launch_develop(c->stream, L->last_img.as<float4>(), (int)L->last_w, (int)L->last_h, g, P.u8);
+    if ((rc = post_finish_u8(c, P, n, out))) return rc;
+    fr->published = 1;
+    fr->generation = L->last_gen;
+    fr->width = L->last_w;
+    fr->height = L->last_h;
+    fr->spp = L->last_spp;
+    fr->half = L->last_half;
+    fr->denoised = L->last_denoised;
+    return AG_OK;
+}
+
+// live.rs:429-504: denoise, settle test, develop, publish.  Returns whether
+// the frame went out as final (converged or settled) through *final_out.
+int live_publish(ag_live* L, ag_film* f, bool is_half, bool converged, double now, uint8_t* out, ag_live_frame* fr,
+                 bool* final_out) {
+    ag_ctx* c = L->ctx;
+    const size_t n = (size_t)f->w * f->h;
+    PostWork P;
+    int rc;
+    if ((rc = post_work(c, n, &P)) || (rc = post_film(c, f, P))) return rc;
+    const float4* img = P.img[0];
+    bool denoised = false;
+    fr->relative_change = -1.0f;
+    if (L->p.denoise && (converged || live_denoise_ms(L, (uint32_t)n) <= DENOISE_MAX_MS)) {
+        const auto t0 = std::chrono::steady_clock::now();
+        launch_atrous(c->stream, (int)f->w, (int)f->h, P.img[0], P.img[1], P.img[2], P.W, P.img[3]);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        const double r = ms / std::max(n / 1e6, 1e-3);
+        L->dn_ms_per_mpx = L->dn_ms_per_mpx == 0.0 ? r : 0.7 * L->dn_ms_per_mpx + 0.3 * r;
+        img = P.img[3];
+        denoised = true;
+        // settled?  compare with the previous full-resolution frame of this set
+        if (!converged && !is_half && f->samples >= SETTLE_MIN_SPP && L->has_last && L->last_gen == L->cur_gen &&
+            !L->last_half && L->last_denoised && L->last_w == f->w && L->last_h == f->h) {
+            int rc2 = L->part.ensure(2 * 512 * sizeof(double) + 2 * sizeof(double));
+            if (rc2) return rc2;
+            double* sums = L->part.as<double>() + 2 * 512;
+            launch_relative_change(c->stream, L->last_img.as<float4>(), img, n, L->part.as<double>(), sums);
+            double hs[2];
+            HIPCHK(hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            const double rel = hs[1] <= 1e-9 ? 0.0 : hs[0] / hs[1];
+            fr->relative_change = (float)rel;
+            if (rel < SETTLE_DELTA) converged = true;
+        }
+    }
+    if ((rc = L->last_img.ensure(n * sizeof(float4)))) return rc;
+    HIPCHK(hipMemcpyAsync(L->last_img.p, img, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    L->has_last = true;
+    L->last_half = is_half;
+    L->last_denoised = denoised;
+    L->last_gen = L->cur_gen;
+    L->last_w = f->w;
+    L->last_h = f->h;
+    L->last_spp = f->samples;
+    if ((rc = live_emit(L, P, out, fr))) return rc;
+    fr->converged = converged;
+    L->last_post = now;
+    L->has_last_post = true;
+    *final_out = converged;
+    return AG_OK;
+}
+
+int live_render(ag_live* L, ag_film* f, uint32_t count, double* ms) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = ag_render_pass(L->ctx, L->scene, &L->cam, f, &L->p.ps, count, LIVE_SEED);
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+}  // namespace
+
+extern "C" int ag_relative_change(ag_ctx* ctx, const float* a, const float* b, uint32_t w, uint32_t h, double* out) {
+    if (!ctx || !out || ((!a || !b) && (size_t)w * h)) return fail(AG_E_INVALID, "null argument");
+    const size_t n = (size_t)w * h;
+    *out = 0.0;
+    if (n == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    PostWork P;
+    int rc;
+    if ((rc = post_work(ctx, n, &P)) || (rc = post_upload(ctx, a, n, P.stage[0], P.img[0])) ||
+        (rc = post_upload(ctx, b, n, P.stage[1], P.img[1])))
+        return rc;
+    DevBuf tmp;  // 2 partial sums per block (at most 512 blocks), then the two totals
+    if ((rc = tmp.ensure((2 * 512 + 2) * sizeof(double)))) return rc;
+    double* part = tmp.as<double>();
+    launch_relative_change(ctx->stream, P.img[0], P.img[1], n, part, part + 2 * 512);
+    double hs[2];
+    HIPCHK(hipMemcpyAsync(hs, part + 2 * 512, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *out = hs[1] <= 1e-9 ? 0.0 : hs[0] / hs[1];
+    return AG_OK;
+}
+
+extern "C" int ag_live_create(ag_ctx* ctx, ag_live** out) {
+    if (!ctx || !out) return fail(AG_E_INVALID, "null argument");
+    auto* L = new ag_live();
+    L->ctx = ctx;
+    *out = L;
+    return AG_OK;
+}
+
+extern "C" void ag_live_destroy(ag_live* L) { delete L; }
+
+extern "C" int ag_live_set(ag_live* L, const ag_scene* scene, const ag_camera* cam, const ag_live_params* p,
+                           double now_ms) {
+    if (!L || !scene || !cam || !p) return fail(AG_E_INVALID, "null argument");
+    if (scene->ctx != L->ctx) return fail(AG_E_INVALID, "scene belongs to another context");
+    if (p->width < 2 || p->height < 2 || p->max_spp < 1 || p->max_spp > 65536)
+        return fail(AG_E_INVALID, "live view needs width, height >= 2 and max_spp in 1..65536");
+    L->scene = scene;
+    L->cam = *cam;
+    L->p = *p;
+    L->gen++;
+    L->changed_at = now_ms;
+    return AG_OK;
+}
+
+extern "C" int ag_live_set_look(ag_live* L, const ag_grade* grade, int32_t denoise, double now_ms) {
+    if (!L || !grade) return fail(AG_E_INVALID, "null argument");
+    L->p.grade = *grade;
+    L->p.denoise = denoise;
+    L->dgen++;
+    (void)now_ms;
+    return AG_OK;
+}
+
+// One iteration of live.rs:656-823.
+extern "C" int ag_live_step(ag_live* L, double now_ms, ag_live_frame* fr, uint8_t* out) {
+    if (!L || !fr || !out) return fail(AG_E_INVALID, "null argument");
+    *fr = ag_live_frame{};
+    fr->relative_change = -1.0f;
+    if (!L->scene) {
+        fr->idle = 1;
+        return AG_OK;
+    }
+    HIPCHK(hipSetDevice(L->ctx->device));
+    const ag_live_params& p = L->p;
+    const bool more = (!L->film || L->film->samples < p.max_spp) && !(L->settled && L->settled_max == p.max_spp);
+    if (!(L->gen != L->cur_gen || L->dgen != L->cur_dgen || more || !L->final_done)) {
+        fr->idle = 1;
+        fr->generation = L->cur_gen;
+        fr->spp = L->film ? L->film->samples : 0;
+        fr->max_spp = p.max_spp;
+        return AG_OK;
+    }
+    double now = now_ms;
+    int rc;
+    if (L->gen != L->cur_gen) {
+        // a change: restart accumulation (live.rs:683-719)
+        const uint32_t w = p.width, h = p.height;
+        if (!L->film || L->film->w != w || L->film->h != h) {
+            if (L->film) ag_film_destroy(L->film);
+            if (L->half) ag_film_destroy(L->half);
+            L->film = L->half = nullptr;
+            if ((rc = ag_film_create(L->ctx, w, h, &L->film)) ||
+                (rc = ag_film_create(L->ctx, std::max(w / 2, 2u), std::max(h / 2, 2u), &L->half)))
+                return rc;
+        } else {
+            if ((rc = ag_film_clear(L->film)) || (rc = ag_film_clear(L->half))) return rc;
+        }
+        L->cur_gen = L->gen;
+        L->cur_dgen = L->dgen;
+        L->half_done = false;
+        L->final_done = false;
+        L->settled = false;
+        L->burst = now - L->gen_started < MOTION_WINDOW_MS;
+        L->gen_started = now;
+    }
+    fr->generation = L->cur_gen;
+    fr->max_spp = p.max_spp;
+    bool fin = false;
+    if (L->dgen != L->cur_dgen) {
+        // development-only change: re-develop what was published (live.rs:723-740)
+        L->cur_dgen = L->dgen;
+        if (L->has_last && L->last_gen == L->cur_gen && L->last_denoised == (p.denoise != 0)) {
+            PostWork P;
+            if ((rc = post_work(L->ctx, (size_t)L->last_w * L->last_h, &P)) || (rc = live_emit(L, P, out, fr))) return rc;
+            fr->redeveloped = 1;
+            L->last_post = now;
+            L->has_last_post = true;
+            return AG_OK;
+        }
+        if (L->film->samples > 0 || L->half->samples > 0) {
+            const bool is_half = L->film->samples == 0;
+            return live_publish(L, is_half ? L->half : L->film, is_half, L->film->samples >= p.max_spp, now, out, fr,
+                                &fin);
+        }
+    }
+    const bool motion = now - L->gen_started < MOTION_WINDOW_MS;
+    if (!L->half_done) {
+        // first look at a new set: half resolution, 2 spp (live.rs:744-756)
+        double ms;
+        if ((rc = live_render(L, L->half, 2, &ms))) return rc;
+        now += ms;
+        fr->pass_spp = 2;
+        fr->pass_ms = (float)ms;
+        L->half_done = true;
+        const double gap = L->burst ? MOTION_INTERVAL_MS : MOTION_INTERVAL_MS / 4;
+        if (live_due(L, now, live_paced(L, gap, L->half->w * L->half->h))) {
+            if ((rc = live_publish(L, L->half, true, false, now, out, fr, &fin))) return rc;
+            L->published_gen = L->cur_gen;
+        }
+        fr->spp = L->film->samples;
+        return AG_OK;
+    }
+    const uint32_t max = p.max_spp;
+    if (L->film->samples < max && !(L->settled && L->settled_max == max)) {
+        // refinement pass sized toward PASS_TARGET_MS (live.rs:757-812)
+        uint32_t count = 1;
+        if (!motion && L->ms_per_spp > 0.0)
+            count = (uint32_t)std::min(std::max(std::floor(PASS_TARGET_MS / L->ms_per_spp), 1.0), 32.0);
+        count = std::min(count, max - L->film->samples);
+        double ms;
+        if ((rc = live_render(L, L->film, count, &ms))) return rc;
+        now += ms;
+        const double per = ms / count;
+        L->ms_per_spp = L->ms_per_spp == 0.0 ? per : 0.75 * L->ms_per_spp + 0.25 * per;
+        fr->pass_spp = count;
+        fr->pass_ms = (float)ms;
+        fr->ms_per_spp = (float)L->ms_per_spp;
+        fr->spp = L->film->samples;
+        const bool converged = L->film->samples >= max;
+        const uint32_t pixels = L->film->w * L->film->h;
+        const double interval = motion ? live_paced(L, MOTION_INTERVAL_MS, pixels)
+                                       : live_refine_interval(L, L->film->samples, pixels);
+        if (!converged && motion && L->film->samples < 4) {
+            // while in motion the half-resolution look stands in until 4 spp
+            if (L->published_gen != L->cur_gen &&
+                live_due(L, now, live_paced(L, MOTION_INTERVAL_MS, L->half->w * L->half->h))) {
+                if ((rc = live_publish(L, L->half, true, false, now, out, fr, &fin))) return rc;
+                L->published_gen = L->cur_gen;
+            }
+            return AG_OK;
+        }
+        if (converged || live_due(L, now, interval)) {
+            if ((rc = live_publish(L, L->film, false, converged, now, out, fr, &fin))) return rc;
+            L->published_gen = L->cur_gen;
+            if (fin) {
+                L->final_done = true;
+                if (!converged) {
+                    L->settled = true;
+                    L->settled_max = max;
+                    fr->settled = 1;
+                }
+            }
+        }
+        return AG_OK;
+    }
+    if (!L->final_done) {
+        // max_spp lowered below what the film holds: publish it as final
+        if ((rc = live_publish(L, L->film, false, true, now, out, fr, &fin))) return rc;
+        L->published_gen = L->cur_gen;
+        L->final_done = true;
+    }
+    fr->spp = L->film->samples;
+    return AG_OK;
+}
+
 extern "C" int ag_device_alloc(ag_ctx* ctx, size_t bytes, void** out) {
     if (!ctx || !out) return fail(AG_E_INVALID, "null argument");
     HIPCHK(hipSetDevice(ctx->device));

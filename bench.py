@@ -442,6 +442,91 @@ def mesh_ingest(args):
     print(json.dumps(res), flush=True)
 
 
+def live_view(args):
+    """The live view's worker (live.rs:637-830) on the signet at the preview
+    size (540x960, params.rs:346-351), max_spp 256, denoised frames: each of
+    --steps generations starts with a change (ag_live_set) and steps on the
+    wall clock until its final frame; reported are the medians of time to the
+    first look (half resolution, 2 spp, developed 8-bit on the host), to the
+    first full-resolution frame and to the final frame.  The oracle's first
+    look (render on the host cores, à-trous and develop on one) is the CPU
+    baseline."""
+    import argent_amd as ag
+    from argent_amd import workloads
+    wl = workloads.signet_still()
+    w, h, max_spp = 540, 960, 256
+    ctx = ag.Context(0)
+    desc = wl["piece"].scene(wl["metal"], 0.07, 0.8, wl["studio"])
+    cam, turn = wl["piece"].shot("hero", 100.0, wl["f_stop"], w / h)
+    desc.d.turntable_deg = turn
+    sc = ag.Scene(ctx, desc)
+    lv = ag.Live(ctx)
+
+    clock = {"offset": 0.0}
+
+    def generation():
+        # each generation is a lone change: the session clock jumps 10 s ahead
+        # (a change within 350 ms of the previous one is a burst, whose first
+        # look is paced, live.rs:709, :750)
+        clock["offset"] += 1e4
+        t0 = time.monotonic() * 1e3
+        lv.set(sc, cam, w, h, max_spp=max_spp, now_ms=t0 + clock["offset"])
+        g = {"first_look_ms": None, "first_full_ms": None, "final_ms": None, "frames": 0, "passes": 0}
+        while True:
+            f, _ = lv.step(now_ms=time.monotonic() * 1e3 + clock["offset"])
+            now = time.monotonic() * 1e3 - t0
+            g["passes"] += 1 if f["pass_spp"] else 0
+            if f["published"]:
+                g["frames"] += 1
+                if f["half"] and g["first_look_ms"] is None:
+                    g["first_look_ms"] = now
+                if not f["half"] and g["first_full_ms"] is None:
+                    g["first_full_ms"] = now
+            if f["converged"]:
+                g.update(final_ms=now, spp=f["spp"], settled=bool(f["settled"]), last_change=f["relative_change"],
+                         ms_per_spp=f["ms_per_spp"])
+                return g
+            if f["idle"]:
+                raise RuntimeError("live view went idle before its final frame")
+
+    for _ in range(args.warmup):
+        generation()
+    gens = [generation() for _ in range(max(args.steps, 1))]
+    med = lambda k: round(float(np.median([g[k] for g in gens])), 2)
+    res = {"metric": "live view time to first look (half resolution, 2 spp, denoised, 8-bit), 1 GPU",
+           "value": med("first_look_ms"), "unit": "ms", "n_gpus": 1, "steps": len(gens), "warmup": args.warmup,
+           "ms_per_step": med("final_ms"), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "signet ring, sterling, white rig, preview 540x960, max_spp 256, seed 1, "
+                                  "PathSettings::default(); a generation = one change stepped to its final frame",
+                      "width": w, "height": h, "max_spp": max_spp},
+           "first_full_ms": med("first_full_ms"), "final_ms": med("final_ms"), "frames": med("frames"),
+           "passes": med("passes"), "final_spp": med("spp"), "settled": [g["settled"] for g in gens],
+           "ms_per_spp": med("ms_per_spp"),
+           "timing": "wall clock from ag_live_set to the end of the step that published the frame, host bytes in hand"}
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        osc = O.OracleScene(desc)
+        hw, hh = w // 2, h // 2
+        sums = np.zeros((3, hh, hw, 3), np.float32)
+        t0 = time.perf_counter()
+        osc.render_pass(cam, hw, hh, sums, 0, workloads.path_settings(32), 2, 1, threads=threads)
+        t1 = time.perf_counter()
+        k = np.float32(1.0 / 2.0)
+        O.develop(O.atrous(sums[0] * k, sums[1] * k, sums[2] * k), "agx_punchy")
+        t2 = time.perf_counter()
+        res["cpu_baseline"] = {"value": round((t2 - t0) * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+                               "sample": f"the first look once: render {hw}x{hh} at 2 spp on {threads} threads "
+                                         f"({(t1 - t0) * 1e3:.0f} ms), à-trous + develop on one "
+                                         f"({(t2 - t1) * 1e3:.0f} ms)", "cpu_model": _cpu_model()}
+        res["cpu_over_gpu"] = round(res["cpu_baseline"]["value"] / res["value"], 1)
+    lv.close()
+    sc.close()
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -458,6 +543,7 @@ def main():
     ap.add_argument("--ao", action="store_true", help="cavity ambient occlusion of the signet (scene prep) instead")
     ap.add_argument("--develop", action="store_true", help="preview development (à-trous + tone map) instead")
     ap.add_argument("--ingest", action="store_true", help="binary STL ingest (parse + weld) of the signet instead")
+    ap.add_argument("--live", action="store_true", help="the live view's first look / refinement loop instead")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -486,6 +572,9 @@ def main():
         return
     if args.ingest:
         mesh_ingest(args)
+        return
+    if args.live:
+        live_view(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "

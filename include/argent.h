@@ -401,6 +401,61 @@ int ag_mesh_weld(ag_mesh* m, float rel_tolerance, ag_weld_report* report);
 /* compact (weld.rs:66-91): drops collapsed triangles and unreferenced vertices. */
 int ag_mesh_compact(ag_mesh* m, uint64_t* dropped_triangles);
 
+/* ------------------------------------------------------------ live view
+ * The live view's render worker (argent serve live.rs:637-830) on the device:
+ * after a change, a half-resolution first look at 2 spp; then refinement
+ * passes sized toward 70 ms (1 spp while the set is "in motion", younger than
+ * 350 ms); frames published at most every 83 ms in motion, else every
+ * 166 ms x sqrt(spp / 32) up to 1.5 s; each frame denoised (à-trous, the
+ * film's albedo and normal guides) and developed to dithered 8-bit on the
+ * device; refinement stops at max_spp, or once a denoised frame differs from
+ * the previous full-resolution one by less than 0.5 % (mean |dY| / mean Y
+ * over every 4th pixel) from 64 spp on.  The caller's thread drives the loop
+ * (one ag_live_step per worker iteration) and passes its clock in
+ * milliseconds; the session adds the measured time of its own passes. */
+typedef struct ag_live ag_live;
+
+/* The Params fields the worker reads (params.rs): preview size (540 x 960
+ * for the default portrait format), max_spp (default 256), denoise, the grade
+ * of the look, and the path settings (PathSettings::default(): 32, 24, 6). */
+typedef struct ag_live_params {
+    uint32_t width, height;
+    uint32_t max_spp;
+    int32_t denoise;
+    ag_grade grade;
+    ag_path_settings ps;
+} ag_live_params;
+
+/* What one step did. */
+typedef struct ag_live_frame {
+    int32_t published;    /* a frame was developed into out */
+    int32_t idle;         /* nothing to do until the next change */
+    uint64_t generation;  /* of the frame (published) or of the session */
+    uint32_t width, height, spp, max_spp;
+    int32_t half;         /* the half-resolution first look */
+    int32_t converged;    /* final: max_spp reached, or settled */
+    int32_t settled;      /* final before max_spp by the settle test */
+    int32_t denoised;
+    int32_t redeveloped;  /* a look-only change re-developed the last frame */
+    uint32_t pass_spp;    /* samples this step rendered (0: none) */
+    float pass_ms, ms_per_spp;
+    float relative_change; /* the settle test's value, -1 when not evaluated */
+} ag_live_frame;
+
+int ag_live_create(ag_ctx* ctx, ag_live** out);
+void ag_live_destroy(ag_live* live);
+/* A change of scene, camera or params: a new generation (accumulation restarts). */
+int ag_live_set(ag_live* live, const ag_scene* scene, const ag_camera* cam, const ag_live_params* params,
+                double now_ms);
+/* A look-only change (grade, denoise): the last frame is re-developed. */
+int ag_live_set_look(ag_live* live, const ag_grade* grade, int32_t denoise, double now_ms);
+/* One worker iteration.  out: width * height * 3 bytes of the params' size. */
+int ag_live_step(ag_live* live, double now_ms, ag_live_frame* frame, uint8_t* out);
+/* relative_change (live.rs:208-218) of two linear images (host, w*h*3 floats),
+ * computed on the device: sum |lum a - lum b| / sum |lum b| over every 4th
+ * pixel in f64, 0 when the base is below 1e-9. */
+int ag_relative_change(ag_ctx* ctx, const float* a, const float* b, uint32_t w, uint32_t h, double* out);
+
 /* Material::bsdf + Bsdf::{eval,pdf,sample} (material.rs:96-138, bsdf.rs:53-239)
  * evaluated on the device for n given inputs, for checking the shading code:
  * material index, 4 wavelengths, cavity mask, u = [layer, u0, u1, u2], wo and

@@ -66,6 +66,7 @@ def lib():
             ("or_env_sample", C.c_int, [_f, _f, C.c_int32, C.c_int32, C.c_float, _f, C.c_uint32, _f, _f]),
             ("or_env_pdf", C.c_int, [_f, _f, C.c_int32, C.c_int32, C.c_float, _f, C.c_uint32, _f]),
             ("or_stl_count", C.c_int64, [C.c_char_p, C.c_size_t]),
+            ("or_relative_change", C.c_double, [_f, _f, C.c_uint32]),
             ("or_weld", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f, _u32, _u32, _u32, _u64, _f]),
             ("or_compact", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, _u32, _u32, _u32, _u64]),
             ("or_load_stl", C.c_int, [C.c_char_p, C.c_size_t, C.c_int32, C.c_float, C.c_float, _f, _u32, _u32,
@@ -360,3 +361,10 @@ def load_stl(data, weld=True, weld_tolerance=1e-6, unit_scale=1.0):
     if weld:
         r["weld"]["tolerance_mm"] = tol.value
     return op[:nv.value].copy(), ot[:nt.value].copy(), r
+
+
+def relative_change(a, b):
+    """live.rs:208-218."""
+    a = np.ascontiguousarray(np.asarray(a, np.float32))
+    b = np.ascontiguousarray(np.asarray(b, np.float32))
+    return lib().or_relative_change(_p(a), _p(b), a.size // 3)

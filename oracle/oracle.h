@@ -92,6 +92,7 @@ int or_develop_encoded(const float* rgb, uint32_t n, float ev, int32_t op, float
 float or_srgb_encode(float x);
 int or_develop(const float* rgb, uint32_t w, uint32_t h, float ev, int32_t op, float sat, uint8_t* out);
 int or_atrous(uint32_t w, uint32_t h, const float* color, const float* albedo, const float* normal, float* out);
+double or_relative_change(const float* a, const float* b, uint32_t n);
 
 // Mesh ingest (mesh.cpp): argent-mesh io/stl.rs, io.rs, weld.rs.
 // rep (weld): input vertices, output vertices, exact merges, tolerance merges,

@@ -426,3 +426,15 @@ extern "C" int or_atrous(uint32_t w, uint32_t h, const float* color, const float
     orp::atrous((int)w, (int)h, color, albedo, normal, out);
     return 0;
 }
+
+// live.rs:208-218 (relative_change): every 4th pixel, f64 luma sums in pixel order.
+extern "C" double or_relative_change(const float* a, const float* b, uint32_t n) {
+    double diff = 0.0, base = 0.0;
+    for (uint32_t i = 0; i < n; i += 4) {
+        const double la = 0.2126 * (double)a[3 * i] + 0.7152 * (double)a[3 * i + 1] + 0.0722 * (double)a[3 * i + 2];
+        const double lb = 0.2126 * (double)b[3 * i] + 0.7152 * (double)b[3 * i + 1] + 0.0722 * (double)b[3 * i + 2];
+        diff += std::fabs(la - lb);
+        base += std::fabs(lb);
+    }
+    return base <= 1e-9 ? 0.0 : diff / base;
+}
