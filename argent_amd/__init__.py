@@ -268,16 +268,19 @@ class SceneDesc:
 class Scene:
     """Device scene (ag_scene): BVH, mesh attributes, tables resident in HBM."""
 
-    def __init__(self, ctx, desc, share_mesh_with=None):
+    def __init__(self, ctx, desc, share_mesh_with=None, gpu_bvh=False):
         """With `share_mesh_with` (a Scene of the same mesh), only the look --
         materials, panels, environment, floor -- comes from `desc`; the mesh
-        and BVH on the device are shared (ag_scene_create_shared)."""
+        and BVH on the device are shared (ag_scene_create_shared).  With
+        `gpu_bvh` the BVH is built on the device (ag_scene_create_ex)."""
         self.ctx = ctx
         self.desc = desc
         self.base = share_mesh_with
         h = C.c_void_p()
         desc._bind()
-        if share_mesh_with is None:
+        if share_mesh_with is None and gpu_bvh:
+            check(A.lib().ag_scene_create_ex(ctx.h, C.byref(desc.d), 1, C.byref(h)))
+        elif share_mesh_with is None:
             check(A.lib().ag_scene_create(ctx.h, C.byref(desc.d), C.byref(h)))
         else:
             check(A.lib().ag_scene_create_shared(ctx.h, share_mesh_with.h, C.byref(desc.d), C.byref(h)))

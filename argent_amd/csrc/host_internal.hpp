@@ -67,4 +67,14 @@ void quantize_axis(float p, float s, float lo, float hi, uint8_t* qlo, uint8_t* 
 
 void build_wide_bvh(const float* positions, const uint32_t* triangles, uint32_t n_tris, WideBvh* out);
 
+// The same tree format built on the device (bvh.hip) from device positions
+// and triangle indices: nodes needs room for n_tris + 1 nodes, tris for
+// n_tris triangles; temporaries live in the context workspace.
+struct GpuBvhInfo {
+    uint32_t n_nodes = 0, n_leaves = 0, depth = 0, max_stack = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+int gpu_build_wide_bvh(ag_ctx* ctx, const float* positions, const uint32_t* triangles, uint32_t n_tris, float* nodes,
+                       float* tris, GpuBvhInfo* info);
+
 }  // namespace ag

@@ -914,6 +914,13 @@ constexpr uint32_t MAX_VERTICES = 0x7fffffffu;
 }  // namespace
 }  // namespace ag
 
+namespace ag {
+void device_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* work, uint32_t* total) {
+    scan(st, in, out, n, work, total);
+}
+size_t device_scan_work_words(size_t n) { return scan_work_words(n); }
+}  // namespace ag
+
 using namespace ag;
 
 extern "C" int ag_mesh_create(ag_ctx* ctx, const float* positions, uint32_t n_vertices, const uint32_t* triangles,

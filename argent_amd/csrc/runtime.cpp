@@ -416,11 +416,44 @@ static int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
     return AG_OK;
 }
 
-static int build_geometry(ag_ctx* ctx, const ag_scene_desc* d, std::shared_ptr<Geometry>* out) {
+static int build_geometry(ag_ctx* ctx, const ag_scene_desc* d, std::shared_ptr<Geometry>* out, bool gpu_bvh = false) {
     hipStream_t st = ctx->stream;
     auto g = std::make_shared<Geometry>();
     WideBvh bvh;
-    build_wide_bvh(d->positions, d->triangles, d->n_triangles, &bvh);
+    if (!gpu_bvh) {
+        build_wide_bvh(d->positions, d->triangles, d->n_triangles, &bvh);
+    } else {
+        // upload the mesh, then build the same node format on the device
+        size_t nv = std::max<uint32_t>(d->n_vertices, 1), nt = std::max<uint32_t>(d->n_triangles, 1);
+        int rc;
+        if ((rc = g->pos.ensure(3 * nv * 4)) || (rc = g->idx.ensure(3 * nt * 4)) ||
+            (rc = g->nodes.ensure(((size_t)d->n_triangles + 1) * BVH_NODE_FLOATS * 4)) ||
+            (rc = g->tris.ensure(nt * 48)))
+            return rc;
+        if (d->n_vertices) HIPCHK(hipMemcpyAsync(g->pos.p, d->positions, 12 * (size_t)d->n_vertices, hipMemcpyHostToDevice, st));
+        if (d->n_triangles)
+            HIPCHK(hipMemcpyAsync(g->idx.p, d->triangles, 12 * (size_t)d->n_triangles, hipMemcpyHostToDevice, st));
+        GpuBvhInfo info;
+        if ((rc = gpu_build_wide_bvh(ctx, g->pos.as<float>(), g->idx.as<uint32_t>(), d->n_triangles,
+                                     g->nodes.as<float>(), g->tris.as<float>(), &info)))
+            return rc;
+        // the node buffer was sized for the worst case (n_tris + 1 nodes): keep what the tree uses
+        DevBuf exact;
+        if ((rc = exact.ensure((size_t)info.n_nodes * BVH_NODE_FLOATS * 4))) return rc;
+        HIPCHK(hipMemcpyAsync(exact.p, g->nodes.p, (size_t)info.n_nodes * BVH_NODE_FLOATS * 4, hipMemcpyDeviceToDevice,
+                              st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::swap(g->nodes.p, exact.p);
+        std::swap(g->nodes.bytes, exact.bytes);
+        bvh.n_nodes = info.n_nodes;
+        bvh.n_leaves = info.n_leaves;
+        bvh.depth = info.depth;
+        bvh.max_stack = info.max_stack;
+        for (int a = 0; a < 3; a++) {
+            bvh.lo[a] = info.lo[a];
+            bvh.hi[a] = info.hi[a];
+        }
+    }
     if (bvh.max_stack + 1 > (uint32_t)(STACK_LDS + STACK_SPILL))
         return fail(AG_E_INVALID, "BVH too deep for the traversal stack (" + std::to_string(bvh.max_stack) +
                                       " entries needed)");
@@ -441,16 +474,18 @@ static int build_geometry(ag_ctx* ctx, const ag_scene_desc* d, std::shared_ptr<G
         }
     }
     int rc;
-    if ((rc = upload(g->nodes, bvh.nodes.data(), bvh.nodes.size() * 4, st))) return rc;
-    std::vector<float> dummy(12, 0.0f);
-    if ((rc = upload(g->tris, bvh.tris.empty() ? dummy.data() : bvh.tris.data(),
-                     std::max<size_t>(bvh.tris.size(), 12) * 4, st)))
-        return rc;
     size_t nv = std::max<uint32_t>(d->n_vertices, 1), nt = std::max<uint32_t>(d->n_triangles, 1);
-    std::vector<float> zeros3(3 * nv, 0.0f);
-    std::vector<uint32_t> zidx(3 * nt, 0);
-    if ((rc = upload(g->pos, d->n_vertices ? d->positions : zeros3.data(), 3 * nv * 4, st))) return rc;
-    if ((rc = upload(g->idx, d->n_triangles ? d->triangles : zidx.data(), 3 * nt * 4, st))) return rc;
+    if (!gpu_bvh) {
+        if ((rc = upload(g->nodes, bvh.nodes.data(), bvh.nodes.size() * 4, st))) return rc;
+        std::vector<float> dummy(12, 0.0f);
+        if ((rc = upload(g->tris, bvh.tris.empty() ? dummy.data() : bvh.tris.data(),
+                         std::max<size_t>(bvh.tris.size(), 12) * 4, st)))
+            return rc;
+        std::vector<float> zeros3(3 * nv, 0.0f);
+        std::vector<uint32_t> zidx(3 * nt, 0);
+        if ((rc = upload(g->pos, d->n_vertices ? d->positions : zeros3.data(), 3 * nv * 4, st))) return rc;
+        if ((rc = upload(g->idx, d->n_triangles ? d->triangles : zidx.data(), 3 * nt * 4, st))) return rc;
+    }
     std::vector<uint16_t> tm(nt, 0);
     if (d->tri_material && d->n_triangles) std::memcpy(tm.data(), d->tri_material, d->n_triangles * 2);
     for (uint16_t m : tm) g->max_material = std::max(g->max_material, m);
@@ -588,6 +623,17 @@ extern "C" int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* d, ag_scene** o
     HIPCHK(hipSetDevice(ctx->device));
     std::shared_ptr<Geometry> g;
     if ((rc = build_geometry(ctx, d, &g))) return rc;
+    return scene_over(ctx, std::move(g), d, out);
+}
+
+extern "C" int ag_scene_create_ex(ag_ctx* ctx, const ag_scene_desc* d, uint32_t flags, ag_scene** out) {
+    if (!ctx || !out) return fail(AG_E_INVALID, "null argument");
+    if (flags & ~(uint32_t)AG_SCENE_GPU_BVH) return fail(AG_E_INVALID, "unknown scene flags");
+    int rc = validate_desc(d);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    std::shared_ptr<Geometry> g;
+    if ((rc = build_geometry(ctx, d, &g, (flags & AG_SCENE_GPU_BVH) != 0))) return rc;
     return scene_over(ctx, std::move(g), d, out);
 }
 

@@ -527,6 +527,56 @@ def live_view(args):
     print(json.dumps(res), flush=True)
 
 
+def bvh_build(args):
+    """Scene preparation's BVH for the 1.2 M-triangle signet: the host binned
+    SAH (ag_scene_create) against the device build (ag_scene_create_ex with
+    AG_SCENE_GPU_BVH), each timed as the whole scene creation from host
+    arrays (uploads and tables included), and the 2^24-ray closest-hit batch
+    on each tree.  The reference builds its 4-wide binned SAH on the host
+    (bvh.rs:117-146) inside a ~3 s prepare (API.md:19)."""
+    import argent_amd as ag
+    from argent_amd import synth
+    pos, tris, nrm = synth.signet(seed=2)
+    desc = ag.SceneDesc(pos, tris, normals=nrm)
+    desc.add_material(ag.Material.metal("sterling"))
+    ctx = ag.Context(0)
+    c = (pos.min(0) + pos.max(0)) * 0.5
+    o, d = synth.sphere_rays(1 << 24, c, seed=3)
+    rays = ag.DeviceRays(ctx, o, d)
+    res = {}
+    for name, gpu in (("host_sah", False), ("gpu", True)):
+        reps = max(args.steps, 1) if gpu else 1
+        for _ in range(args.warmup if gpu else 0):
+            ag.Scene(ctx, desc, gpu_bvh=gpu).close()
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            sc = ag.Scene(ctx, desc, gpu_bvh=gpu)
+            times.append((time.perf_counter() - t0) * 1e3)
+            st = sc.stats()
+            if len(times) < reps:
+                sc.close()
+        ms = min(ag.intersect_device(ctx, sc, rays) for _ in range(3))
+        res[name] = {"scene_ms": round(float(np.median(times)), 1), "nodes": st["nodes"], "depth": st["depth"],
+                     "bytes": st["bytes"], "rays_grays_per_s": round((1 << 24) / (ms * 1e-3) / 1e9, 2)}
+        sc.close()
+    rays.close()
+    ctx.close()
+    g, h = res["gpu"], res["host_sah"]
+    out = {"metric": "scene creation with its BVH, 1.2 M-triangle signet, 1 GPU", "value": g["scene_ms"],
+           "unit": "ms", "n_gpus": 1, "steps": max(args.steps, 1), "warmup": args.warmup,
+           "ms_per_step": g["scene_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "signet ring (1,198,332 triangles), positions/triangles/normals from host arrays; "
+                                  "rays: 2^24 closest-hit batch", "triangles": int(len(tris))},
+           "gpu_bvh": g, "host_sah": h, "build_speedup": round(h["scene_ms"] / g["scene_ms"], 1),
+           "traversal_ratio": round(g["rays_grays_per_s"] / h["rays_grays_per_s"], 3),
+           "cpu_baseline": {"value": h["scene_ms"], "unit": "ms", "cores": min(16, os.cpu_count() or 1),
+                            "kind": "port", "sample": "the host binned-SAH build of this scene once",
+                            "cpu_model": _cpu_model()}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -544,6 +594,7 @@ def main():
     ap.add_argument("--develop", action="store_true", help="preview development (à-trous + tone map) instead")
     ap.add_argument("--ingest", action="store_true", help="binary STL ingest (parse + weld) of the signet instead")
     ap.add_argument("--live", action="store_true", help="the live view's first look / refinement loop instead")
+    ap.add_argument("--bvh", action="store_true", help="scene creation with the device BVH build vs the host SAH")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -575,6 +626,9 @@ def main():
         return
     if args.live:
         live_view(args)
+        return
+    if args.bvh:
+        bvh_build(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "

@@ -237,6 +237,14 @@ int ag_desc_light_cdf(const ag_scene_desc* d, int32_t* n_lights, int32_t lights[
 /* Scene::new + Piece::new (scene.rs:21-32, :82-86): copies the arrays to the
  * device and builds the BVH.  The scene owns its device memory. */
 int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* desc, ag_scene** out);
+/* Scene::new with a choice of BVH builder.  flags: AG_SCENE_GPU_BVH builds the
+ * tree on the device (Morton-ordered radix tree collapsed to the same 8-wide
+ * format: about two orders of magnitude faster to build than the host's
+ * binned SAH, somewhat slower to traverse), for a new mesh whose first
+ * picture matters more than the render's last percent.  0 = ag_scene_create.
+ * Closest-hit results are the same with either tree. */
+enum { AG_SCENE_GPU_BVH = 1 };
+int ag_scene_create_ex(ag_ctx* ctx, const ag_scene_desc* desc, uint32_t flags, ag_scene** out);
 /* Scene::new(Arc<Piece>, ...) over the piece of `base` (scene.rs:48, :82-86;
  * jobs.rs:690-696 and shot.rs:206-221 clone the Arc per look): materials,
  * panels, environment, floor and white come from `desc`, the mesh and BVH are
