@@ -502,6 +502,18 @@ class Mesh:
         check(A.lib().ag_mesh_weld(self.h, rel_tolerance, C.byref(r)))
         return _weld_dict(r)
 
+    def normals(self, crease_angle_deg=50.0):
+        """normals::compute (normals.rs:47-49) on the GPU; returns the NormalReport as a dict."""
+        r = A.NormalReport()
+        check(A.lib().ag_mesh_normals(self.h, crease_angle_deg, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in A.NormalReport._fields_}
+
+    def read_normals(self):
+        nv, _ = self.counts
+        n = np.empty((nv, 3), np.float32)
+        check(A.lib().ag_mesh_read_normals(self.h, _ptr(n)))
+        return n
+
     def compact(self):
         """weld.rs:66-91; returns the number of triangles dropped."""
         d = C.c_uint64()

@@ -104,6 +104,13 @@ class WeldReport(C.Structure):
                 ("tolerance_merges", C.c_uint64), ("dropped_triangles", C.c_uint64), ("tolerance_mm", C.c_float)]
 
 
+class NormalReport(C.Structure):
+    """ag_normal_report (normals.rs:23-42)."""
+    _fields_ = [(k, C.c_uint64) for k in ("shells", "flipped_shells", "cavity_shells", "repaired_faces",
+                                           "boundary_edges", "nonmanifold_edges", "inconsistent_edges",
+                                           "degenerate_faces", "split_vertices")]
+
+
 class LoadReport(C.Structure):
     """ag_load_report (io.rs:21-40, binary STL)."""
     _fields_ = [("file_bytes", C.c_uint64), ("raw_triangles", C.c_uint64), ("raw_vertices", C.c_uint64),
@@ -249,6 +256,8 @@ _SIGS = [
     ("ag_mesh_device_arrays", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("ag_mesh_weld", C.c_int, [C.c_void_p, C.c_float, C.POINTER(WeldReport)]),
     ("ag_mesh_compact", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("ag_mesh_normals", C.c_int, [C.c_void_p, C.c_float, C.POINTER(NormalReport)]),
+    ("ag_mesh_read_normals", C.c_int, [C.c_void_p, f32p]),
     ("ag_live_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("ag_live_destroy", None, [C.c_void_p]),
     ("ag_live_set", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(LiveParams), C.c_double]),

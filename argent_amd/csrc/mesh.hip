@@ -34,24 +34,14 @@
 #include <string>
 #include <vector>
 
-#include "host_internal.hpp"
+#include "mesh_internal.hpp"
 
 namespace ag {
 namespace {
 
-constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr unsigned long long CELL_EMPTY = ~0ull;
-constexpr int BLK = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = BLK * SCAN_ITEMS;
-
-#define MCHK(expr)                                                                                    \
-    do {                                                                                              \
-        hipError_t e_ = (expr);                                                                       \
-        if (e_ != hipSuccess) return fail(AG_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-unsigned grid_of(size_t n) { return (unsigned)std::max<size_t>(1, (n + BLK - 1) / BLK); }
 
 // ------------------------------------------------------------------ scan
 __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
@@ -407,44 +397,6 @@ __global__ __launch_bounds__(BLK) void k_cell_scatter(const uint32_t* slot_of, u
     members[start[s] + atomicSub(&cnt[s], 1u) - 1u] = u;
 }
 
-// Union-find over the classes.  parent[x] <= x always holds (a node only
-// ever moves under a smaller one), and only roots are swapped, by atomicCAS.
-// Loads are plain (workgroup scope, may hit a stale L1 line): a stale parent
-// is still an ancestor, and a stale "is a root" answer only makes the CAS
-// fail and the loop retry with the fresh value the CAS returns.
-__device__ inline uint32_t uf_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Root of x with path halving.
-__device__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
-    for (;;) {
-        const uint32_t p = uf_load(&parent[x]);
-        if (p == x) return x;
-        const uint32_t gp = uf_load(&parent[p]);
-        if (gp == p) return p;
-        __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = gp;
-    }
-}
-
-// Unites the sets of a and b; returns the root of the union as seen here.
-__device__ uint32_t uf_unite(uint32_t* parent, uint32_t a, uint32_t b) {
-    for (;;) {
-        a = uf_find(parent, a);
-        b = uf_find(parent, b);
-        if (a == b) return a;
-        if (a > b) {
-            const uint32_t t = a;
-            a = b;
-            b = t;
-        }
-        const uint32_t old = atomicCAS(&parent[b], b, a);  // hook the larger root under the smaller
-        if (old == b) return a;
-        b = old;
-    }
-}
-
 __global__ __launch_bounds__(BLK) void k_iota(uint32_t* a, uint32_t n) {
     const uint32_t i = blockIdx.x * BLK + threadIdx.x;
     if (i < n) a[i] = i;
@@ -603,57 +555,6 @@ __global__ __launch_bounds__(BLK) void k_gather_used(const float* pos, const uin
 }
 
 // ------------------------------------------------------------------ host side
-// Device buffer owned by a mesh.
-struct Buf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    Buf() = default;
-    Buf(const Buf&) = delete;
-    Buf& operator=(const Buf&) = delete;
-    ~Buf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    int alloc(size_t n) {
-        release();
-        if (n == 0) n = 4;
-        if (hipMalloc(&p, n) != hipSuccess) {
-            p = nullptr;
-            return fail(AG_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed");
-        }
-        bytes = n;
-        return AG_OK;
-    }
-    template <class T>
-    T* as() const {
-        return reinterpret_cast<T*>(p);
-    }
-};
-
-// 256-byte-aligned pieces of the context's mesh workspace (grow-only, kept
-// by the context between calls, so a load allocates only its outputs).
-struct Arena {
-    char* base = nullptr;
-    size_t used = 0;
-    size_t add(size_t bytes) {
-        const size_t o = used;
-        used += (bytes + 255) & ~(size_t)255;
-        return o;
-    }
-    template <class T>
-    T* at(size_t off) const {
-        return reinterpret_cast<T*>(base + off);
-    }
-    int bind(ag_ctx* ctx) {
-        void* p = nullptr;
-        const int rc = ctx_workspace(ctx, used, &p);
-        base = static_cast<char*>(p);
-        return rc;
-    }
-};
-
 uint32_t pow2_at_least(size_t n) {
     uint32_t s = 1024;
     while (s < n) s <<= 1;
@@ -683,12 +584,6 @@ struct Timer {
 
 }  // namespace
 }  // namespace ag
-
-struct ag_mesh {
-    ag_ctx* ctx = nullptr;
-    ag::Buf pos, tris;  // capacity >= nv * 3 floats, nt * 3 words
-    uint32_t nv = 0, nt = 0;
-};
 
 namespace ag {
 namespace {
@@ -919,6 +814,16 @@ void device_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t
     scan(st, in, out, n, work, total);
 }
 size_t device_scan_work_words(size_t n) { return scan_work_words(n); }
+
+int mesh_drop_collapsed(ag_mesh* m, uint32_t* dropped) {
+    *dropped = 0;
+    if (m->nt == 0) return AG_OK;
+    Arena a;
+    const TriWork tw = plan_tris(a, m->nt);
+    int rc = a.bind(m->ctx);
+    if (rc) return rc;
+    return remap_drop_collapsed(ctx_stream(m->ctx), m, a, tw, nullptr, dropped);
+}
 }  // namespace ag
 
 using namespace ag;
@@ -980,6 +885,7 @@ extern "C" int ag_mesh_device_arrays(const ag_mesh* m, const float** positions, 
 extern "C" int ag_mesh_weld(ag_mesh* m, float rel_tolerance, ag_weld_report* report) {
     if (!m) return fail(AG_E_INVALID, "null mesh");
     MCHK(hipSetDevice(ctx_device(m->ctx)));
+    m->has_nrm = false;  // weld.rs:56: welding clears the normals
     if (m->nt == 0 || m->nv == 0) {
         if (report) {
             *report = ag_weld_report{};
@@ -995,6 +901,7 @@ extern "C" int ag_mesh_weld(ag_mesh* m, float rel_tolerance, ag_weld_report* rep
 extern "C" int ag_mesh_compact(ag_mesh* m, uint64_t* dropped_triangles) {
     if (!m) return fail(AG_E_INVALID, "null mesh");
     MCHK(hipSetDevice(ctx_device(m->ctx)));
+    m->has_nrm = false;
     if (m->nt == 0) {
         if (dropped_triangles) *dropped_triangles = 0;
         m->nv = 0;

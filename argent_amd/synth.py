@@ -62,6 +62,71 @@ def cuboid(size):
     return pos, tris
 
 
+def icosphere(r, subdiv):
+    """synth.rs:81-144: icosahedron, `subdiv` midpoint subdivisions (shared
+    midpoints), vertices pushed to radius r (20 * 4^subdiv triangles)."""
+    t = np.float32((1.0 + np.sqrt(np.float32(5.0))) / 2.0)
+    p = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    p = [np.array(q, np.float32) for q in p]
+    tris = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+            [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+            [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    for _ in range(subdiv):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p.append(((p[a] + p[b]) * np.float32(0.5)).astype(np.float32))
+                mid[key] = len(p) - 1
+            return mid[key]
+
+        nxt = []
+        for a, b, c in tris:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nxt += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        tris = nxt
+    P = np.array(p, np.float32)
+    ln = np.sqrt(P[:, 0] * P[:, 0] + P[:, 1] * P[:, 1] + P[:, 2] * P[:, 2]).astype(np.float32)
+    P = (P / ln[:, None] * np.float32(r)).astype(np.float32)
+    return P, np.array(tris, np.uint32)
+
+
+def subdivided_cuboid(size, n):
+    """synth.rs:39-78: each face an n x n grid of quads, outward wound, shared
+    corner and edge vertices merged (they coincide bit for bit)."""
+    n = max(int(n), 1)
+    h = [np.float32(s) * np.float32(0.5) for s in size]
+    size = [np.float32(s) for s in size]
+    pos, tris = [], []
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            a, b = (axis + 1) % 3, (axis + 2) % 3
+            base = len(pos)
+            for i in range(n + 1):
+                for j in range(n + 1):
+                    q = [np.float32(0)] * 3
+                    q[axis] = np.float32(sign) * h[axis]
+                    q[a] = np.float32(-h[a] + np.float32(size[a] * np.float32(i) / np.float32(n)))
+                    q[b] = np.float32(-h[b] + np.float32(size[b] * np.float32(j) / np.float32(n)))
+                    pos.append(q)
+            idx = lambda i, j: base + i * (n + 1) + j
+            for i in range(n):
+                for j in range(n):
+                    if sign > 0:
+                        tris += [[idx(i, j), idx(i + 1, j), idx(i + 1, j + 1)], [idx(i, j), idx(i + 1, j + 1), idx(i, j + 1)]]
+                    else:
+                        tris += [[idx(i, j), idx(i + 1, j + 1), idx(i + 1, j)], [idx(i, j), idx(i, j + 1), idx(i + 1, j + 1)]]
+    P = np.array(pos, np.float32)
+    key = (P + np.float32(0.0)).view(np.uint32)
+    _, first, inv = np.unique(key.view([("", np.uint32)] * 3).ravel(), return_index=True, return_inverse=True)
+    order = np.argsort(first)
+    new_id = np.empty(len(first), np.int64)
+    new_id[order] = np.arange(len(first))
+    return P[np.sort(first)], new_id[inv.ravel()][np.array(tris)].astype(np.uint32)
+
+
 def unweld(pos, tris):
     """synth.rs:257-268: every triangle gets its own three vertices."""
     p = np.asarray(pos, np.float32)[np.asarray(tris).reshape(-1)]

@@ -381,7 +381,8 @@ def preview_develop(args):
 
 def mesh_ingest(args):
     """Mesh ingest of a binary STL (io.rs:87-126: parse io/stl.rs:34-62, drop
-    invalid triangles, weld weld.rs:38-62) for the 1.2 M-triangle signet
+    invalid triangles, weld weld.rs:38-62, crease normals normals.rs:47-220
+    at 50 degrees) for the 1.2 M-triangle signet
     written as an STL triangle soup (3.6 M vertices, 59.9 MB), the file's
     bytes already in HBM; the oracle (one thread) loads the same bytes on the
     host.  A step is one ag_mesh_load_stl call, allocations and the few
@@ -398,10 +399,13 @@ def mesh_ingest(args):
         m.close()
     steps = max(args.steps, 1)
     ctx.synchronize()
-    parse_ms, weld_ms = [], []
+    parse_ms, weld_ms, normals_ms = [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         m, r = ag.load_stl(ctx, len(data), device_ptr=buf.ptr)
+        t1 = time.perf_counter()
+        nr = m.normals(50.0)
+        normals_ms.append((time.perf_counter() - t1) * 1e3)
         parse_ms.append(r["parse_ms"])
         weld_ms.append(r["weld_ms"])
         nv_out = m.counts[0]
@@ -410,18 +414,23 @@ def mesh_ingest(args):
     dt = (time.perf_counter() - t0) / steps
     w = r["weld"]
     # algorithmic bytes of the whole chain: the file read once (50 B per
-    # record), the welded mesh written once (12 B per vertex and triangle)
-    alg = len(data) + 12 * nv_out + 12 * (nt - w["dropped_triangles"])
-    res = {"metric": "binary STL ingest (parse + weld) throughput, 1 GPU", "value": round(nt / dt / 1e6, 2),
+    # record), the welded mesh written once (12 B per vertex and triangle,
+    # 12 B per vertex normal)
+    alg = len(data) + 24 * nv_out + 12 * (nt - w["dropped_triangles"])
+    res = {"metric": "binary STL ingest (parse + weld + crease normals) throughput, 1 GPU",
+           "value": round(nt / dt / 1e6, 2),
            "unit": "Mtriangles/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32/u32", "data": "synthetic",
            "config": {"workload": "signet ring (1,198,332 triangles) as a binary STL soup, 59.9 MB in HBM; "
-                                  "weld tolerance 1e-6 x bbox diagonal, unit scale 1",
+                                  "weld tolerance 1e-6 x bbox diagonal, unit scale 1, crease angle 50 degrees",
                       "triangles": nt, "stl_bytes": len(data)},
            "report": {"output_vertices": w["output_vertices"], "exact_merges": w["exact_merges"],
-                      "tolerance_merges": w["tolerance_merges"], "dropped_triangles": w["dropped_triangles"]},
-           "stage_ms": {"parse": round(sorted(parse_ms)[steps // 2], 3), "weld": round(sorted(weld_ms)[steps // 2], 3)},
+                      "tolerance_merges": w["tolerance_merges"], "dropped_triangles": w["dropped_triangles"],
+                      "output_vertices_after_normals": nv_out},
+           "stage_ms": {"parse": round(sorted(parse_ms)[steps // 2], 3), "weld": round(sorted(weld_ms)[steps // 2], 3),
+                        "normals": round(sorted(normals_ms)[steps // 2], 3)},
+           "normals_report": nr,
            "hbm_alg_gbps": round(alg / dt / 1e9, 1),
            "timing": "wall clock over the timed steps; stage_ms are device event pairs (median) around each stage, "
                      "host waits for counts included"}
@@ -429,7 +438,8 @@ def mesh_ingest(args):
         import oracle as O
         n, t1 = 0, time.perf_counter()
         while True:
-            O.load_stl(data)
+            wp, wt, _ = O.load_stl(data)
+            O.normals(wp, wt, 50.0)
             n += 1
             el = time.perf_counter() - t1
             if el >= args.cpu_seconds:

@@ -409,6 +409,22 @@ int ag_mesh_weld(ag_mesh* m, float rel_tolerance, ag_weld_report* report);
 /* compact (weld.rs:66-91): drops collapsed triangles and unreferenced vertices. */
 int ag_mesh_compact(ag_mesh* m, uint64_t* dropped_triangles);
 
+/* NormalReport (normals.rs:23-42). */
+typedef struct ag_normal_report {
+    uint64_t shells, flipped_shells, cavity_shells, repaired_faces, boundary_edges, nonmanifold_edges,
+        inconsistent_edges, degenerate_faces, split_vertices;
+} ag_normal_report;
+
+/* normals::compute (normals.rs:47-49): drops collapsed triangles, repairs
+ * inconsistent winding (majority per edge-connected component), flips closed
+ * inside-out shells that no other shell encloses, splits vertices along
+ * creases sharper than crease_angle_deg (0: faceted) and gives every vertex
+ * the angle-weighted normal of its fan.  Split copies are appended after the
+ * original vertices; triangle order is kept.  The mesh gains normals
+ * (ag_mesh_read_normals); a later weld or compact drops them. */
+int ag_mesh_normals(ag_mesh* m, float crease_angle_deg, ag_normal_report* report);
+int ag_mesh_read_normals(const ag_mesh* m, float* normals);
+
 /* ------------------------------------------------------------ live view
  * The live view's render worker (argent serve live.rs:637-830) on the device:
  * after a change, a half-resolution first look at 2 spp; then refinement

@@ -67,6 +67,7 @@ def lib():
             ("or_env_pdf", C.c_int, [_f, _f, C.c_int32, C.c_int32, C.c_float, _f, C.c_uint32, _f]),
             ("or_stl_count", C.c_int64, [C.c_char_p, C.c_size_t]),
             ("or_relative_change", C.c_double, [_f, _f, C.c_uint32]),
+            ("or_normals", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f, _f, _u32, _u32, _u32, _u64]),
             ("or_weld", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f, _u32, _u32, _u32, _u64, _f]),
             ("or_compact", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, _u32, _u32, _u32, _u64]),
             ("or_load_stl", C.c_int, [C.c_char_p, C.c_size_t, C.c_int32, C.c_float, C.c_float, _f, _u32, _u32,
@@ -368,3 +369,23 @@ def relative_change(a, b):
     a = np.ascontiguousarray(np.asarray(a, np.float32))
     b = np.ascontiguousarray(np.asarray(b, np.float32))
     return lib().or_relative_change(_p(a), _p(b), a.size // 3)
+
+
+NORMAL_FIELDS = ("shells", "flipped_shells", "cavity_shells", "repaired_faces", "boundary_edges",
+                 "nonmanifold_edges", "inconsistent_edges", "degenerate_faces", "split_vertices")
+
+
+def normals(pos, tris, crease_angle_deg=50.0):
+    """normals.rs:47-49: (positions, normals, triangles, report dict)."""
+    pos, tris = _mesh_in(pos, tris)
+    cap = len(pos) + 3 * len(tris)
+    op = np.empty((cap, 3), np.float32)
+    on = np.empty((cap, 3), np.float32)
+    ot = np.empty((max(len(tris), 1), 3), np.uint32)
+    nv, nt = C.c_uint32(), C.c_uint32()
+    rep = np.zeros(9, np.uint64)
+    if lib().or_normals(_p(pos), len(pos), _p(tris, C.c_uint32), len(tris), crease_angle_deg, _p(op), _p(on),
+                        C.byref(nv), _p(ot, C.c_uint32), C.byref(nt), _p(rep, C.c_uint64)):
+        raise IndexError("vertex index out of range")
+    return (op[:nv.value].copy(), on[:nv.value].copy(), ot[:nt.value].copy(),
+            dict(zip(NORMAL_FIELDS, (int(x) for x in rep))))

@@ -103,6 +103,11 @@ int or_weld(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, fl
             uint32_t* out_nv, uint32_t* out_tris, uint32_t* out_nt, uint64_t* rep5, float* tol_mm);
 int or_compact(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out_pos, uint32_t* out_nv,
                uint32_t* out_tris, uint32_t* out_nt, uint64_t* dropped);
+// Crease-aware normals (normals.cpp): normals::compute with one crease angle.
+// rep9: shells, flipped shells, cavity shells, repaired faces, boundary,
+// non-manifold and inconsistent edges, degenerate faces, split vertices.
+int or_normals(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, float crease_deg, float* out_pos,
+               float* out_nrm, uint32_t* out_nv, uint32_t* out_tris, uint32_t* out_nt, uint64_t* rep9);
 int or_load_stl(const uint8_t* b, size_t len, int32_t do_weld, float rel_tolerance, float unit_scale, float* out_pos,
                 uint32_t* out_nv, uint32_t* out_tris, uint32_t* out_nt, uint64_t* rep10, float* tol_mm);
 
