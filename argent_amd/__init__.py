@@ -297,6 +297,18 @@ class Scene:
     def set_material(self, m, mat):
         check(A.lib().ag_scene_set_material(self.h, m, C.byref(mat)))
 
+    @classmethod
+    def from_mesh(cls, ctx, mesh, desc, gpu_bvh=True):
+        """Scene over a device Mesh (ag_scene_create_from_mesh): the look comes
+        from `desc`, the geometry stays on the device."""
+        sc = cls.__new__(cls)
+        sc.ctx, sc.desc, sc.base = ctx, desc, None
+        h = C.c_void_p()
+        desc._bind()
+        check(A.lib().ag_scene_create_from_mesh(ctx.h, mesh.h, C.byref(desc.d), 1 if gpu_bvh else 0, C.byref(h)))
+        sc.h = h
+        return sc
+
     def stats(self):
         n, l, dp = C.c_uint32(), C.c_uint32(), C.c_uint32()
         b = C.c_size_t()
@@ -488,12 +500,12 @@ class Mesh:
         check(A.lib().ag_mesh_info(self.h, C.byref(nv), C.byref(nt)))
         return nv.value, nt.value
 
-    def read(self):
-        """(positions (nv, 3) float32, triangles (nt, 3) uint32) in host memory."""
+    def read(self, triangles=True):
+        """(positions (nv, 3) float32, triangles (nt, 3) uint32 or None) in host memory."""
         nv, nt = self.counts
         p = np.empty((nv, 3), np.float32)
-        t = np.empty((nt, 3), np.uint32)
-        check(A.lib().ag_mesh_read(self.h, _ptr(p), _ptr(t, C.c_uint32)))
+        t = np.empty((nt, 3), np.uint32) if triangles else None
+        check(A.lib().ag_mesh_read(self.h, _ptr(p), _ptr(t, C.c_uint32) if triangles else None))
         return p, t
 
     def weld(self, rel_tolerance=1e-6):

@@ -197,6 +197,8 @@ _SIGS = [
     ("ag_camera_frame", C.c_int, [f32p, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                   C.c_float, C.POINTER(Camera)]),
     ("ag_scene_create", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    ("ag_scene_create_from_mesh", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SceneDesc), C.c_uint32,
+                                            C.POINTER(C.c_void_p)]),
     ("ag_scene_create_ex", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_uint32, C.POINTER(C.c_void_p)]),
     ("ag_scene_create_shared", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     ("ag_scene_destroy", None, [C.c_void_p]),

@@ -397,6 +397,11 @@ __global__ __launch_bounds__(BLK) void k_cell_scatter(const uint32_t* slot_of, u
     members[start[s] + atomicSub(&cnt[s], 1u) - 1u] = u;
 }
 
+__global__ __launch_bounds__(BLK) void k_fill_ones(uint32_t* a, uint32_t n) {
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    if (i < n) a[i] = 1u;
+}
+
 __global__ __launch_bounds__(BLK) void k_iota(uint32_t* a, uint32_t n) {
     const uint32_t i = blockIdx.x * BLK + threadIdx.x;
     if (i < n) a[i] = i;
@@ -814,6 +819,43 @@ void device_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t
     scan(st, in, out, n, work, total);
 }
 size_t device_scan_work_words(size_t n) { return scan_work_words(n); }
+
+// Every triangle index below nv?  And the min / max of all vertex positions
+// (argent-mesh lib.rs:122-134, Mesh::bounds: zeros without vertices).
+int mesh_check_bounds(ag_mesh* m, float lo[3], float hi[3]) {
+    hipStream_t st = ctx_stream(m->ctx);
+    const uint32_t nv = m->nv;
+    const unsigned nb = std::min<unsigned>(grid_of(nv), BOUNDS_BLOCKS);
+    Arena a;
+    const size_t o_ones = a.add(4 * (size_t)std::max(nv, 1u)), o_flag = a.add(256), o_part = a.add(28 * (size_t)nb);
+    int rc = a.bind(m->ctx);
+    if (rc) return rc;
+    uint32_t* flag = a.at<uint32_t>(o_flag);
+    if ((rc = mark_used(st, m, a.at<uint32_t>(o_ones), flag))) return rc;  // out-of-range indices -> flag[0]
+    k_fill_ones<<<grid_of(nv), BLK, 0, st>>>(a.at<uint32_t>(o_ones), nv);
+    Init6 init;
+    const float fmax_ = 3.40282347e38f;
+    uint32_t bmax;
+    std::memcpy(&bmax, &fmax_, 4);
+    for (int k = 0; k < 3; k++) {
+        init.v[k] = bmax | 0x80000000u;
+        init.v[3 + k] = ~(bmax | 0x80000000u);
+    }
+    if (nv) {
+        k_bounds<<<nb, BLK, 0, st>>>(m->pos.as<float>(), a.at<uint32_t>(o_ones), nv, a.at<uint32_t>(o_part));
+        k_bounds_final<<<1, BLK, 0, st>>>(a.at<uint32_t>(o_part), nb, init, flag + 1);
+    }
+    MCHK(hipGetLastError());
+    uint32_t hv[7] = {0, 0, 0, 0, 0, 0, 0};
+    MCHK(hipMemcpyAsync(hv, flag, sizeof hv, hipMemcpyDeviceToHost, st));
+    MCHK(hipStreamSynchronize(st));
+    if (hv[0]) return fail(AG_E_INVALID, "triangle vertex index out of range");
+    for (int k = 0; k < 3; k++) {
+        lo[k] = nv ? from_ord(hv[1 + k]) : 0.0f;
+        hi[k] = nv ? from_ord(hv[4 + k]) : 0.0f;
+    }
+    return AG_OK;
+}
 
 int mesh_drop_collapsed(ag_mesh* m, uint32_t* dropped) {
     *dropped = 0;

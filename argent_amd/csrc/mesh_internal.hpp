@@ -132,4 +132,6 @@ namespace ag {
 // Drops triangles with a repeated index, in order (weld.rs:104-112); returns
 // the number dropped through *dropped.  Synchronizes.
 int mesh_drop_collapsed(ag_mesh* m, uint32_t* dropped);
+// Rejects out-of-range triangle indices; lo / hi = bounds of all vertices.
+int mesh_check_bounds(ag_mesh* m, float lo[3], float hi[3]);
 }  // namespace ag

@@ -17,6 +17,7 @@
 #include "host_internal.hpp"
 #include "host_spectra.hpp"
 #include "kernels.hpp"
+#include "mesh_internal.hpp"
 #include "post.hpp"
 #include "tables/spectral_data.h"
 
@@ -634,6 +635,90 @@ extern "C" int ag_scene_create_ex(ag_ctx* ctx, const ag_scene_desc* d, uint32_t 
     HIPCHK(hipSetDevice(ctx->device));
     std::shared_ptr<Geometry> g;
     if ((rc = build_geometry(ctx, d, &g, (flags & AG_SCENE_GPU_BVH) != 0))) return rc;
+    return scene_over(ctx, std::move(g), d, out);
+}
+
+// Scene::new over a device mesh (ag_mesh): its positions, triangles and
+// (when computed) normals are copied on the device and the BVH built there,
+// or on the host after a read-back without AG_SCENE_GPU_BVH.
+extern "C" int ag_scene_create_from_mesh(ag_ctx* ctx, const ag_mesh* mesh, const ag_scene_desc* d, uint32_t flags,
+                                         ag_scene** out) {
+    if (!ctx || !mesh || !d || !out) return fail(AG_E_INVALID, "null argument");
+    if (mesh->ctx != ctx) return fail(AG_E_INVALID, "mesh belongs to another context");
+    if (flags & ~(uint32_t)AG_SCENE_GPU_BVH) return fail(AG_E_INVALID, "unknown scene flags");
+    if (mesh->nt >= (1u << 27)) return fail(AG_E_INVALID, "too many triangles (max 2^27)");
+    ag_scene_desc look = *d;  // the mesh comes from `mesh`; the arrays in `d` are ignored
+    look.positions = look.normals = look.cavity = nullptr;
+    look.triangles = nullptr;
+    look.tri_material = nullptr;
+    look.n_vertices = look.n_triangles = 0;
+    int rc = validate_desc(&look);
+    if (rc) return rc;
+    if (look.n_materials < 1) return fail(AG_E_INVALID, "the mesh needs material slot 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    ag_mesh* m = const_cast<ag_mesh*>(mesh);
+    auto g = std::make_shared<Geometry>();
+    if ((rc = mesh_check_bounds(m, g->vlo, g->vhi))) return rc;
+    const uint32_t nv = mesh->nv, nt = mesh->nt;
+    const size_t nv1 = std::max<uint32_t>(nv, 1), nt1 = std::max<uint32_t>(nt, 1);
+    if ((rc = g->pos.ensure(12 * nv1)) || (rc = g->idx.ensure(12 * nt1)) || (rc = g->tmat.ensure(2 * nt1))) return rc;
+    if (nv) HIPCHK(hipMemcpyAsync(g->pos.p, mesh->pos.p, 12 * (size_t)nv, hipMemcpyDeviceToDevice, st));
+    if (nt) HIPCHK(hipMemcpyAsync(g->idx.p, mesh->tris.p, 12 * (size_t)nt, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemsetAsync(g->tmat.p, 0, 2 * nt1, st));  // one material for the whole mesh (slot 0)
+    if (mesh->has_nrm && nv) {
+        if ((rc = g->nrm.ensure(12 * nv1))) return rc;
+        HIPCHK(hipMemcpyAsync(g->nrm.p, mesh->nrm.p, 12 * (size_t)nv, hipMemcpyDeviceToDevice, st));
+        g->has_nrm = true;
+    }
+    WideBvh bvh;
+    if (flags & AG_SCENE_GPU_BVH) {
+        if ((rc = g->nodes.ensure(((size_t)nt + 1) * BVH_NODE_FLOATS * 4)) || (rc = g->tris.ensure(nt1 * 48))) return rc;
+        GpuBvhInfo info;
+        if ((rc = gpu_build_wide_bvh(ctx, g->pos.as<float>(), g->idx.as<uint32_t>(), nt, g->nodes.as<float>(),
+                                     g->tris.as<float>(), &info)))
+            return rc;
+        DevBuf exact;
+        if ((rc = exact.ensure((size_t)info.n_nodes * BVH_NODE_FLOATS * 4))) return rc;
+        HIPCHK(hipMemcpyAsync(exact.p, g->nodes.p, (size_t)info.n_nodes * BVH_NODE_FLOATS * 4,
+                              hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::swap(g->nodes.p, exact.p);
+        std::swap(g->nodes.bytes, exact.bytes);
+        bvh.n_nodes = info.n_nodes;
+        bvh.n_leaves = info.n_leaves;
+        bvh.depth = info.depth;
+        bvh.max_stack = info.max_stack;
+        for (int k = 0; k < 3; k++) {
+            bvh.lo[k] = info.lo[k];
+            bvh.hi[k] = info.hi[k];
+        }
+    } else {
+        std::vector<float> hp(3 * (size_t)nv);
+        std::vector<uint32_t> ht(3 * (size_t)nt);
+        if (nv) HIPCHK(hipMemcpyAsync(hp.data(), mesh->pos.p, 12 * (size_t)nv, hipMemcpyDeviceToHost, st));
+        if (nt) HIPCHK(hipMemcpyAsync(ht.data(), mesh->tris.p, 12 * (size_t)nt, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        build_wide_bvh(hp.data(), ht.data(), nt, &bvh);
+        if ((rc = upload(g->nodes, bvh.nodes.data(), bvh.nodes.size() * 4, st))) return rc;
+        std::vector<float> dummy(12, 0.0f);
+        if ((rc = upload(g->tris, bvh.tris.empty() ? dummy.data() : bvh.tris.data(),
+                         std::max<size_t>(bvh.tris.size(), 12) * 4, st)))
+            return rc;
+    }
+    if (bvh.max_stack + 1 > (uint32_t)(STACK_LDS + STACK_SPILL))
+        return fail(AG_E_INVALID, "BVH too deep for the traversal stack (" + std::to_string(bvh.max_stack) +
+                                      " entries needed)");
+    g->n_nodes = bvh.n_nodes;
+    g->n_leaves = bvh.n_leaves;
+    g->depth = bvh.depth;
+    g->n_vertices = nv;
+    g->n_triangles = nt;
+    for (int k = 0; k < 3; k++) {
+        g->lo[k] = bvh.lo[k];
+        g->hi[k] = bvh.hi[k];
+    }
+    HIPCHK(hipStreamSynchronize(st));
     return scene_over(ctx, std::move(g), d, out);
 }
 

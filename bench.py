@@ -587,6 +587,90 @@ def bvh_build(args):
     print(json.dumps(out), flush=True)
 
 
+def new_mesh(args):
+    """Serving a new STL: from the file's bytes in host memory to the first
+    developed preview (the live view's half-resolution first look at
+    540x960) -- upload, parse and weld (ag_mesh_load_stl), crease normals
+    (ag_mesh_normals), framing on the host from the welded positions, scene
+    with its BVH built on the device (ag_scene_create_from_mesh), then the
+    first live step.  The reference spends ~3 s in prepare for a 1.5 M-triangle
+    mesh (API.md:19) before its first pass; the oracle's chain (load, normals,
+    host BVH, first look) is timed as the CPU baseline."""
+    import argent_amd as ag
+    from argent_amd import synth, workloads
+    pos, tris, _ = synth.signet()
+    data = synth.to_binary_stl(pos, tris)
+    w, h = 540, 960
+    ctx = ag.Context(0)
+    clock = {"offset": 0.0}
+
+    def once():
+        t0 = time.perf_counter()
+        m, _ = ag.load_stl(ctx, data)
+        t1 = time.perf_counter()
+        m.normals(50.0)
+        t2 = time.perf_counter()
+        mp, _ = m.read(triangles=False)  # framing needs the positions only; the scene's arrays stay on the device
+        piece = workloads.Piece(mp, np.zeros((1, 3), np.uint32))
+        desc = piece.scene("sterling", 0.07, 0.8, "white")
+        cam, turn = piece.shot("hero", 100.0, 0.0, w / h)
+        desc.d.turntable_deg = turn
+        t3 = time.perf_counter()
+        sc = ag.Scene.from_mesh(ctx, m, desc, gpu_bvh=True)
+        t4 = time.perf_counter()
+        lv = ag.Live(ctx)
+        clock["offset"] += 1e4
+        lv.set(sc, cam, w, h, max_spp=256, now_ms=time.monotonic() * 1e3 + clock["offset"])
+        f, img = lv.step(now_ms=time.monotonic() * 1e3 + clock["offset"])
+        t5 = time.perf_counter()
+        assert f["published"] and f["half"]
+        lv.close()
+        sc.close()
+        m.close()
+        ms = lambda a, b: (b - a) * 1e3
+        return {"total_ms": ms(t0, t5), "load_ms": ms(t0, t1), "normals_ms": ms(t1, t2), "framing_ms": ms(t2, t3),
+                "scene_ms": ms(t3, t4), "first_look_ms": ms(t4, t5)}
+
+    for _ in range(args.warmup):
+        once()
+    runs = [once() for _ in range(max(args.steps, 1))]
+    med = {k: round(float(np.median([r[k] for r in runs])), 2) for k in runs[0]}
+    res = {"metric": "new STL to first preview (upload, ingest, normals, scene + BVH, first look), 1 GPU",
+           "value": med["total_ms"], "unit": "ms", "n_gpus": 1, "steps": len(runs), "warmup": args.warmup,
+           "ms_per_step": med["total_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "signet ring as a binary STL (1,198,332 triangles, 59.9 MB in host memory), "
+                                  "sterling, white rig, preview 540x960, first look at 270x480 x 2 spp",
+                      "stl_bytes": len(data)},
+           "stages_ms": med}
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        wp, wt, _ = O.load_stl(data)
+        wq, wn, ws, _ = O.normals(wp, wt, 50.0)
+        piece = workloads.Piece(wq, ws, wn)
+        desc = piece.scene("sterling", 0.07, 0.8, "white")
+        cam, turn = piece.shot("hero", 100.0, 0.0, w / h)
+        desc.d.turntable_deg = turn
+        t1 = time.perf_counter()
+        osc = O.OracleScene(desc)
+        t2 = time.perf_counter()
+        hw, hh = w // 2, h // 2
+        sums = np.zeros((3, hh, hw, 3), np.float32)
+        osc.render_pass(cam, hw, hh, sums, 0, workloads.path_settings(32), 2, 1, threads=threads)
+        k = np.float32(0.5)
+        O.develop(O.atrous(sums[0] * k, sums[1] * k, sums[2] * k), "agx_punchy")
+        t3 = time.perf_counter()
+        res["cpu_baseline"] = {"value": round((t3 - t0) * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+                               "sample": f"the chain once: load + normals {(t1 - t0) * 1e3:.0f} ms (1 thread), "
+                                         f"BVH {(t2 - t1) * 1e3:.0f} ms, first look {(t3 - t2) * 1e3:.0f} ms "
+                                         f"({threads} threads render, 1 thread post)", "cpu_model": _cpu_model()}
+        res["cpu_over_gpu"] = round(res["cpu_baseline"]["value"] / res["value"], 1)
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -605,6 +689,7 @@ def main():
     ap.add_argument("--ingest", action="store_true", help="binary STL ingest (parse + weld) of the signet instead")
     ap.add_argument("--live", action="store_true", help="the live view's first look / refinement loop instead")
     ap.add_argument("--bvh", action="store_true", help="scene creation with the device BVH build vs the host SAH")
+    ap.add_argument("--new-mesh", action="store_true", help="a new STL's bytes to its first preview (serving)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -639,6 +724,9 @@ def main():
         return
     if args.bvh:
         bvh_build(args)
+        return
+    if args.new_mesh:
+        new_mesh(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "

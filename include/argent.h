@@ -245,6 +245,15 @@ int ag_scene_create(ag_ctx* ctx, const ag_scene_desc* desc, ag_scene** out);
  * Closest-hit results are the same with either tree. */
 enum { AG_SCENE_GPU_BVH = 1 };
 int ag_scene_create_ex(ag_ctx* ctx, const ag_scene_desc* desc, uint32_t flags, ag_scene** out);
+/* Scene::new over a device mesh (see ag_mesh_load_stl below): the mesh's
+ * positions, triangles and normals (if ag_mesh_normals ran) are copied on the
+ * device, every triangle takes material slot 0, and the BVH is built on the
+ * device with AG_SCENE_GPU_BVH (else on the host after a read-back).
+ * Materials, panels, environment, floor and white come from `desc`, whose
+ * mesh arrays are ignored.  The mesh may be destroyed afterwards. */
+typedef struct ag_mesh ag_mesh;
+int ag_scene_create_from_mesh(ag_ctx* ctx, const ag_mesh* mesh, const ag_scene_desc* desc, uint32_t flags,
+                              ag_scene** out);
 /* Scene::new(Arc<Piece>, ...) over the piece of `base` (scene.rs:48, :82-86;
  * jobs.rs:690-696 and shot.rs:206-221 clone the Arc per look): materials,
  * panels, environment, floor and white come from `desc`, the mesh and BVH are
@@ -355,7 +364,6 @@ int ag_film_develop(ag_ctx* ctx, const ag_film* film, const ag_grade* grade, int
  * triangles (n_triangles x 3 indices), argent-mesh lib.rs:33-39 (Mesh) without
  * parts.  Welding and compaction run on the device and give the reference's
  * output bit for bit. */
-typedef struct ag_mesh ag_mesh;
 
 /* LoadOptions (argent-mesh lib.rs:53-74); defaults weld 1, weld_tolerance
  * 1e-6, crease_angle_deg 50, unit_scale 1. */
