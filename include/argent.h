@@ -476,10 +476,14 @@ typedef struct ag_live_frame {
 
 int ag_live_create(ag_ctx* ctx, ag_live** out);
 void ag_live_destroy(ag_live* live);
-/* A change of scene, camera or params: a new generation (accumulation restarts). */
+/* A change of scene, camera or params: a new generation (accumulation restarts).
+ * The session keeps a pointer to `scene`, which must outlive its use (until
+ * the next ag_live_set or ag_live_destroy). */
 int ag_live_set(ag_live* live, const ag_scene* scene, const ag_camera* cam, const ag_live_params* params,
                 double now_ms);
-/* A look-only change (grade, denoise): the last frame is re-developed. */
+/* A look-only change (grade, denoise): the last frame is re-developed at the
+ * next step.  now_ms is accepted for symmetry with ag_live_set; the look
+ * change does not restart the motion window. */
 int ag_live_set_look(ag_live* live, const ag_grade* grade, int32_t denoise, double now_ms);
 /* One worker iteration.  out: width * height * 3 bytes of the params' size. */
 int ag_live_step(ag_live* live, double now_ms, ag_live_frame* frame, uint8_t* out);
