@@ -1447,6 +1447,33 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow(const DView* _
     traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st);
 }
 
+// The shadow rays of one depth, then the extension rays of the next, in one
+// persistent launch.  The two queues are independent: a shadow ray adds to its
+// path's radiance (P.L) and reads only the shadow queue, an extension ray
+// writes its hit record and material bucket.  Waves that run out of shadow
+// rays go straight on to the extension queue, so the two launches' latency
+// tails overlap; that is where a small pass spends most of its bounces.  The
+// next shade, which adds emission to P.L, runs after this launch, so every
+// path's radiance sums in the same order as with separate launches.
+__global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const DView* __restrict__ views, DPaths P,
+                                                              DQueues Q, int depth, uint32_t refill_shadow,
+                                                              uint32_t refill_extend, uint2* spill) {
+    TRAV_STACK_DECL
+    __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
+    {
+        uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
+        ShadowJob J{views, P, Q};
+        traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET,
+                                  refill_shadow, st);
+    }
+    {
+        const int nd = depth + 1;
+        uint32_t* cnt = Q.counters + nd * CNT_STRIDE;
+        ExtendJob J{views, P, Q, nd, cnt, Q.ray_o[nd & 1], Q.ray_d[nd & 1], s_stage[threadIdx.x / 64], 0u};
+        traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st);
+    }
+}
+
 // render.rs:96-111: per pixel, samples in order, into the pass accumulators
 __global__ __launch_bounds__(256) void k_accumulate(const DSeg* __restrict__ segs, int nseg,
                                                     const DView* __restrict__ views, const float* __restrict__ cmf,
@@ -1577,10 +1604,10 @@ void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const fl
 }
 int traversal_blocks_per_cu(bool batch) {
     int lo = 1 << 30;
-    const void* render[2] = {(const void*)k_extend, (const void*)k_shadow};
+    const void* render[3] = {(const void*)k_extend, (const void*)k_shadow, (const void*)k_shadow_extend};
     const void* rays[3] = {(const void*)k_rays_intersect, (const void*)k_rays_occluded, (const void*)k_ao};
     const void* const* ks = batch ? rays : render;
-    for (int i = 0; i < (batch ? 3 : 2); i++) {
+    for (int i = 0; i < 3; i++) {
         const void* k = ks[i];
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, TRAV_BLOCK, 0) != hipSuccess || b < 1) b = 1;
@@ -1610,6 +1637,11 @@ void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint2* spill) {
     hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spill);
+}
+void launch_shadow_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth,
+                          uint32_t refill_shadow, uint32_t refill_extend, uint2* spill) {
+    hipLaunchKernelGGL(k_shadow_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill_shadow,
+                       refill_extend, spill);
 }
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
                        DPaths P) {

@@ -1019,6 +1019,10 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
         (void)hipEventRecord(b, st);
         c->ev_marks.push_back({stage, depth, i0});
     };
+    // Shadow rays of depth d and extension rays of depth d + 1 run as one
+    // launch, except with per-stage timing on, where each stage keeps its own
+    // launch so that it can be timed (the results are the same either way).
+    const bool fuse = !c->timing || ev;
     auto t0 = std::chrono::steady_clock::now();
     uint32_t launches = 0;
     for (auto& w : waves) {
@@ -1029,13 +1033,18 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
                         (uint32_t)(counter_words(ps->max_depth) + (size_t)(ps->max_depth + 2) * CURSORS_PER_DEPTH));
         timed(3, -1, [&] { launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), npaths, views, c->P, c->Q); });
         for (uint32_t d = 0; d < ps->max_depth; d++) {
-            timed(0, (int)d, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, c->spill_buf.as<uint2>()); });
+            if (!fuse || d == 0)
+                timed(0, (int)d, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, c->spill_buf.as<uint2>()); });
             timed(1, (int)d, [&] {
                 launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, ps->rr_depth, ps->clamp_indirect,
                              c->energy_buf.as<float>(), c->P,
                              c->Q, (int)d);
             });
-            timed(2, (int)d, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.shadow_refill_min, c->spill_buf.as<uint2>()); });
+            if (fuse && d + 1 < ps->max_depth)
+                launch_shadow_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.shadow_refill_min,
+                                     c->tuning.refill_min, c->spill_buf.as<uint2>());
+            else
+                timed(2, (int)d, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.shadow_refill_min, c->spill_buf.as<uint2>()); });
             launches++;
         }
         timed(4, -1, [&] { launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P); });

@@ -401,3 +401,31 @@ def test_async_passes_equal_sync_passes(ctx):
     f_sync.close()
     f_async.close()
     sc.close()
+
+
+def test_fused_shadow_extend_equals_split_launches(ctx):
+    """The render loop runs the shadow rays of depth d and the extension rays
+    of depth d + 1 as one launch; with per-stage timing on they run as two.
+    Same film bit for bit, same ray counts (gems scene: panel and sky shadow
+    rays, refraction, the HDRI, depth 24)."""
+    wl = workloads.signet_gems_still(hdri_size=(256, 128))
+    desc = wl["piece"].scene(wl["metal"], 0.07, 0.8, wl["studio"], gems=wl["gems"])
+    desc.set_env_map(wl["hdri"])
+    cam, turn = wl["piece"].shot("hero", 100.0, 0.0, 1.0)
+    desc.d.turntable_deg = turn
+    sc = ag.Scene(ctx, desc)
+    ps = workloads.path_settings(24)
+    out = []
+    try:
+        for timing in (True, False):
+            ctx.enable_timing(timing)
+            film = ag.Film(ctx, 80, 80)
+            ag.render_pass(ctx, sc, cam, film, ps, 6, 11)
+            s = ctx.stats()
+            out.append((film.sums(), s["extension_rays"], s["shadow_rays"]))
+            film.close()
+    finally:
+        ctx.enable_timing(False)
+        sc.close()
+    assert out[0][1:] == out[1][1:] and out[0][2] > 0
+    assert np.array_equal(out[0][0], out[1][0])
