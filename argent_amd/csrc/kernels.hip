@@ -268,6 +268,7 @@ struct RaySupply {
     uint32_t* cursors;
     uint32_t lo, hi;      // grabbed range not yet handed out
     uint32_t slice, tried;
+    uint32_t grab;        // rays per cursor atomic (GRAB, or fewer for a spread queue)
 };
 
 AGD uint32_t slice_begin(uint32_t n, uint32_t s) { return (uint32_t)(((uint64_t)n * s) / CURSOR_SLICES); }
@@ -286,12 +287,12 @@ AGD uint32_t supply_take(RaySupply& R, uint32_t want, uint32_t* first) {
             // other slices once the queue runs dry).
             uint32_t* cur = R.cursors + R.slice * CURSOR_STRIDE;
             if (R.tried == 0 || __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e - b)
-                got = atomicAdd(cur, GRAB);
+                got = atomicAdd(cur, R.grab);
         }
         got = __shfl(got, 0);
         if (got < e - b) {
             R.lo = b + got;
-            R.hi = b + min(got + GRAB, e - b);
+            R.hi = b + min(got + R.grab, e - b);
         } else {
             R.slice = (R.slice + 1) % CURSOR_SLICES;
             R.tried++;
@@ -432,16 +433,30 @@ AGD void step_any(TravLane& L, const TravStack& st) {
 // trace, finish at once); void finish(uint32_t i, const TravLane& L, bool mine)
 // is called by every lane of the wavefront, `mine` where ray i just finished;
 // void drain() once by every lane after the last ray.
+//
+// A queue too short to give every wavefront of the grid 64 rays is spread
+// over all of them: each takes `lanes` = ceil(n / wavefronts) rays at a time
+// (at least min_lanes, at most 64) in its lowest lanes.  A launch over a few
+// thousand rays then lasts about as long as its slowest ray instead of its
+// slowest batch of 64 divergent ones, and every CU takes part.  Which rays
+// share a wavefront does not change any ray's result.
 template <bool ANY, class Job>
-AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t refill_min, const TravStack& st) {
+AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t refill_min, const TravStack& st,
+                             uint32_t min_lanes = 64u) {
     TravLane L;
     L.cur = TRAV_DONE;
     L.sp = 0;
     uint32_t idx = NO_RAY;
-    RaySupply R{n, cursors, 0u, 0u, blockIdx.x % CURSOR_SLICES, 0u};
-    // wavefronts beyond what n rays can occupy (GRAB each) leave at once
+    const uint32_t nwaves = gridDim.x * (blockDim.x / 64u);
+    uint32_t lanes = (uint32_t)(((uint64_t)n + nwaves - 1u) / nwaves);
+    lanes = min(max(lanes, min_lanes), 64u);
+    const uint32_t grab = lanes < 64u ? lanes : GRAB;
+    const bool lane_ok = lane_id() < lanes;
+    const uint32_t refill = min(refill_min, lanes);
+    RaySupply R{n, cursors, 0u, 0u, blockIdx.x % CURSOR_SLICES, 0u, grab};
+    // wavefronts beyond what n rays can occupy (`grab` each) leave at once
     uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
-    bool more = wave < (n + GRAB - 1) / GRAB;  // uniform across the wavefront
+    bool more = wave < (n + grab - 1u) / grab;  // uniform across the wavefront
     // A lane whose ray has finished traversing stays parked (idx kept, L.cur
     // == TRAV_DONE) until the next refill, when all parked lanes are finished
     // in one call: a job's finish reads memory (a material id, a radiance
@@ -449,7 +464,7 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
     // at which some lane happened to finish.
     for (;;) {
         bool parked = idx != NO_RAY && L.cur == TRAV_DONE;
-        if (more && (uint32_t)__popcll(__ballot(idx == NO_RAY || parked)) >= refill_min) {
+        if (more && (uint32_t)__popcll(__ballot(lane_ok && (idx == NO_RAY || parked))) >= refill) {
             if (__ballot(parked) != 0) {
                 J.finish(idx, L, parked);
                 if (parked) idx = NO_RAY;
@@ -460,7 +475,7 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
             // the piece finishes at once and its lane takes the next one, so
             // traversal batches hold only rays that need it.
             for (;;) {
-                uint64_t idle = __ballot(idx == NO_RAY);
+                uint64_t idle = __ballot(lane_ok && idx == NO_RAY);
                 uint32_t cnt = (uint32_t)__popcll(idle);
                 if (cnt == 0) break;
                 uint32_t first = 0;
@@ -470,7 +485,7 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
                     break;
                 }
                 bool fin = false;
-                if (idx == NO_RAY) {
+                if (lane_ok && idx == NO_RAY) {
                     uint32_t r = lane_prefix(idle);
                     if (r < k) {
                         idx = first + r;
@@ -1075,12 +1090,12 @@ struct ExtendJob {
 };
 
 __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
-                                                       uint32_t refill, uint2* spill) {
+                                                       uint32_t refill, uint32_t spread, uint2* spill) {
     TRAV_STACK_DECL
     __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64], 0u};
-    traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st);
+    traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread);
 }
 
 struct ShadeArgs {
@@ -1440,11 +1455,12 @@ struct ShadowJob {
 };
 
 __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow(const DView* __restrict__ views, DPaths P, DQueues Q, int depth,
-                                                       uint32_t refill, uint2* spill) {
+                                                       uint32_t refill, uint32_t spread, uint2* spill) {
     TRAV_STACK_DECL
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ShadowJob J{views, P, Q};
-    traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st);
+    traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st,
+                              spread);
 }
 
 // The shadow rays of one depth, then the extension rays of the next, in one
@@ -1457,20 +1473,21 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow(const DView* _
 // path's radiance sums in the same order as with separate launches.
 __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const DView* __restrict__ views, DPaths P,
                                                               DQueues Q, int depth, uint32_t refill_shadow,
-                                                              uint32_t refill_extend, uint2* spill) {
+                                                              uint32_t refill_extend, uint32_t spread,
+                                                              uint2* spill) {
     TRAV_STACK_DECL
     __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
     {
         uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
         ShadowJob J{views, P, Q};
         traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET,
-                                  refill_shadow, st);
+                                  refill_shadow, st, spread);
     }
     {
         const int nd = depth + 1;
         uint32_t* cnt = Q.counters + nd * CNT_STRIDE;
         ExtendJob J{views, P, Q, nd, cnt, Q.ray_o[nd & 1], Q.ray_d[nd & 1], s_stage[threadIdx.x / 64], 0u};
-        traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st);
+        traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread);
     }
 }
 
@@ -1626,8 +1643,8 @@ void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t
     hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, s, segs, nseg, npaths, views, P, Q);
 }
 void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
-                   uint2* spill) {
-    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spill);
+                   uint32_t spread, uint2* spill) {
+    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spread, spill);
 }
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
                   float clamp_indirect, const float* energy, DPaths P, DQueues Q, int depth) {
@@ -1635,13 +1652,13 @@ void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
 }
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
-                   uint2* spill) {
-    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spill);
+                   uint32_t spread, uint2* spill) {
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spread, spill);
 }
 void launch_shadow_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth,
-                          uint32_t refill_shadow, uint32_t refill_extend, uint2* spill) {
+                          uint32_t refill_shadow, uint32_t refill_extend, uint32_t spread, uint2* spill) {
     hipLaunchKernelGGL(k_shadow_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill_shadow,
-                       refill_extend, spill);
+                       refill_extend, spread, spill);
 }
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
                        DPaths P) {

@@ -40,15 +40,16 @@ void launch_ao(hipStream_t s, int grid, const float4* nodes, const float4* tris,
 int traversal_blocks_per_cu(bool batch);  // resident blocks per CU: ray-batch or render kernels
 void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,
                    DQueues Q);
+// spread: the least lanes per wavefront when a short queue is spread over the grid (64: never)
 void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
-                   uint2* spill);
+                   uint32_t spread, uint2* spill);
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
                   float clamp_indirect, const float* energy, DPaths P, DQueues Q, int depth);
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
-                   uint2* spill);
+                   uint32_t spread, uint2* spill);
 // k_shadow then k_extend of depth + 1 as one launch (same results)
 void launch_shadow_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth,
-                          uint32_t refill_shadow, uint32_t refill_extend, uint2* spill);
+                          uint32_t refill_shadow, uint32_t refill_extend, uint32_t spread, uint2* spill);
 void launch_accumulate(hipStream_t s, int grid, const DSeg* segs, int nseg, const DView* views, const float* cmf,
                        DPaths P);
 void launch_tally(hipStream_t s, const uint32_t* counters, uint32_t depth_slots, unsigned long long* totals);

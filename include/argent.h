@@ -583,6 +583,11 @@ typedef struct ag_tuning {
     uint32_t batch_blocks_per_cu; /* the same two for ag_intersect / ag_occluded */
     uint32_t batch_refill_min;
     uint32_t shadow_refill_min;   /* render passes, shadow rays (1..64) */
+    uint32_t spread_min_lanes;    /* render passes: a ray queue too short to give
+                                     every wavefront of the grid 64 rays is
+                                     spread over all of them, ceil(n / wavefronts)
+                                     lanes each but at least this many (1..64;
+                                     64 = never spread; default 1) */
 } ag_tuning;
 int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
 int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);

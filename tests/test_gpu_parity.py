@@ -429,3 +429,35 @@ def test_fused_shadow_extend_equals_split_launches(ctx):
         sc.close()
     assert out[0][1:] == out[1][1:] and out[0][2] > 0
     assert np.array_equal(out[0][0], out[1][0])
+
+
+def test_spread_queues_equal_full_batches(ctx):
+    """A short ray queue spread over every wavefront (spread_min_lanes 1: as
+    few as one ray per wavefront) gives the same film bit for bit and the
+    same ray counts as batches of 64 rays per wavefront."""
+    wl = workloads.signet_gems_still(hdri_size=(256, 128))
+    desc = wl["piece"].scene(wl["metal"], 0.07, 0.8, wl["studio"], gems=wl["gems"])
+    desc.set_env_map(wl["hdri"])
+    cam, turn = wl["piece"].shot("hero", 100.0, 0.0, 1.0)
+    desc.d.turntable_deg = turn
+    sc = ag.Scene(ctx, desc)
+    ps = workloads.path_settings(24)
+    keep = ctx.tuning
+    out = []
+    try:
+        for lanes in (64, 1, 7):
+            ctx.set_tuning(spread_min_lanes=lanes)
+            film = ag.Film(ctx, 72, 72)
+            ag.render_pass(ctx, sc, cam, film, ps, 5, 13)
+            s = ctx.stats()
+            out.append((film.sums(), s["extension_rays"], s["shadow_rays"]))
+            film.close()
+    finally:
+        ctx.set_tuning(**keep)
+        sc.close()
+    for o in out[1:]:
+        assert o[1:] == out[0][1:]
+        assert np.array_equal(o[0], out[0][0])
+    with pytest.raises(ag.ArgentError):
+        ctx.set_tuning(spread_min_lanes=0)
+    assert ctx.tuning == keep
