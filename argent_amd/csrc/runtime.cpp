@@ -1321,8 +1321,14 @@ extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queu
         });
         std::vector<uint64_t> chosen;
         uint64_t room = s->step_samples;
+        // A step with requests still waiting for their first preview renders
+        // only those, so the previews are published after a short step rather
+        // than a full one.  The others sit the step out; each request's chunk
+        // sequence, and so its image, does not depend on the steps it skips.
+        const bool preview_step = s->reqs[order.front()].st.t_first_preview < 0.0;
         for (uint64_t k : order) {
             const auto& q = s->reqs[k];
+            if (preview_step && q.st.t_first_preview >= 0.0) break;
             uint64_t cost = (uint64_t)q.next_chunk * q.r.width * q.r.height;
             if (!chosen.empty() && cost > room) continue;
             chosen.push_back(k);

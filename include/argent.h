@@ -533,10 +533,11 @@ int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_material* mater
  * Requests are admitted FIFO while their films fit the HBM budget.  Every
  * ag_server_step renders, in one shared wavefront schedule (requests on
  * different scenes share waves), the next sample chunk of as many active
- * requests as fit in step_samples samples: first those still waiting for their
- * first preview, then those with the least work left (at least one request per
- * step).  A request arriving while others run thus gets its preview within
- * about one step.  A request's chunk sizes depend only on its own spp and the
+ * requests as fit in step_samples samples, those with the least work left
+ * first (at least one request per step).  A step in which any request still
+ * waits for its first preview renders only such requests, so a request
+ * arriving while others run gets its preview after the running step plus one
+ * short preview step.  A request's chunk sizes depend only on its own spp and the
  * server's settings (first chunk preview_spp, then doubling up to a cap of
  * step_samples / pixels), so its film is bit-identical whether it is served
  * alone or in a mix.  Films stay resident until released; ag_server_read
