@@ -198,6 +198,7 @@ def main():
     ap.add_argument("--spp-hi", type=int, default=1024)
     ap.add_argument("--meshes", type=int, default=8)
     ap.add_argument("--max-wall", type=float, default=None)
+    ap.add_argument("--step-log2", type=int, default=25, help="server step budget, log2 samples")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
@@ -206,8 +207,9 @@ def main():
     pieces = build_pieces(args.meshes)
     log("meshes:", [len(p.triangles) for p in pieces])
     with Context(0) as ctx:
-        res = run_trace(ctx, trace, pieces, max_wall=args.max_wall, log=log)
-    res["config"] = dict(seconds=args.seconds, rate=args.rate, spp_hi=args.spp_hi, meshes=args.meshes)
+        res = run_trace(ctx, trace, pieces, step_samples=1 << args.step_log2, max_wall=args.max_wall, log=log)
+    res["config"] = dict(seconds=args.seconds, rate=args.rate, spp_hi=args.spp_hi, meshes=args.meshes,
+                         step_samples=1 << args.step_log2)
     print(json.dumps(res))
     if args.out:
         json.dump(res, open(args.out, "w"), indent=1)
