@@ -140,16 +140,14 @@ AGD uint32_t wave_append_keyed(uint32_t* counters, int key) {
 // a pop is an LDS read or a global read: with plain pointers the compiler
 // merges the two branches into one flat load, which waits on both counters.
 struct TravStack {
-    AG_LDS uint32_t* node;  // LDS [STACK_LDS][blockDim]
-    AG_LDS float* tn;       // LDS [STACK_LDS][blockDim]
+    AG_LDS uint2* ent;      // LDS [STACK_LDS][blockDim]: (entry, entry distance bits), one 64-bit access each
     AG_GLOBAL uint2* spill; // HBM [STACK_SPILL] for this thread
     uint32_t tid, nthreads;
 };
 
 AGD void st_push(const TravStack& s, int& sp, uint32_t e, float t) {
     if (sp < STACK_LDS) {
-        s.node[sp * s.nthreads + s.tid] = e;
-        s.tn[sp * s.nthreads + s.tid] = t;
+        s.ent[sp * s.nthreads + s.tid] = make_uint2(e, __float_as_uint(t));
     } else if (sp < STACK_LDS + STACK_SPILL) {
         s.spill[sp - STACK_LDS] = make_uint2(e, __float_as_uint(t));
     } else {
@@ -159,14 +157,13 @@ AGD void st_push(const TravStack& s, int& sp, uint32_t e, float t) {
 }
 AGD void st_pop(const TravStack& s, int& sp, uint32_t& e, float& t) {
     sp--;
-    if (sp < STACK_LDS) {
-        e = s.node[sp * s.nthreads + s.tid];
-        t = s.tn[sp * s.nthreads + s.tid];
-    } else {
-        uint2 v = s.spill[sp - STACK_LDS];
-        e = v.x;
-        t = __uint_as_float(v.y);
-    }
+    uint2 v;
+    if (sp < STACK_LDS)
+        v = s.ent[sp * s.nthreads + s.tid];
+    else
+        v = s.spill[sp - STACK_LDS];
+    e = v.x;
+    t = __uint_as_float(v.y);
 }
 
 constexpr float BOX_LO = 1.0f - 1.52587890625e-5f;  // 1 - 2^-16
@@ -180,7 +177,23 @@ constexpr float BOX_HI = 1.0f + 1.52587890625e-5f;
 #endif
 constexpr int NODE_F4 = AG_NODE_FLOATS / 4;  // node stride in float4 (the node itself is 5 float4)
 
-AGD void slab8(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float tm[8], uint32_t ch[8]) {
+// What a node holds to name child k: the first inner child and first leaf
+// triangle, the 8 child bytes (count << 5 | offset) and the inner-child mask.
+struct NodeKids {
+    uint32_t inner_base, leaf_base, mlo, mhi, imask;
+};
+
+// Child k's stack entry: an inner node index, or a leaf (bit 31, count - 1
+// in bits 27-30, first triangle below).
+AGD uint32_t child_entry(const NodeKids& K, uint32_t k) {
+    uint32_t m = ((k < 4u ? K.mlo : K.mhi) >> ((k & 3u) * 8u)) & 0xFFu;
+    bool inner = (K.imask >> k) & 1u;
+    return inner ? K.inner_base + m : (0x80000000u | (((m >> 5) - 1u) << 27) | (K.leaf_base + (m & 31u)));
+}
+
+// Slab tests of a node's 8 children: tm[k] = child k's entry distance, or
+// INF_F for a miss or an empty slot.
+AGD void slab8_t(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float tm[8], NodeKids& K) {
     float4 n0 = nd[0];
     uint4 n1 = *reinterpret_cast<const uint4*>(&nd[1]);
     uint4 qa = *reinterpret_cast<const uint4*>(&nd[2]);
@@ -191,6 +204,7 @@ AGD void slab8(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float tm
     float sy = __uint_as_float(((w >> 8) & 0xFFu) << 23);
     float sz = __uint_as_float(((w >> 16) & 0xFFu) << 23);
     uint32_t imask = w >> 24;
+    K = NodeKids{n1.x, n1.y, n1.z, n1.w, imask};
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int sh = (k & 3) * 8;
@@ -204,7 +218,6 @@ AGD void slab8(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float tm
         float hz = fmaf((float)(((hiw ? qc.w : qc.z) >> sh) & 0xFFu), sz, n0.z);
         bool inner = (imask >> k) & 1u;
         uint32_t cnt = m >> 5;
-        ch[k] = inner ? n1.x + m : (0x80000000u | ((cnt - 1u) << 27) | (n1.y + (m & 31u)));
         float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
         float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
         float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
@@ -215,6 +228,13 @@ AGD void slab8(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float tm
         bool hit = (inner || cnt != 0u) && tmin <= tmx;
         tm[k] = hit ? tmin : INF_F;
     }
+}
+
+AGD void slab8(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float tm[8], uint32_t ch[8]) {
+    NodeKids K;
+    slab8_t(nd, o, inv, tmax, tm, K);
+#pragma unroll
+    for (int k = 0; k < 8; k++) ch[k] = child_entry(K, (uint32_t)k);
 }
 
 // bvh.rs:391-409, returning t (or -1) and u, v
@@ -337,23 +357,34 @@ AGD void lane_begin(TravLane& L, const float4* nodes, const float4* tris, f3 o, 
 AGD void step_closest(TravLane& L, const TravStack& st) {
     if (!(L.cur & 0x80000000u)) {
         float tm[8];
-        uint32_t ch[8];
-        slab8(L.nodes + (size_t)L.cur * NODE_F4, L.o, L.inv, L.t, tm, ch);
-        // ascending t: the 19-comparator sorting network for 8 keys
-#define CSWAP(i, j)                                       \
-    if (tm[j] < tm[i]) {                                  \
-        float tt = tm[i]; tm[i] = tm[j]; tm[j] = tt;      \
-        uint32_t cc = ch[i]; ch[i] = ch[j]; ch[j] = cc;   \
+        NodeKids K;
+        slab8_t(L.nodes + (size_t)L.cur * NODE_F4, L.o, L.inv, L.t, tm, K);
+        // Sort keys: the entry distance (>= 0, so its bits order as integers)
+        // with the child slot in the 3 low mantissa bits; a miss sorts last.
+        // The distance pushed is the key's, rounded down by at most 7 ulp:
+        // a box is only ever visited more readily, and the closest hit
+        // (least t, then least triangle id) does not depend on visit order.
+        uint32_t key[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            key[k] = tm[k] != INF_F ? ((__float_as_uint(tm[k]) & 0x7FFFFFF8u) | (uint32_t)k) : 0xFFFFFFFFu;
+        // ascending: the 19-comparator sorting network for 8 keys
+#define KSWAP(i, j)                                             \
+    {                                                           \
+        uint32_t a = min(key[i], key[j]), b = max(key[i], key[j]); \
+        key[i] = a;                                             \
+        key[j] = b;                                             \
     }
-        CSWAP(0, 2) CSWAP(1, 3) CSWAP(4, 6) CSWAP(5, 7) CSWAP(0, 4) CSWAP(1, 5) CSWAP(2, 6) CSWAP(3, 7)
-        CSWAP(0, 1) CSWAP(2, 3) CSWAP(4, 5) CSWAP(6, 7) CSWAP(2, 4) CSWAP(3, 5) CSWAP(1, 4) CSWAP(3, 6)
-        CSWAP(1, 2) CSWAP(3, 4) CSWAP(5, 6)
-#undef CSWAP
-        if (tm[0] != INF_F) {
+        KSWAP(0, 2) KSWAP(1, 3) KSWAP(4, 6) KSWAP(5, 7) KSWAP(0, 4) KSWAP(1, 5) KSWAP(2, 6) KSWAP(3, 7)
+        KSWAP(0, 1) KSWAP(2, 3) KSWAP(4, 5) KSWAP(6, 7) KSWAP(2, 4) KSWAP(3, 5) KSWAP(1, 4) KSWAP(3, 6)
+        KSWAP(1, 2) KSWAP(3, 4) KSWAP(5, 6)
+#undef KSWAP
+        if (key[0] != 0xFFFFFFFFu) {
 #pragma unroll
             for (int k = 7; k >= 1; k--)
-                if (tm[k] != INF_F) st_push(st, L.sp, ch[k], tm[k]);
-            L.cur = ch[0];
+                if (key[k] != 0xFFFFFFFFu)
+                    st_push(st, L.sp, child_entry(K, key[k] & 7u), __uint_as_float(key[k] & ~7u));
+            L.cur = child_entry(K, key[0] & 7u);
             return;
         }
     } else {
@@ -533,11 +564,9 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
 #endif
 
 #define TRAV_STACK_DECL                                                     \
-    __shared__ uint32_t s_node[STACK_LDS * TRAV_BLOCK];                     \
-    __shared__ float s_tn[STACK_LDS * TRAV_BLOCK];                          \
+    __shared__ uint2 s_ent[STACK_LDS * TRAV_BLOCK];                         \
     TravStack st;                                                           \
-    st.node = (AG_LDS uint32_t*)s_node;                                     \
-    st.tn = (AG_LDS float*)s_tn;                                            \
+    st.ent = (AG_LDS uint2*)s_ent;                                          \
     st.tid = threadIdx.x;                                                   \
     st.nthreads = blockDim.x;                                               \
     st.spill = (AG_GLOBAL uint2*)(spill + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * STACK_SPILL);
