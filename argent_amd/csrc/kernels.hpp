@@ -27,15 +27,16 @@ struct ProbeIn {
 void launch_bsdf_probe(hipStream_t s, const DScene* S, ProbeIn in, float* out, uint32_t n);
 void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n);
 // `cursor` must hold 0 on entry (the persistent kernels pull rays from it).
-// `refill`: idle lanes (1..64) before a wavefront takes new rays.
+// `refill`: idle lanes (1..64) before a wavefront takes new rays; `spread`:
+// the least lanes per wavefront when a short queue is spread over the grid.
 void launch_rays_intersect(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, HitsSoA h,
-                           uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill);
+                           uint32_t n, uint32_t* cursor, uint32_t refill, uint32_t spread, uint2* spill);
 void launch_rays_occluded(hipStream_t s, int grid, const float4* nodes, const float4* tris, RaysSoA r, uint8_t* out,
-                          uint32_t n, uint32_t* cursor, uint32_t refill, uint2* spill);
+                          uint32_t n, uint32_t* cursor, uint32_t refill, uint32_t spread, uint2* spill);
 // cavity ambient occlusion: n = n_vertices * samples rays, hit counts added to counts[vertex]
 void launch_ao(hipStream_t s, int grid, const float4* nodes, const float4* tris, const float* pos, const float* nrm,
                uint32_t samples, float dist, float eps, uint32_t* counts, uint32_t n, uint32_t* cursor,
-               uint32_t refill, uint2* spill);
+               uint32_t refill, uint32_t spread, uint2* spill);
 // Resident blocks per CU of the traversal kernels (occupancy limit).
 int traversal_blocks_per_cu(bool batch);  // resident blocks per CU: ray-batch or render kernels
 void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t npaths, const DView* views, DPaths P,

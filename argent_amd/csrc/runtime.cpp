@@ -1454,7 +1454,7 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
         launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
         HIPCHK(hipEventRecord(ctx->ev[2], st));
         launch_rays_intersect(st, grid, nodes, tris, r, h, n, ctx->cursor_buf.as<uint32_t>(), refill,
-                              ctx->spill_buf.as<uint2>());
+                              ctx->tuning.spread_min_lanes, ctx->spill_buf.as<uint2>());
         HIPCHK(hipEventRecord(ctx->ev[3], st));
         HIPCHK(hipGetLastError());
         if (!device_ptrs) {
@@ -1472,7 +1472,7 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
         launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
         HIPCHK(hipEventRecord(ctx->ev[2], st));
         launch_rays_occluded(st, grid, nodes, tris, r, o, n, ctx->cursor_buf.as<uint32_t>(), refill,
-                             ctx->spill_buf.as<uint2>());
+                             ctx->tuning.spread_min_lanes, ctx->spill_buf.as<uint2>());
         HIPCHK(hipEventRecord(ctx->ev[3], st));
         HIPCHK(hipGetLastError());
         if (!device_ptrs) HIPCHK(hipMemcpyAsync(occ, o, n, hipMemcpyDeviceToHost, st));
@@ -1511,7 +1511,7 @@ extern "C" int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t sam
     HIPCHK(hipEventRecord(ctx->ev[2], st));
     launch_ao(st, ctx->grid_batch, g.nodes.as<float4>(), g.tris.as<float4>(), g.pos.as<float>(), g.nrm.as<float>(), n,
               distance, eps, counts.as<uint32_t>(), nv * n, ctx->cursor_buf.as<uint32_t>(), ctx->tuning.batch_refill_min,
-              ctx->spill_buf.as<uint2>());
+              ctx->tuning.spread_min_lanes, ctx->spill_buf.as<uint2>());
     HIPCHK(hipEventRecord(ctx->ev[3], st));
     HIPCHK(hipGetLastError());
     std::vector<uint32_t> h(nv);

@@ -584,7 +584,8 @@ typedef struct ag_tuning {
     uint32_t batch_blocks_per_cu; /* the same two for ag_intersect / ag_occluded */
     uint32_t batch_refill_min;
     uint32_t shadow_refill_min;   /* render passes, shadow rays (1..64) */
-    uint32_t spread_min_lanes;    /* render passes: a ray queue too short to give
+    uint32_t spread_min_lanes;    /* every traversal (render passes, ray batches,
+                                     cavity AO): a ray queue too short to give
                                      every wavefront of the grid 64 rays is
                                      spread over all of them, ceil(n / wavefronts)
                                      lanes each but at least this many (1..64;
