@@ -80,8 +80,8 @@ def test_budget_admission_and_rejection(ctx):
 
 def test_new_request_previews_before_long_one_finishes(ctx):
     """With a step budget smaller than the work in flight, a request that
-    arrives while a long one renders gets its first preview at the next step
-    and, having less work left, finishes first; both still match their
+    arrives while a long one renders gets its first preview at the next step,
+    a step that renders nothing else, and, having less work left, finishes first; both still match their
     renders alone."""
     pa, _ = _pieces()
     sa = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
@@ -93,9 +93,12 @@ def test_new_request_previews_before_long_one_finishes(ctx):
         srv.step()
     assert srv.status(long_id)["state"] == A.REQ_ACTIVE
     short_id = srv.submit(sa, cam, 24, 24, 8, 2, turn)
+    before = srv.status(long_id)["samples"]
     srv.step()
     st_s = srv.status(short_id)
     assert st_s["t_first_preview"] >= 0, "no preview after one step"
+    # that step carried only the request awaiting its first preview
+    assert srv.status(long_id)["samples"] == before and st_s["samples"] == 2
     for _ in range(1000):
         a, q = srv.step()
         if a == 0 and q == 0:
