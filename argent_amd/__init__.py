@@ -508,6 +508,19 @@ class Mesh:
         check(A.lib().ag_mesh_read(self.h, _ptr(p), _ptr(t, C.c_uint32) if triangles else None))
         return p, t
 
+    def frame_sample(self, stride=None):
+        """(hull (n, 3) float32: every stride-th vertex, lo (3,), hi (3,)) from
+        the device, without copying the mesh back; stride defaults to the one
+        Piece uses for its hull (max(n_vertices // 4000, 1))."""
+        nv, _ = self.counts
+        stride = max(nv // 4000, 1) if stride is None else int(stride)
+        cap = (nv + stride - 1) // stride if stride > 0 else 0
+        hull = np.empty((max(cap, 1), 3), np.float32)
+        lo, hi = np.empty(3, np.float32), np.empty(3, np.float32)
+        n = C.c_uint32()
+        check(A.lib().ag_mesh_frame_sample(self.h, stride, _ptr(hull), cap, C.byref(n), _ptr(lo), _ptr(hi)))
+        return hull[:n.value], lo, hi
+
     def weld(self, rel_tolerance=1e-6):
         """weld.rs:38-62; returns the WeldReport as a dict."""
         r = A.WeldReport()

@@ -255,6 +255,7 @@ _SIGS = [
     ("ag_mesh_destroy", None, [C.c_void_p]),
     ("ag_mesh_info", C.c_int, [C.c_void_p, u32p, u32p]),
     ("ag_mesh_read", C.c_int, [C.c_void_p, f32p, u32p]),
+    ("ag_mesh_frame_sample", C.c_int, [C.c_void_p, C.c_uint32, f32p, C.c_uint32, C.POINTER(C.c_uint32), f32p, f32p]),
     ("ag_mesh_device_arrays", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("ag_mesh_weld", C.c_int, [C.c_void_p, C.c_float, C.POINTER(WeldReport)]),
     ("ag_mesh_compact", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),

@@ -402,6 +402,16 @@ __global__ __launch_bounds__(BLK) void k_fill_ones(uint32_t* a, uint32_t n) {
     if (i < n) a[i] = 1u;
 }
 
+// out[i] = vertex i * stride, for i < n (i * stride < the vertex count)
+__global__ __launch_bounds__(BLK) void k_gather_stride(const float* pos, uint32_t stride, uint32_t n, float* out) {
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const size_t s = (size_t)i * stride * 3, d = (size_t)i * 3;
+    out[d] = pos[s];
+    out[d + 1] = pos[s + 1];
+    out[d + 2] = pos[s + 2];
+}
+
 __global__ __launch_bounds__(BLK) void k_iota(uint32_t* a, uint32_t n) {
     const uint32_t i = blockIdx.x * BLK + threadIdx.x;
     if (i < n) a[i] = i;
@@ -854,6 +864,49 @@ int mesh_check_bounds(ag_mesh* m, float lo[3], float hi[3]) {
         lo[k] = nv ? from_ord(hv[1 + k]) : 0.0f;
         hi[k] = nv ? from_ord(hv[4 + k]) : 0.0f;
     }
+    return AG_OK;
+}
+
+extern "C" int ag_mesh_frame_sample(const ag_mesh* m, uint32_t stride, float* hull, uint32_t cap, uint32_t* n_hull,
+                                    float* lo, float* hi) {
+    if (!m || !n_hull || !lo || !hi) return fail(AG_E_INVALID, "null argument");
+    if (stride < 1) return fail(AG_E_INVALID, "stride must be at least 1");
+    if (cap && !hull) return fail(AG_E_INVALID, "null hull buffer with cap > 0");
+    MCHK(hipSetDevice(ctx_device(m->ctx)));
+    hipStream_t st = ctx_stream(m->ctx);
+    const uint32_t nv = m->nv;
+    const uint32_t n = (uint32_t)std::min<uint64_t>(cap, ((uint64_t)nv + stride - 1) / stride);
+    const unsigned nb = std::min<unsigned>(grid_of(nv), BOUNDS_BLOCKS);
+    Arena a;
+    const size_t o_ones = a.add(4 * (size_t)std::max(nv, 1u)), o_flag = a.add(256), o_part = a.add(28 * (size_t)nb),
+                 o_hull = a.add(12 * (size_t)std::max(n, 1u));
+    int rc = a.bind(m->ctx);
+    if (rc) return rc;
+    uint32_t* flag = a.at<uint32_t>(o_flag);
+    Init6 init;
+    const float fmax_ = 3.40282347e38f;
+    uint32_t bmax;
+    std::memcpy(&bmax, &fmax_, 4);
+    for (int k = 0; k < 3; k++) {
+        init.v[k] = bmax | 0x80000000u;
+        init.v[3 + k] = ~(bmax | 0x80000000u);
+    }
+    if (nv) {
+        k_fill_ones<<<grid_of(nv), BLK, 0, st>>>(a.at<uint32_t>(o_ones), nv);
+        k_bounds<<<nb, BLK, 0, st>>>(m->pos.as<float>(), a.at<uint32_t>(o_ones), nv, a.at<uint32_t>(o_part));
+        k_bounds_final<<<1, BLK, 0, st>>>(a.at<uint32_t>(o_part), nb, init, flag + 1);
+    }
+    if (n) k_gather_stride<<<grid_of(n), BLK, 0, st>>>(m->pos.as<float>(), stride, n, a.at<float>(o_hull));
+    MCHK(hipGetLastError());
+    uint32_t hv[6] = {0, 0, 0, 0, 0, 0};
+    if (nv) MCHK(hipMemcpyAsync(hv, flag + 1, sizeof hv, hipMemcpyDeviceToHost, st));
+    if (n) MCHK(hipMemcpyAsync(hull, a.at<float>(o_hull), 12 * (size_t)n, hipMemcpyDeviceToHost, st));
+    MCHK(hipStreamSynchronize(st));
+    for (int k = 0; k < 3; k++) {
+        lo[k] = nv ? from_ord(hv[k]) : 0.0f;
+        hi[k] = nv ? from_ord(hv[3 + k]) : 0.0f;
+    }
+    *n_hull = n;
     return AG_OK;
 }
 

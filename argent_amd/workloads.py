@@ -35,6 +35,22 @@ class Piece:
         step = max(len(self.positions) // 4000, 1)
         self.hull = self.positions[::step].copy()
 
+    @classmethod
+    def from_frame(cls, lo, hi, hull):
+        """A Piece for a mesh that stays on the device (Mesh.frame_sample):
+        the same radius, floor height and hull as Piece(positions, ...) on its
+        positions; the scene descriptor it builds carries the hull only, for
+        Scene.from_mesh, which takes the geometry from the device mesh."""
+        p = cls.__new__(cls)
+        lo, hi = np.asarray(lo, np.float32), np.asarray(hi, np.float32)
+        p.positions = np.asarray(hull, np.float32)
+        p.triangles = np.zeros((1, 3), np.uint32)
+        p.normals = p.tri_material = p.cavity = None
+        p.radius = float(max(np.linalg.norm((hi - lo) * 0.5), 1e-3))
+        p.floor_y = float(lo[1])
+        p.hull = p.positions.copy()
+        return p
+
     def scene(self, metal="sterling", roughness=0.07, patina=0.8, studio="noir", gems=()):
         """Prepared::scene (shot.rs:206-245): slot 0 metal, then gems, then
         the studio's floor material."""

@@ -591,7 +591,7 @@ def new_mesh(args):
     """Serving a new STL: from the file's bytes in host memory to the first
     developed preview (the live view's half-resolution first look at
     540x960) -- upload, parse and weld (ag_mesh_load_stl), crease normals
-    (ag_mesh_normals), framing on the host from the welded positions, scene
+    (ag_mesh_normals), framing from the device bounds and hull (ag_mesh_frame_sample), scene
     with its BVH built on the device (ag_scene_create_from_mesh), then the
     first live step.  The reference spends ~3 s in prepare for a 1.5 M-triangle
     mesh (API.md:19) before its first pass; the oracle's chain (load, normals,
@@ -610,8 +610,10 @@ def new_mesh(args):
         t1 = time.perf_counter()
         m.normals(50.0)
         t2 = time.perf_counter()
-        mp, _ = m.read(triangles=False)  # framing needs the positions only; the scene's arrays stay on the device
-        piece = workloads.Piece(mp, np.zeros((1, 3), np.uint32))
+        # framing needs the bounds and a 4000-point hull: taken on the device,
+        # the scene's arrays stay there
+        hull, lo, hi = m.frame_sample()
+        piece = workloads.Piece.from_frame(lo, hi, hull)
         desc = piece.scene("sterling", 0.07, 0.8, "white")
         cam, turn = piece.shot("hero", 100.0, 0.0, w / h)
         desc.d.turntable_deg = turn

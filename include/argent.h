@@ -408,6 +408,14 @@ void ag_mesh_destroy(ag_mesh* m);
 int ag_mesh_info(const ag_mesh* m, uint32_t* n_vertices, uint32_t* n_triangles);
 /* Copies the mesh to host arrays (either may be NULL). */
 int ag_mesh_read(const ag_mesh* m, float* positions, uint32_t* triangles);
+/* What framing a shot takes from a mesh, without copying the mesh back: the
+ * vertex bounds (lo, hi: 3 floats each; zeros for an empty mesh) and every
+ * stride-th vertex position (vertices 0, stride, 2 stride, ... below the
+ * vertex count, at most cap of them, into hull_xyz; *n_hull = how many).
+ * Replaces Prepared's host pass over the positions (shot.rs:73-147: radius
+ * and floor height from the bounds, the framing hull from a subsample). */
+int ag_mesh_frame_sample(const ag_mesh* m, uint32_t stride, float* hull_xyz, uint32_t cap, uint32_t* n_hull,
+                         float* lo, float* hi);
 /* The mesh's device arrays, valid until the next call that changes the mesh. */
 int ag_mesh_device_arrays(const ag_mesh* m, const float** positions, const uint32_t** triangles);
 /* weld (weld.rs:38-62): merges bit-identical vertices, then vertices within

@@ -74,3 +74,26 @@ def test_out_of_range_index_rejected(ctx):
     with pytest.raises(ag.ArgentError):
         ag.Scene.from_mesh(ctx, m, desc)
     m.close()
+
+
+def test_frame_sample_equals_host_framing(ctx):
+    """ag_mesh_frame_sample gives Piece's bounds and hull exactly, and the
+    framed camera is the one framed from the positions on the host."""
+    pos, tris, _ = synth.signet(seed=2, band_nu=128, band_nv=32, plate_nt=48, plate_np=96)
+    m = ag.Mesh.from_arrays(ctx, pos, tris)
+    hull, lo, hi = m.frame_sample()
+    host = workloads.Piece(pos, tris)
+    assert np.array_equal(lo, pos.min(0)) and np.array_equal(hi, pos.max(0))
+    assert np.array_equal(hull, host.hull)
+    dev = workloads.Piece.from_frame(lo, hi, hull)
+    assert dev.radius == host.radius and dev.floor_y == host.floor_y
+    for aspect in (1.0, 540 / 960):
+        ca, ta = host.shot("hero", 100.0, 0.0, aspect)
+        cb, tb = dev.shot("hero", 100.0, 0.0, aspect)
+        assert ta == tb and bytes(ca) == bytes(cb)
+    for stride in (1, 7, len(pos) + 3):
+        h, lo2, hi2 = m.frame_sample(stride)
+        assert np.array_equal(h, pos[::stride]) and np.array_equal(lo2, lo) and np.array_equal(hi2, hi)
+    with pytest.raises(ag.ArgentError):
+        m.frame_sample(0)
+    m.close()
