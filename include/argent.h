@@ -615,6 +615,10 @@ int ag_ctx_stats(const ag_ctx* ctx, ag_stats* out);
  * extend, shade and shadow kernels of that depth; any pointer may be null. */
 int ag_ctx_depth_stats(const ag_ctx* ctx, int32_t n, uint64_t* extension_rays, uint64_t* shadow_rays,
                        double* ms_extend, double* ms_shade, double* ms_shadow);
+/* Per-stage event timing for synchronous render passes (off by default).
+ * With it on, every stage keeps a launch of its own so that it can be timed;
+ * with it off, the shadow rays of each bounce and the extension rays of the
+ * next run as one launch.  Films are the same either way. */
 int ag_ctx_enable_timing(ag_ctx* ctx, int32_t on);
 
 #ifdef __cplusplus
