@@ -94,6 +94,7 @@ def _cpu_model():
 
 WORKLOAD_NAMES = {"signet": "signet still", "torus": "torus ring still", "gems": "signet with 12 brilliants still"}
 EXTEND_KERNEL = "ag::k_extend("
+FUSED_KERNEL = "ag::k_shadow_extend("
 COPY_KERNEL = "neg_kernel"
 
 
@@ -122,8 +123,9 @@ def _pmc_rows(d, counter):
         with open(f) as fh:
             for r in csv.DictReader(fh):
                 if r["Counter_Name"] == counter:
-                    rows.append((r["Kernel_Name"], float(r["Counter_Value"]), int(r.get("Grid_Size", 0) or 0)))
-    return rows
+                    rows.append((int(r.get("Dispatch_Id", 0) or 0), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return [(k, v) for _, k, v in rows]
 
 
 def reduce_traffic(fetch_dir, write_dir, mib):
@@ -133,8 +135,12 @@ def reduce_traffic(fetch_dir, write_dir, mib):
     out = {}
     for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         rows = _pmc_rows(d, name)
-        cal = [v for k, v, _ in rows if COPY_KERNEL in k]
-        ext = [v for k, v, _ in rows if k.startswith(EXTEND_KERNEL)]
+        cal = [v for k, v in rows if COPY_KERNEL in k]
+        # the stage-timed pass comes last and alone runs every depth's
+        # k_extend as a launch of its own: its dispatches follow the last
+        # fused shadow/extend launch of the timed passes
+        fused = [i for i, (k, _) in enumerate(rows) if k.startswith(FUSED_KERNEL)]
+        ext = [v for k, v in rows[(fused[-1] + 1 if fused else 0):] if k.startswith(EXTEND_KERNEL)]
         if not cal or not ext:
             raise SystemExit(f"{name}: calibration dispatches {len(cal)}, k_extend dispatches {len(ext)}")
         known_kib = mib * 1024.0
