@@ -395,6 +395,11 @@ class Film:
         return self._read(A.FILM_NORMAL)
 
     def xyz(self):
+        """Mean colour in linear CIE XYZ: the inverse of the camera matrix
+        (color.rs:48-55) applied to the white-balanced linear sRGB mean."""
+        return self._read(A.FILM_XYZ)
+
+    def xyz(self):
         return self._read(A.FILM_XYZ)
 
     def sums(self):

@@ -41,11 +41,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """Threads for the CPU legs: every CPU this process may run on, at most 16
+    (the host share that goes with one GPU on the measurement machine)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return min(16, avail)
+
+
+def host_info():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = None
+    return {"nproc": os.cpu_count(), "affinity": avail, "threads_used": host_threads(), "cpu_model": _cpu_model(),
+            "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off (oracle/Makefile)"}
+
+
 def cpu_baseline(piece_desc, cam, w, h, spp, ps, seed, budget_s):
-    """The oracle on the host's cores over evenly spaced blocks of rows of the
-    same image (same per-pixel seeds), until about budget_s of work."""
+    """The oracle on the host's cores over evenly spaced rows of the same
+    image (same per-pixel seeds), until about budget_s of work.  Each call
+    renders one row with the pixels dealt to every thread."""
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     osc = O.OracleScene(piece_desc)
     sums = np.zeros((3, h, w, 3), np.float32)
     blocks = 32
@@ -79,6 +99,7 @@ def cpu_baseline(piece_desc, cam, w, h, spp, ps, seed, budget_s):
         "per_core": round(msps / threads, 4),
         "segments_per_sample": round(segs / max(done_samples, 1), 4),
         "cpu_model": _cpu_model(),
+        "host": host_info(),
     }
 
 
@@ -255,7 +276,7 @@ def ray_batch(args):
     if not args.no_cpu:
         import oracle as O
         osc = O.OracleScene(desc)
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         # The whole batch, passed over again until the CPU has worked about
         # --cpu-seconds (one pass takes well under a second on 16 cores).
         done, passes, t0 = 0, 0, time.perf_counter()
@@ -316,7 +337,7 @@ def cavity_ao(args):
     sc.close()
     if not args.no_cpu:
         import oracle as O
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         osc = O.OracleScene(desc)
         passes, t0 = 0, time.perf_counter()
         while True:  # the whole mesh, again until about --cpu-seconds of work
@@ -522,7 +543,7 @@ def live_view(args):
            "timing": "wall clock from ag_live_set to the end of the step that published the frame, host bytes in hand"}
     if not args.no_cpu:
         import oracle as O
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         osc = O.OracleScene(desc)
         hw, hh = w // 2, h // 2
         sums = np.zeros((3, hh, hw, 3), np.float32)
@@ -587,7 +608,7 @@ def bvh_build(args):
                                   "rays: 2^24 closest-hit batch", "triangles": int(len(tris))},
            "gpu_bvh": g, "host_sah": h, "build_speedup": round(h["scene_ms"] / g["scene_ms"], 1),
            "traversal_ratio": round(g["rays_grays_per_s"] / h["rays_grays_per_s"], 3),
-           "cpu_baseline": {"value": h["scene_ms"], "unit": "ms", "cores": min(16, os.cpu_count() or 1),
+           "cpu_baseline": {"value": h["scene_ms"], "unit": "ms", "cores": host_threads(),
                             "kind": "port", "sample": "the host binned-SAH build of this scene once",
                             "cpu_model": _cpu_model()}}
     print(json.dumps(out), flush=True)
@@ -653,7 +674,7 @@ def new_mesh(args):
            "stages_ms": med}
     if not args.no_cpu:
         import oracle as O
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         t0 = time.perf_counter()
         wp, wt, _ = O.load_stl(data)
         wq, wn, ws, _ = O.normals(wp, wt, 50.0)

@@ -15,6 +15,7 @@ from argent_amd import _abi as A
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 MODE_REFERENCE, MODE_ROBUST = 0, 1
+STRATEGY_MIS, STRATEGY_NEE, STRATEGY_BSDF, STRATEGY_DOUBLE = 0, 1, 2, 3
 
 _f = C.POINTER(C.c_float)
 _u32 = C.POINTER(C.c_uint32)
@@ -40,6 +41,7 @@ def lib():
             ("or_scene_set_turntable", None, [C.c_void_p, C.c_float]),
             ("or_scene_set_material", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(A.Material)]),
             ("or_scene_stats", C.c_int, [C.c_void_p, _u32, _u32]),
+            ("or_scene_set_strategy", C.c_int, [C.c_void_p, C.c_int32]),
             ("or_intersect", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _f, _f, _f, _u32]),
             ("or_occluded", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, C.c_int32, _u8]),
             ("or_intersect_stats", C.c_int, [C.c_void_p, *rays, C.c_uint32, C.c_int32, _u64]),
@@ -117,6 +119,11 @@ class OracleScene:
 
     def set_material(self, m, mat):
         assert lib().or_scene_set_material(self.h, m, C.byref(mat)) == 0
+
+    def set_strategy(self, strategy):
+        """STRATEGY_MIS (the reference's), STRATEGY_NEE, STRATEGY_BSDF, or
+        STRATEGY_DOUBLE (a planted double count the closed forms must reject)."""
+        assert lib().or_scene_set_strategy(self.h, strategy) == 0
 
     def intersect(self, o, d, tmax=None, mode=MODE_ROBUST, threads=0):
         cols, tm, n = _rays(o, d, tmax)

@@ -1484,6 +1484,16 @@ struct Scene {
     std::vector<float> light_cdf;
     float env_pick = 0.0f;
     std::vector<float> env_tabs;  // top, horizon, bottom copies
+    // Estimator strategy, a checking knob of the oracle only (the reference
+    // always combines NEE and BSDF sampling by the power heuristic,
+    // integrator.rs:70-98, :130-178): 0 = that MIS; 1 = light sampling
+    // alone (emitters reached by BSDF sampling weigh 0, NEE weighs 1);
+    // 2 = BSDF sampling alone (no NEE contribution, emitters hit weigh 1);
+    // 3 = both unweighted, a deliberate double count that the closed-form
+    // tests must reject.
+    int strategy = 0;
+    float bsdf_hit_weight(float w) const { return strategy == 0 ? w : strategy == 1 ? 0.0f : 1.0f; }
+    float nee_weight(float w) const { return strategy == 0 ? w : strategy == 2 ? 0.0f : 1.0f; }
 
     // scene.rs:88-110
     void rebuild_lights() {
@@ -1645,7 +1655,7 @@ Spec trace(const Scene& scene, Ray ray, Wavelengths& wl, Sampler& s, const ag_pa
         auto env_weight = [&](V3 dir) {
             if (scene.env_map && !specular) {
                 float pl = scene.env_pick * scene.map.pdf(dir);
-                return prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl);
+                return scene.bsdf_hit_weight(prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl));
             }
             return 1.0f;
         };
@@ -1665,7 +1675,7 @@ Spec trace(const Scene& scene, Ray ray, Wavelengths& wl, Sampler& s, const ag_pa
                 else {
                     float cos_l = std::fmax(std::fabs((-ray.d).dot(panel.n)), 1e-8f);
                     float p_light = (1.0f - scene.env_pick) * scene.light_pick_pdf(hit.index) * hit.t * hit.t / (cos_l * panel.area);
-                    float w = prev_pdf * prev_pdf / (prev_pdf * prev_pdf + p_light * p_light);
+                    float w = scene.bsdf_hit_weight(prev_pdf * prev_pdf / (prev_pdf * prev_pdf + p_light * p_light));
                     add_clamped(l, beta * le * w, clamp, ps.clamp_indirect);
                 }
             }
@@ -1721,7 +1731,7 @@ Spec trace(const Scene& scene, Ray ray, Wavelengths& wl, Sampler& s, const ag_pa
                     if (!f.is_black()) {
                         float p_light = scene.env_pick * pdf_dir;
                         float p_bsdf = bsdf.pdf(wo, wi);
-                        float w = p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf);
+                        float w = scene.nee_weight(p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf));
                         cnt.shadows++;
                         float vis = scene.sky_visibility(offset_origin(p, ngu, dir), dir, robust);
                         if (vis > 0.0f) {
@@ -1748,7 +1758,7 @@ Spec trace(const Scene& scene, Ray ray, Wavelengths& wl, Sampler& s, const ag_pa
                         if (!f.is_black()) {
                             float p_light = pick * d2 / (cos_l * panel.area);
                             float p_bsdf = bsdf.pdf(wo, wi);
-                            float w = p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf);
+                            float w = scene.nee_weight(p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf));
                             V3 origin = offset_origin(p, ngu, dir);
                             cnt.shadows++;
                             if (scene.visible(origin, lp, li, robust)) {
@@ -1884,6 +1894,11 @@ extern "C" void or_scene_set_turntable(or_scene* s, float deg) {
 extern "C" int or_scene_set_material(or_scene* s, int32_t m, const ag_material* mat) {
     if (!s || !mat || m < 0 || (size_t)m >= s->sc.materials.size()) return 1;
     s->sc.materials[m] = *mat;
+    return 0;
+}
+extern "C" int or_scene_set_strategy(or_scene* s, int32_t strategy) {
+    if (!s || strategy < 0 || strategy > 3) return 1;
+    s->sc.strategy = strategy;
     return 0;
 }
 extern "C" int or_scene_stats(const or_scene* s, uint32_t* bn, uint32_t* wn) {
@@ -2052,10 +2067,13 @@ extern "C" int or_render_pass(const or_scene* s, const ag_camera* cam, uint32_t 
     camera_matrix(S.has_white, S.white, cam_m);
     const size_t plane = (size_t)w * h * 3;
     std::atomic<uint64_t> segs{0}, shadows{0};
-    parallel_for(row1 - row0, threads, [&](uint32_t ra, uint32_t rb) {
+    // pixels, not rows, are dealt to the threads, so that a pass over a few
+    // rows still runs on every thread (each pixel's sum is independent)
+    parallel_for((row1 - row0) * w, threads, [&](uint32_t pa, uint32_t pb) {
         Counters cnt;
-        for (uint32_t y = row0 + ra; y < row0 + rb; y++) {
-            for (uint32_t x = 0; x < w; x++) {
+        for (uint32_t pi = pa; pi < pb; pi++) {
+            {
+                const uint32_t y = row0 + pi / w, x = pi % w;
                 uint32_t pix_seed = hash32((y * w + x) ^ seed_h);
                 float acc[3] = {0, 0, 0}, aacc[3] = {0, 0, 0}, nacc[3] = {0, 0, 0};
                 for (uint32_t si = first; si < first + count; si++) {

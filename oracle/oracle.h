@@ -35,6 +35,11 @@ void or_scene_destroy(or_scene* s);
 void or_scene_set_turntable(or_scene* s, float deg);
 int or_scene_set_material(or_scene* s, int32_t m, const ag_material* mat);
 int or_scene_stats(const or_scene* s, uint32_t* binary_nodes, uint32_t* wide_nodes);
+/* Estimator strategy (checking knob, not in the reference): 0 = NEE + BSDF
+ * sampling combined by the power heuristic as integrator.rs does; 1 = NEE
+ * alone; 2 = BSDF sampling alone; 3 = both unweighted (a planted double
+ * count). */
+int or_scene_set_strategy(or_scene* s, int32_t strategy);
 
 int or_intersect(const or_scene* s, const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
                  const float* dz, const float* tmax, uint32_t n, int32_t mode, int32_t threads, float* t, float* u,
