@@ -485,7 +485,8 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
     RaySupply R{n, cursors, 0u, 0u, blockIdx.x % CURSOR_SLICES, 0u, grab};
     // wavefronts beyond what n rays can occupy (`grab` each) leave at once
     uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
-    bool more = wave < (n + grab - 1u) / grab;  // uniform across the wavefront
+    // uniform across the wavefront; 64-bit so that n near 2^32 cannot wrap
+    bool more = (uint64_t)wave < ((uint64_t)n + grab - 1u) / grab;
     // A lane whose ray has finished traversing stays parked (idx kept, L.cur
     // == TRAV_DONE) until the next refill, when all parked lanes are finished
     // in one call: a job's finish reads memory (a material id, a radiance
