@@ -228,6 +228,8 @@ _SIGS = [
     ("ag_ambient_occlusion", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, f32p]),
     ("ag_intersect_device", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32,
                                       C.POINTER(HitsSoA)]),
+    ("ag_intersect_device_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32,
+                                              C.POINTER(HitsSoA)]),
     ("ag_occluded_device", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, u8p]),
     ("ag_bsdf_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, i32p, f32p, f32p, f32p, f32p, f32p, f32p]),
     ("ag_render_views", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Material), C.c_int32, C.POINTER(View),

@@ -68,6 +68,22 @@ uint32_t hash32_host(uint32_t x) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ allocation-free ray queries
+// ag_intersect_device_noalloc runs the closest-hit traversal with its scratch
+// -- the per-XCD ray cursors and the per-thread stack spill areas -- in this
+// one array instead of library-allocated buffers.  The array is sized for a
+// traversal grid of at most NOALLOC_BLOCKS blocks (the kernel pulls rays, so a
+// smaller grid serves any batch size); every device that loads the library
+// holds its own copy until the process ends.  Host code never uses the
+// array's name as an address: each device's address comes from
+// hipGetSymbolAddress at context creation (noalloc_scratch_setup).
+constexpr uint32_t NOALLOC_BLOCKS = 512;
+constexpr size_t NOALLOC_CURSOR_BYTES = 4096;  // CURSOR_SET u32 (1 KiB), padded
+constexpr size_t NOALLOC_SPILL_BYTES = (size_t)NOALLOC_BLOCKS * TRAV_BLOCK * STACK_SPILL * sizeof(uint2);  // 96 MiB
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_CURSOR_BYTES + NOALLOC_SPILL_BYTES;
+static_assert(CURSOR_SET * sizeof(uint32_t) <= NOALLOC_CURSOR_BYTES, "cursor area too small");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 struct ag_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -103,6 +119,7 @@ struct ag_ctx {
     DevBuf post_buf;  // preview development: images, à-trous work, grade tables, 8-bit output
     DevBuf energy_buf;  // the library's GGX energy table, staged into LDS by the shading kernel
     double last_query_ms = 0.0;
+    void* noalloc_scratch = nullptr;  // this device's address of `scratch`
     hipEvent_t next_event() {
         if (ev_used == evpool.size()) {
             hipEvent_t e;
@@ -179,6 +196,15 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     c->device = device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) c->n_cu = prop.multiProcessorCount;
+    {
+        // this device's copy of the allocation-free queries' scratch array
+        // (the device is current); its address is kept per context
+        hipError_t e = hipGetSymbolAddress(&c->noalloc_scratch, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !c->noalloc_scratch) {
+            delete c;
+            return fail(AG_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(AG_E_DEVICE, "stream creation failed");
@@ -2031,6 +2057,37 @@ extern "C" int ag_intersect_device(ag_ctx* ctx, const ag_scene* s, const ag_rays
 extern "C" int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n, uint8_t* out) {
     if (!out) return fail(AG_E_INVALID, "null output");
     return ray_batch(ctx, s, rays, n, nullptr, out, true);
+}
+
+// The closest-hit query of ag_intersect_device with no allocation: the
+// scratch is the device's `scratch` array (cursors, then spill stacks for
+// NOALLOC_BLOCKS blocks).  Enqueues a fill kernel for the cursors and the
+// traversal kernel on the context stream and returns; no hipMalloc, hipFree,
+// memset or synchronisation.  Calls that overlap on one device race on the
+// array (it holds the last call's cursors and stacks).
+extern "C" int ag_intersect_device_noalloc(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, uint32_t n,
+                                           ag_hits_soa* out) {
+    if (!ctx || !s || !rays || !out) return fail(AG_E_INVALID, "null argument");
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+        return fail(AG_E_INVALID, "missing ray component");
+    if (!out->t || !out->u || !out->v || !out->prim) return fail(AG_E_INVALID, "missing hit component");
+    if (!ctx->noalloc_scratch) return fail(AG_E_STATE, "context has no scratch address for this device");
+    if (n == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int want = ctx->n_cu * (int)ctx->tuning.batch_blocks_per_cu;
+    const int grid = std::max(1, std::min(want, (int)NOALLOC_BLOCKS));
+    if ((size_t)grid * TRAV_BLOCK * STACK_SPILL * sizeof(uint2) > NOALLOC_SPILL_BYTES)
+        return fail(AG_E_INVALID, "traversal grid exceeds the scratch array");
+    unsigned char* base = static_cast<unsigned char*>(ctx->noalloc_scratch);
+    uint32_t* cursors = reinterpret_cast<uint32_t*>(base);
+    uint2* spill = reinterpret_cast<uint2*>(base + NOALLOC_CURSOR_BYTES);
+    const RaysSoA r{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+    const HitsSoA h{out->t, out->u, out->v, out->prim};
+    launch_zero_u32(ctx->stream, cursors, CURSOR_SET);
+    launch_rays_intersect(ctx->stream, grid, s->geo->nodes.as<float4>(), s->geo->tris.as<float4>(), r, h, n, cursors,
+                          ctx->tuning.batch_refill_min, ctx->tuning.spread_min_lanes, spill);
+    HIPCHK(hipGetLastError());
+    return AG_OK;
 }
 
 extern "C" int ag_bsdf_probe(ag_ctx* ctx, const ag_scene* s, uint32_t n, const int32_t* mat, const float* lam,

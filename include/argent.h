@@ -314,6 +314,18 @@ int ag_intersect_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_
                         ag_hits_soa* out_dev);
 int ag_occluded_device(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_dev, uint32_t n,
                        uint8_t* out_dev);
+/* ag_intersect_device without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * traversal's scratch -- ray cursors and per-thread stack spill areas -- lives
+ * in one static device array of 96 MiB + 4 KiB per device, sized for a grid
+ * of at most 512 traversal blocks; any n fits, since the kernel pulls rays.
+ * Results equal ag_intersect_device's bit for bit.  Enqueues on the context
+ * stream and returns without waiting (ag_ctx_synchronize waits).  Every call
+ * on a device shares that device's array: calls that overlap on one device
+ * (other contexts, other threads) race and must be serialised by the caller. */
+int ag_intersect_device_noalloc(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays_dev, uint32_t n,
+                                ag_hits_soa* out_dev);
 
 /* Cavity ambient occlusion of the scene's piece: ambient_occlusion
  * (argent-mesh cavity.rs:95-128) with the piece as its Occluder (argent-core
