@@ -144,7 +144,7 @@ class RequestStatus(C.Structure):
                 ("t_first_preview", C.c_double), ("t_done", C.c_double)]
 
 
-REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_REJECTED = range(4)
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_REJECTED, REQ_FAILED = range(5)
 
 
 class Tuning(C.Structure):
@@ -242,6 +242,8 @@ _SIGS = [
     ("ag_server_status", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStatus)]),
     ("ag_server_read", C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, f32p]),
     ("ag_server_release", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("ag_server_error", C.c_char_p, [C.c_void_p, C.c_uint64]),
+    ("ag_server_budget", C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     ("ag_device_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     ("ag_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("ag_memcpy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -1256,16 +1256,27 @@ extern "C" int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_mate
 // ------------------------------------------------------------------ server
 struct ag_server {
     ag_ctx* ctx = nullptr;
-    size_t budget = 0, used = 0;
+    // HBM accounting: films of admitted requests, plus the device bytes of
+    // every scene look and piece (geometry) they render, each charged once
+    // while at least one admitted request uses it, plus the context's wave
+    // state for the largest material-key count admitted so far
+    size_t budget = 0, films = 0, scenes = 0;
+    int max_keys = 2;
+    std::map<const ag_scene*, int> scene_refs;
+    std::map<const Geometry*, int> geo_refs;
     uint64_t step_samples = 1u << 24;
     ag_path_settings ps{};
     std::chrono::steady_clock::time_point t0;
+    uint64_t steps = 0;
+    int preview_run = 0;  // consecutive steps that carried only first previews
     struct Req {
         ag_request r;
         ag_request_status st;
         ag_film* film = nullptr;
         uint32_t next_chunk = 0;
-        size_t cost = 0;
+        size_t cost = 0;         // film bytes
+        uint64_t last_step = 0;  // step that last rendered a chunk of it
+        std::string error;       // why it failed (AG_REQ_FAILED)
     };
     std::map<uint64_t, Req> reqs;
     std::vector<uint64_t> queue;   // FIFO of queued ids
@@ -1274,9 +1285,138 @@ struct ag_server {
     double now() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
+namespace {
+
+// At most this many steps in a row carry only requests awaiting their first
+// preview; the next one renders the requests in progress too (least recently
+// rendered first), so a stream of arrivals cannot stall them.
+constexpr int SERVER_PREVIEW_RUN_MAX = 4;
+
+// Device bytes of a context's wave state (ensure_wave, ensure_spill) for
+// n_keys material keys: path state, ray / hit / shadow queues, buckets,
+// counters, segments and the traversal spill stacks.
+size_t wave_state_bytes(const ag_ctx* c, int n_keys, uint32_t max_depth) {
+    const size_t cap = c->tuning.paths_per_wave;
+    const size_t per_path = 6 * sizeof(float4) + sizeof(float) + 4 * sizeof(uint32_t) + sizeof(uint64_t);
+    size_t b = cap * (per_path + 4 * sizeof(float4) + sizeof(float4) + 4 * sizeof(float4) + (size_t)n_keys * 4);
+    b += (counter_words(max_depth) + (size_t)(max_depth + 2) * CURSORS_PER_DEPTH) * sizeof(uint32_t);
+    b += MAX_SEGMENTS * sizeof(DSeg);
+    const size_t blocks = (size_t)c->n_cu * std::max(c->tuning.blocks_per_cu, c->tuning.batch_blocks_per_cu);
+    b += blocks * TRAV_BLOCK * STACK_SPILL * sizeof(uint2);
+    return b;
+}
+
+size_t film_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * 9 * sizeof(float) * 2; }  // sums + pass accumulators
+
+// Device bytes a request adds to the server's account when admitted.
+size_t admit_bytes(const ag_server* s, const ag_server::Req& q) {
+    const ag_scene* sc = q.r.scene;
+    size_t b = q.cost;
+    if (!s->scene_refs.count(sc)) b += sc->bytes;
+    if (!s->geo_refs.count(sc->geo.get())) b += sc->geo->bytes();
+    return b;
+}
+
+size_t server_used(const ag_server* s, int n_keys) {
+    return s->films + s->scenes + wave_state_bytes(s->ctx, n_keys, s->ps.max_depth);
+}
+
+void server_charge(ag_server* s, ag_server::Req& q) {
+    const ag_scene* sc = q.r.scene;
+    if (s->scene_refs[sc]++ == 0) s->scenes += sc->bytes;
+    if (s->geo_refs[sc->geo.get()]++ == 0) s->scenes += sc->geo->bytes();
+    s->films += q.cost;
+    s->max_keys = std::max(s->max_keys, sc->n_keys);
+}
+
+// Frees the request's film and returns its bytes to the budget.
+void server_uncharge(ag_server* s, ag_server::Req& q) {
+    if (!q.film) return;
+    ag_film_destroy(q.film);
+    q.film = nullptr;
+    s->films -= q.cost;
+    const ag_scene* sc = q.r.scene;
+    if (--s->scene_refs[sc] == 0) {
+        s->scene_refs.erase(sc);
+        s->scenes -= sc->bytes;
+    }
+    if (--s->geo_refs[sc->geo.get()] == 0) {
+        s->geo_refs.erase(sc->geo.get());
+        s->scenes -= sc->geo->bytes();
+    }
+}
+
+void server_fail(ag_server* s, uint64_t k, const std::string& why) {
+    auto& q = s->reqs[k];
+    server_uncharge(s, q);
+    q.st.state = AG_REQ_FAILED;
+    q.st.t_done = s->now();
+    q.error = why;
+    s->active.erase(std::remove(s->active.begin(), s->active.end(), k), s->active.end());
+    s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), k), s->queue.end());
+}
+
+// One shared wavefront schedule over the next chunk of each request in `set`.
+int server_render(ag_server* s, const std::vector<uint64_t>& set) {
+    ag_ctx* ctx = s->ctx;
+    std::vector<ag_material> mats;
+    std::vector<ViewJob> jobs;
+    for (uint64_t k : set) {
+        auto& q = s->reqs[k];
+        ViewJob j;
+        j.film = q.film;
+        j.count = q.next_chunk;
+        j.first = q.film->samples;
+        int m0 = -1;
+        if (q.r.has_material0) {
+            m0 = (int)mats.size();
+            mats.push_back(q.r.material0);
+        }
+        make_dview(j.dv, q.r.scene, &q.r.camera, q.r.turntable_deg, m0, q.film, q.r.seed);
+        j.scene = q.r.scene;
+        jobs.push_back(j);
+    }
+    int rc = upload_view_materials(ctx, mats.data(), (int32_t)mats.size());
+    if (rc) return rc;
+    return run_views(ctx, jobs[0].scene, jobs, &s->ps, ctx->vmat_buf.as<DMat>());
+}
+
+// A failed pass leaves film sums and sample counts as they were (the commit
+// into the sums is its last kernel and the count moves only on success), but
+// its accumulators may hold part of the pass: clear them before a retry.
+void server_clear_acc(ag_server* s, const std::vector<uint64_t>& set) {
+    for (uint64_t k : set) {
+        ag_film* f = s->reqs[k].film;
+        launch_zero_u32(s->ctx->stream, f->acc.as<uint32_t>(), (uint32_t)((size_t)f->w * f->h * 9));
+    }
+    (void)hipStreamSynchronize(s->ctx->stream);
+}
+
+void server_progress(ag_server* s, const std::vector<uint64_t>& set) {
+    double t = s->now();
+    for (uint64_t k : set) {
+        auto& q = s->reqs[k];
+        q.st.samples = q.film->samples;
+        q.last_step = s->steps;
+        if (q.st.t_first_preview < 0.0) q.st.t_first_preview = t;
+        uint32_t rest = q.r.spp - q.st.samples;
+        if (rest == 0) {
+            q.st.state = AG_REQ_DONE;
+            q.st.t_done = t;
+            continue;
+        }
+        uint64_t px = (uint64_t)q.r.width * q.r.height;
+        uint32_t cap = (uint32_t)std::max<uint64_t>(1, s->step_samples / px);
+        q.next_chunk = std::min(rest, std::max(q.next_chunk, std::min(cap, q.next_chunk * 2)));
+    }
+}
+
+}  // namespace
+
 extern "C" int ag_server_create(ag_ctx* ctx, size_t budget, uint64_t step_samples, const ag_path_settings* ps,
                                 ag_server** out) {
     if (!ctx || !ps || !out || step_samples == 0) return fail(AG_E_INVALID, "bad arguments");
+    if (ps->max_depth > 64) return fail(AG_E_INVALID, "max_depth above 64");
     auto* s = new ag_server();
     s->ctx = ctx;
     s->budget = budget;
@@ -1298,6 +1438,7 @@ extern "C" int ag_server_submit(ag_server* s, const ag_request* r, uint64_t* id)
     if (!s || !r || !id) return fail(AG_E_INVALID, "null argument");
     if (!r->scene || r->scene->ctx != s->ctx) return fail(AG_E_INVALID, "request scene missing or from another context");
     if (r->width == 0 || r->height == 0 || r->spp == 0) return fail(AG_E_INVALID, "empty request");
+    if ((uint64_t)r->width * r->height > 0xFFFFFFFFull) return fail(AG_E_INVALID, "more than 2^32 - 1 pixels");
     if (r->has_material0 && (r->material0.kind < AG_MAT_METAL || r->material0.kind > AG_MAT_OPAL ||
                              r->material0.kind == AG_MAT_GEM))
         return fail(AG_E_INVALID, "bad material override");
@@ -1308,10 +1449,13 @@ extern "C" int ag_server_submit(ag_server* s, const ag_request* r, uint64_t* id)
     q.st.samples = 0;
     q.st.t_submit = s->now();
     q.st.t_admit = q.st.t_first_preview = q.st.t_done = -1.0;
-    q.cost = (size_t)r->width * r->height * 9 * sizeof(float) * 2;
+    q.cost = film_bytes(r->width, r->height);
     q.next_chunk = std::min(q.r.preview_spp, q.r.spp);
     uint64_t k = s->next_id++;
-    if (q.cost > s->budget) q.st.state = AG_REQ_REJECTED;
+    // a request that could not be admitted even into an idle server
+    const size_t alone = q.cost + r->scene->bytes + r->scene->geo->bytes() +
+                         wave_state_bytes(s->ctx, std::max(2, r->scene->n_keys), s->ps.max_depth);
+    if (alone > s->budget) q.st.state = AG_REQ_REJECTED;
     s->reqs[k] = q;
     if (q.st.state == AG_REQ_QUEUED) s->queue.push_back(k);
     *id = k;
@@ -1321,81 +1465,97 @@ extern "C" int ag_server_submit(ag_server* s, const ag_request* r, uint64_t* id)
 extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queued) {
     if (!s) return fail(AG_E_INVALID, "null server");
     ag_ctx* ctx = s->ctx;
-    // admit FIFO while the films fit
+    // admit FIFO while the request's film and any scene it brings fit
     while (!s->queue.empty()) {
-        auto& q = s->reqs[s->queue.front()];
-        if (s->used + q.cost > s->budget) break;
-        int rc = ag_film_create(ctx, q.r.width, q.r.height, &q.film);
-        if (rc) return rc;
-        s->used += q.cost;
+        const uint64_t k = s->queue.front();
+        auto& q = s->reqs[k];
+        const int nk = std::max(s->max_keys, q.r.scene->n_keys);
+        if (server_used(s, nk) + admit_bytes(s, q) > s->budget) break;
+        size_t free_b = 0, total_b = 0;
+        if (hipSetDevice(ctx->device) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+            q.cost > free_b) {
+            server_fail(s, k, "film of " + std::to_string(q.cost) + " bytes exceeds the device's free memory (" +
+                                  std::to_string(free_b) + " bytes)");
+            continue;
+        }
+        if (ag_film_create(ctx, q.r.width, q.r.height, &q.film)) {
+            q.film = nullptr;
+            server_fail(s, k, std::string("film creation failed: ") + ag_last_error());
+            continue;
+        }
+        server_charge(s, q);
         q.st.state = AG_REQ_ACTIVE;
         q.st.t_admit = s->now();
-        s->active.push_back(s->queue.front());
+        s->active.push_back(k);
         s->queue.erase(s->queue.begin());
     }
     if (!s->active.empty()) {
+        s->steps++;
         // choose: no preview yet first, then least work left, within the step budget
         std::vector<uint64_t> order = s->active;
         auto left = [&](uint64_t k) {
             const auto& q = s->reqs[k];
             return (uint64_t)(q.r.spp - q.st.samples) * q.r.width * q.r.height;
         };
+        auto waiting = [&](uint64_t k) { return s->reqs[k].st.t_first_preview < 0.0; };
         std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
-            bool pa = s->reqs[a].st.t_first_preview < 0.0, pb = s->reqs[b].st.t_first_preview < 0.0;
+            bool pa = waiting(a), pb = waiting(b);
             if (pa != pb) return pa;
             return left(a) < left(b);
         });
-        std::vector<uint64_t> chosen;
-        uint64_t room = s->step_samples;
         // A step with requests still waiting for their first preview renders
         // only those, so the previews are published after a short step rather
         // than a full one.  The others sit the step out; each request's chunk
         // sequence, and so its image, does not depend on the steps it skips.
-        const bool preview_step = s->reqs[order.front()].st.t_first_preview < 0.0;
+        // After SERVER_PREVIEW_RUN_MAX such steps in a row the requests in
+        // progress go first, least recently rendered first.
+        bool preview_step = waiting(order.front());
+        if (preview_step && s->preview_run >= SERVER_PREVIEW_RUN_MAX) {
+            std::vector<uint64_t> prog;
+            for (uint64_t k : order)
+                if (!waiting(k)) prog.push_back(k);
+            if (!prog.empty()) {
+                std::stable_sort(prog.begin(), prog.end(),
+                                 [&](uint64_t a, uint64_t b) { return s->reqs[a].last_step < s->reqs[b].last_step; });
+                for (uint64_t k : order)
+                    if (waiting(k)) prog.push_back(k);
+                order = prog;
+                preview_step = false;
+            }
+        }
+        s->preview_run = preview_step ? s->preview_run + 1 : 0;
+        std::vector<uint64_t> chosen;
+        uint64_t room = s->step_samples;
         for (uint64_t k : order) {
             const auto& q = s->reqs[k];
-            if (preview_step && q.st.t_first_preview >= 0.0) break;
+            if (preview_step && !waiting(k)) break;
             uint64_t cost = (uint64_t)q.next_chunk * q.r.width * q.r.height;
             if (!chosen.empty() && cost > room) continue;
             chosen.push_back(k);
             room = cost >= room ? 0 : room - cost;
             if (room == 0) break;
         }
-        std::vector<ag_material> mats;
-        std::vector<ViewJob> jobs;
-        for (uint64_t k : chosen) {
-            auto& q = s->reqs[k];
-            ViewJob j;
-            j.film = q.film;
-            j.count = q.next_chunk;
-            j.first = q.film->samples;
-            int m0 = -1;
-            if (q.r.has_material0) {
-                m0 = (int)mats.size();
-                mats.push_back(q.r.material0);
+        if (server_render(s, chosen) == AG_OK) {
+            server_progress(s, chosen);
+        } else {
+            // one request must not fail the others: retry each alone, fail
+            // those that fail alone (with the pass's error), keep the rest
+            const std::string first_error = ag_last_error();
+            server_clear_acc(s, chosen);
+            if (chosen.size() == 1) {
+                server_fail(s, chosen[0], first_error);
+            } else {
+                for (uint64_t k : chosen) {
+                    std::vector<uint64_t> one{k};
+                    if (server_render(s, one) == AG_OK) {
+                        server_progress(s, one);
+                    } else {
+                        std::string why = ag_last_error();
+                        server_clear_acc(s, one);
+                        server_fail(s, k, why);
+                    }
+                }
             }
-            make_dview(j.dv, q.r.scene, &q.r.camera, q.r.turntable_deg, m0, q.film, q.r.seed);
-            j.scene = q.r.scene;
-            jobs.push_back(j);
-        }
-        int rc = upload_view_materials(ctx, mats.data(), (int32_t)mats.size());
-        if (rc) return rc;
-        rc = run_views(ctx, jobs[0].scene, jobs, &s->ps, ctx->vmat_buf.as<DMat>());
-        if (rc) return rc;
-        double t = s->now();
-        for (uint64_t k : chosen) {
-            auto& q = s->reqs[k];
-            q.st.samples = q.film->samples;
-            if (q.st.t_first_preview < 0.0) q.st.t_first_preview = t;
-            uint32_t rest = q.r.spp - q.st.samples;
-            if (rest == 0) {
-                q.st.state = AG_REQ_DONE;
-                q.st.t_done = t;
-                continue;
-            }
-            uint64_t px = (uint64_t)q.r.width * q.r.height;
-            uint32_t cap = (uint32_t)std::max<uint64_t>(1, s->step_samples / px);
-            q.next_chunk = std::min(rest, std::max(q.next_chunk, std::min(cap, q.next_chunk * 2)));
         }
         std::vector<uint64_t> still;
         for (uint64_t k : s->active)
@@ -1415,11 +1575,24 @@ extern "C" int ag_server_status(const ag_server* s, uint64_t id, ag_request_stat
     return AG_OK;
 }
 
+extern "C" const char* ag_server_error(const ag_server* s, uint64_t id) {
+    if (!s) return "";
+    auto it = s->reqs.find(id);
+    return it == s->reqs.end() ? "" : it->second.error.c_str();
+}
+
+extern "C" int ag_server_budget(const ag_server* s, size_t* used, size_t* budget) {
+    if (!s) return fail(AG_E_INVALID, "null server");
+    if (used) *used = server_used(s, s->max_keys);
+    if (budget) *budget = s->budget;
+    return AG_OK;
+}
+
 extern "C" int ag_server_read(const ag_server* s, uint64_t id, int32_t which, float* out) {
     if (!s || !out) return fail(AG_E_INVALID, "null argument");
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return fail(AG_E_NOTFOUND, "no such request");
-    if (!it->second.film) return fail(AG_E_STATE, "request not admitted yet");
+    if (!it->second.film) return fail(AG_E_STATE, "request has no film (not admitted, failed or released)");
     return ag_film_read(it->second.film, which, out);
 }
 
@@ -1428,10 +1601,7 @@ extern "C" int ag_server_release(ag_server* s, uint64_t id) {
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return fail(AG_E_NOTFOUND, "no such request");
     if (it->second.st.state == AG_REQ_ACTIVE) return fail(AG_E_STATE, "request still rendering");
-    if (it->second.film) {
-        ag_film_destroy(it->second.film);
-        s->used -= it->second.cost;
-    }
+    server_uncharge(s, it->second);
     s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), id), s->queue.end());
     s->reqs.erase(it);
     return AG_OK;

@@ -70,6 +70,16 @@ class Server:
     def release(self, i):
         check(A.lib().ag_server_release(self.h, i))
 
+    def error(self, i):
+        """Why request i failed ('' unless its state is REQ_FAILED)."""
+        return A.lib().ag_server_error(self.h, i).decode()
+
+    def budget(self):
+        """(bytes charged now: films, scenes, wave state; the budget)."""
+        u, b = C.c_size_t(), C.c_size_t()
+        check(A.lib().ag_server_budget(self.h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
 
 ALLOYS = ["sterling", "gold14k", "rose14k", "bronze", "gold18k", "platinum"]
 STUDIOS = ["noir", "white", "velvet", "ember", "window"]
@@ -114,7 +124,13 @@ def build_pieces(n_meshes=8, seed=6):
 
 def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, depth=12, max_wall=None,
               log=lambda *a: None):
-    """Replay `trace` in real time against one server; returns metrics."""
+    """Replay `trace` in real time against one server; returns metrics.
+
+    Each request is released as soon as it is done (its film's bytes go back
+    to the budget).  Latencies run from the arrival time; a request still
+    unfinished when the replay stops counts in the time-to-final percentiles
+    as censored at that moment (a lower bound), and `final_censored` says how
+    many did."""
     from . import workloads
     ps = workloads.path_settings(depth)
     scenes = {}
@@ -134,8 +150,10 @@ def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, d
     srv = Server(ctx, budget_bytes, step_samples, ps)
     pending = list(trace)
     ids = []
+    finished = {}  # id -> status when it left the server
     t0 = time.perf_counter()
     steps = 0
+    active = queued = 0
     while True:
         now = time.perf_counter() - t0
         while pending and pending[0]["t"] <= now:
@@ -152,26 +170,43 @@ def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, d
             ids.append((i, r, ((time.perf_counter() - t0) - r["t"]) * 1e3))
         active, queued = srv.step()
         steps += 1
-        if active == 0 and queued == 0:
-            if not pending:
+        for i, _, _ in ids:
+            if i in finished:
+                continue
+            st = srv.status(i)
+            if st["state"] in (A.REQ_DONE, A.REQ_FAILED, A.REQ_REJECTED):
+                if st["state"] == A.REQ_FAILED:
+                    log(f"request {i} failed: {srv.error(i)}")
+                finished[i] = st
+                srv.release(i)
+        if active == 0:
+            if queued == 0 and not pending:
                 break
-            time.sleep(max(0.0, min(0.05, pending[0]["t"] - (time.perf_counter() - t0))))
+            # nothing on the device: wait for the next arrival (or, should the
+            # head of the queue not fit, yield instead of spinning)
+            nxt = pending[0]["t"] - (time.perf_counter() - t0) if pending else 0.001
+            time.sleep(max(0.0005, min(0.05, nxt)))
         if max_wall is not None and time.perf_counter() - t0 > max_wall:
             log("wall limit reached")
             break
     wall = time.perf_counter() - t0
     backlog_active, backlog_queued = active, queued
-    ttfp, ttf, done_samples, served = [], [], 0, 0
+    ttfp, ttf, done_samples, served, failed, censored = [], [], 0, 0, 0, 0
     offered = sum(r["width"] * r["height"] * r["spp"] for _, r, _ in ids) + sum(
         r["width"] * r["height"] * r["spp"] for r in pending)
     for i, r, lag in ids:  # latencies from the arrival time
-        st = srv.status(i)
+        st = finished.get(i) or srv.status(i)
         done_samples += st["samples"] * r["width"] * r["height"]
         if st["t_first_preview"] >= 0:
             ttfp.append(st["t_first_preview"] - st["t_submit"] + lag)
         if st["state"] == A.REQ_DONE:
             served += 1
             ttf.append(st["t_done"] - st["t_submit"] + lag)
+        elif st["state"] in (A.REQ_FAILED, A.REQ_REJECTED):
+            failed += 1
+        else:
+            censored += 1
+            ttf.append(wall * 1e3 - r["t"] * 1e3)
     srv.close()
     for sc, _, _ in scenes.values():
         if sc.base is not None:
@@ -180,10 +215,10 @@ def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, d
         sc.close()
     pct = lambda v, q: float(np.percentile(v, q)) if v else None
     return {
-        "requests": len(trace), "submitted": len(ids), "completed": served, "steps": steps, "wall_s": round(wall, 2),
-        "latency_from": "arrival",
+        "requests": len(trace), "submitted": len(ids), "completed": served, "failed": failed, "steps": steps,
+        "wall_s": round(wall, 2), "latency_from": "arrival",
         "ttfp_ms_p50": pct(ttfp, 50), "ttfp_ms_p99": pct(ttfp, 99), "previewed": len(ttfp),
-        "final_ms_p50": pct(ttf, 50), "final_ms_p99": pct(ttf, 99),
+        "final_ms_p50": pct(ttf, 50), "final_ms_p99": pct(ttf, 99), "final_censored": censored,
         "served_msamples_per_s": round(done_samples / wall / 1e6, 2),
         "backlog_active": backlog_active, "backlog_queued": backlog_queued,
         "not_submitted": len(pending),
@@ -197,7 +232,8 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--spp-hi", type=int, default=1024)
     ap.add_argument("--meshes", type=int, default=8)
-    ap.add_argument("--max-wall", type=float, default=None)
+    ap.add_argument("--max-wall", type=float, default=None,
+                    help="stop the replay after this many seconds (default: 3 x --seconds + 30)")
     ap.add_argument("--step-log2", type=int, default=25, help="server step budget, log2 samples")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
@@ -207,7 +243,8 @@ def main():
     pieces = build_pieces(args.meshes)
     log("meshes:", [len(p.triangles) for p in pieces])
     with Context(0) as ctx:
-        res = run_trace(ctx, trace, pieces, step_samples=1 << args.step_log2, max_wall=args.max_wall, log=log)
+        wall = args.max_wall if args.max_wall is not None else 3 * args.seconds + 30
+        res = run_trace(ctx, trace, pieces, step_samples=1 << args.step_log2, max_wall=wall, log=log)
     res["config"] = dict(seconds=args.seconds, rate=args.rate, spp_hi=args.spp_hi, meshes=args.meshes,
                          step_samples=1 << args.step_log2)
     print(json.dumps(res))

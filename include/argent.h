@@ -561,8 +561,18 @@ int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_material* mater
  * server's settings (first chunk preview_spp, then doubling up to a cap of
  * step_samples / pixels), so its film is bit-identical whether it is served
  * alone or in a mix.  Films stay resident until released; ag_server_read
- * gives the progressive image. */
-enum { AG_REQ_QUEUED = 0, AG_REQ_ACTIVE = 1, AG_REQ_DONE = 2, AG_REQ_REJECTED = 3 };
+ * gives the progressive image.
+ *
+ * Admission charges the HBM budget with the request's film, the device bytes
+ * of its scene look and piece (each charged once while any admitted request
+ * uses it; ag_scene_stats gives them) and the context's wave state; a request
+ * that would not fit an idle server is REJECTED at submit.  Preview-only
+ * steps run at most 4 in a row; then the requests in progress go first.
+ * A request whose film cannot be created, or whose chunk fails to render when
+ * retried alone, becomes FAILED (ag_server_error tells why) and frees its
+ * film; the other requests carry on, as the reference's job queue isolates
+ * each job (queue.rs:514-557). */
+enum { AG_REQ_QUEUED = 0, AG_REQ_ACTIVE = 1, AG_REQ_DONE = 2, AG_REQ_REJECTED = 3, AG_REQ_FAILED = 4 };
 typedef struct ag_request {
     const ag_scene* scene;
     ag_camera camera;
@@ -590,6 +600,10 @@ int ag_server_status(const ag_server* s, uint64_t id, ag_request_status* out);
 /* Progressive image of a request (as ag_film_read), host memory w*h*3. */
 int ag_server_read(const ag_server* s, uint64_t id, int32_t which, float* out);
 int ag_server_release(ag_server* s, uint64_t id);
+/* Why a request failed ("" otherwise); valid until the request is released. */
+const char* ag_server_error(const ag_server* s, uint64_t id);
+/* Bytes charged against the budget now (films, scenes, wave state), and the budget. */
+int ag_server_budget(const ag_server* s, size_t* used, size_t* budget);
 
 /* ------------------------------------------------------------ tuning
  * Launch shape of the wavefront kernels (the autotuner's knobs). */

@@ -18,7 +18,7 @@ def _pieces():
     return a, b
 
 
-def _serve(ctx, reqs, budget=8 << 30, step_samples=1 << 14):
+def _serve(ctx, reqs, budget=64 << 30, step_samples=1 << 14):
     srv = Server(ctx, budget, step_samples, workloads.path_settings(8))
     ids = [srv.submit(**r) for r in reqs]
     for _ in range(1000):
@@ -51,12 +51,38 @@ def test_alone_equals_mix(ctx):
     sb.close()
 
 
+def _one_film_budget(ctx, sa, cam, turn):
+    """The budget that admits exactly one 32 x 32 request on `sa`: what a server
+    charges with it admitted (its film, the scene look and piece, the wave state)."""
+    probe = Server(ctx, budget_bytes=1 << 50, step_samples=1 << 12, ps=workloads.path_settings(6))
+    # a first request sizes the wave state for the scene's material keys
+    first = probe.submit(sa, cam, 8, 8, 1, 1, turn)
+    probe.step()
+    probe.release(first)
+    idle, _ = probe.budget()
+    i = probe.submit(sa, cam, 32, 32, 1, 1, turn)
+    j = probe.submit(sa, cam, 32, 32, 1, 2, turn)
+    probe.step()
+    used2, _ = probe.budget()
+    assert probe.status(i)["state"] == A.REQ_DONE and probe.status(j)["state"] == A.REQ_DONE
+    film_bytes = 32 * 32 * 9 * 4 * 2
+    # two films, and the look and piece they share charged once
+    assert used2 - idle == 2 * film_bytes + sa.stats()["bytes"]
+    probe.release(j)
+    used = probe.budget()[0]
+    assert used - idle == film_bytes + sa.stats()["bytes"]
+    probe.release(i)
+    assert probe.budget()[0] == idle
+    probe.close()
+    return used
+
+
 def test_budget_admission_and_rejection(ctx):
     pa, _ = _pieces()
     sa = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
     cam, turn = pa.shot("hero", 100.0, 0.0, 1.0)
-    film_bytes = 32 * 32 * 9 * 4 * 2
-    srv = Server(ctx, budget_bytes=film_bytes + 10, step_samples=1 << 12, ps=workloads.path_settings(6))
+    srv = Server(ctx, budget_bytes=_one_film_budget(ctx, sa, cam, turn) + 10, step_samples=1 << 12,
+                 ps=workloads.path_settings(6))
     big = srv.submit(sa, cam, 64, 64, 4, 1, turn)
     a1 = srv.submit(sa, cam, 32, 32, 4, 1, turn)
     a2 = srv.submit(sa, cam, 32, 32, 4, 2, turn)
@@ -78,6 +104,60 @@ def test_budget_admission_and_rejection(ctx):
     sa.close()
 
 
+def test_failed_request_does_not_stop_the_others(ctx):
+    """A request whose film cannot exist on the device (65535 x 65535: 309 GB)
+    fails alone with a reason; the requests around it finish and equal their
+    renders alone (queue.rs:514-557 isolates each job the same way)."""
+    pa, _ = _pieces()
+    sa = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
+    cam, turn = pa.shot("hero", 100.0, 0.0, 1.0)
+    ok1 = dict(scene=sa, camera=cam, width=32, height=32, spp=6, seed=3, turntable_deg=turn)
+    ok2 = dict(scene=sa, camera=cam, width=24, height=40, spp=5, seed=4, turntable_deg=turn)
+    srv = Server(ctx, 1 << 50, 1 << 14, workloads.path_settings(8))
+    i1 = srv.submit(**ok1)
+    bad = srv.submit(sa, cam, 65535, 65535, 4, 1, turn)
+    i2 = srv.submit(**ok2)
+    for _ in range(200):
+        a, q = srv.step()
+        if a == 0 and q == 0:
+            break
+    assert srv.status(bad)["state"] == A.REQ_FAILED
+    assert "free memory" in srv.error(bad)
+    with pytest.raises(ag.ArgentError):
+        srv.read(bad)
+    imgs = []
+    for i in (i1, i2):
+        assert srv.status(i)["state"] == A.REQ_DONE and srv.error(i) == ""
+        imgs.append(srv.read(i))
+    srv.release(bad)
+    srv.close()
+    assert np.array_equal(imgs[0], _serve(ctx, [ok1])[0][1])
+    assert np.array_equal(imgs[1], _serve(ctx, [ok2])[0][1])
+    sa.close()
+
+
+def test_arrivals_every_step_do_not_starve_work_in_progress(ctx):
+    """A new request before every step: at most 4 preview-only steps run in a
+    row, so a request in progress keeps gaining samples."""
+    pa, _ = _pieces()
+    sa = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
+    cam, turn = pa.shot("hero", 100.0, 0.0, 1.0)
+    srv = Server(ctx, 1 << 40, 32 * 32 * 4, workloads.path_settings(4))
+    long_id = srv.submit(sa, cam, 32, 32, 64, 1, turn)
+    srv.step()
+    assert srv.status(long_id)["t_first_preview"] >= 0
+    seen = [srv.status(long_id)["samples"]]
+    for k in range(15):
+        srv.submit(sa, cam, 16, 16, 2, 100 + k, turn)
+        srv.step()
+        seen.append(srv.status(long_id)["samples"])
+    grew = [i for i in range(1, len(seen)) if seen[i] > seen[i - 1]]
+    assert len(grew) >= 3, seen  # progress at least every fifth step
+    assert max(np.diff([0] + grew)) <= 5, seen
+    srv.close()
+    sa.close()
+
+
 def test_new_request_previews_before_long_one_finishes(ctx):
     """With a step budget smaller than the work in flight, a request that
     arrives while a long one renders gets its first preview at the next step,
@@ -87,7 +167,7 @@ def test_new_request_previews_before_long_one_finishes(ctx):
     sa = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
     cam, turn = pa.shot("hero", 100.0, 0.0, 1.0)
     step = 48 * 48 * 4
-    srv = Server(ctx, 8 << 30, step, workloads.path_settings(8))  # the depth _serve renders with
+    srv = Server(ctx, 64 << 30, step, workloads.path_settings(8))  # the depth _serve renders with
     long_id = srv.submit(sa, cam, 48, 48, 64, 1, turn)
     for _ in range(3):
         srv.step()
