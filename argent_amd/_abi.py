@@ -154,6 +154,10 @@ class Tuning(C.Structure):
                 ("shadow_refill_min", C.c_uint32), ("spread_min_lanes", C.c_uint32)]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("best", Tuning), ("ms_start", C.c_double), ("ms_best", C.c_double), ("trials", C.c_uint32)]
+
+
 class Stats(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("ms_raygen", C.c_double), ("ms_accum", C.c_double),
@@ -251,6 +255,10 @@ _SIGS = [
     ("ag_last_query_ms", C.c_double, [C.c_void_p]),
     ("ag_ctx_get_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("ag_ctx_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    ("ag_autotune_render", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.POINTER(PathSettingsC), C.c_uint32, C.POINTER(AutotuneResult)]),
+    ("ag_autotune_rays", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, C.POINTER(HitsSoA),
+                                   C.c_uint32, C.POINTER(AutotuneResult)]),
     ("ag_ctx_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("ag_ctx_enable_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("ag_mesh_load_stl", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int32, C.POINTER(LoadOptions),

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libargent_hip.so")
 
 HIP_SOURCES = ["kernels.hip", "post.hip", "mesh.hip", "bvh.hip", "normals.hip"]
-CPP_SOURCES = ["runtime.cpp", "host_build.cpp", "host_spectra.cpp", "host_bvh.cpp", "host_post.cpp"]
+CPP_SOURCES = ["runtime.cpp", "host_build.cpp", "host_spectra.cpp", "host_bvh.cpp", "host_post.cpp", "autotune.cpp"]
 
 
 def hipcc():

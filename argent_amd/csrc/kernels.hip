@@ -1086,6 +1086,7 @@ struct ExtendJob {
                 Q.hit[i] = h;
             });
         }
+        if (!Q.sort && key > 0) key = 0;  // unsorted shading: one bucket in arrival order
         uint64_t m = __ballot(key >= 0);
         if (key >= 0) stage[staged + lane_prefix(m)] = i | ((uint32_t)key << 26);
         staged += (uint32_t)__popcll(m);

@@ -277,6 +277,7 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
     if (t->batch_blocks_per_cu < 1 || (int)t->batch_blocks_per_cu > c->max_batch_blocks)
         return fail(AG_E_INVALID, "batch_blocks_per_cu must be 1.." + std::to_string(c->max_batch_blocks) +
                                       " (resident ray-batch traversal blocks per CU)");
+    if (t->sort_by_material > 1) return fail(AG_E_INVALID, "sort_by_material must be 0 or 1");
     for (uint32_t r : {t->refill_min, t->batch_refill_min, t->shadow_refill_min, t->spread_min_lanes})
         if (r < 1 || r > 64)
             return fail(AG_E_INVALID,
@@ -384,6 +385,7 @@ static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) 
     c->Q.cursors = c->Q.counters + n_cnt;
     c->Q.cap = cap;
     c->Q.n_keys = n_keys;
+    c->Q.sort = c->tuning.sort_by_material ? 1 : 0;
     if ((rc = c->seg_buf.ensure(MAX_SEGMENTS * sizeof(DSeg)))) return rc;
     return AG_OK;
 }

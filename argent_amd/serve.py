@@ -226,6 +226,62 @@ def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, d
     }
 
 
+def serial_plain(ctx, trace, pieces, depth=12, budget_s=30.0, log=lambda *a: None):
+    """The plain per-request path on the same trace: each request, in arrival
+    order, rendered alone by one render_pass of its full spp over a look whose
+    slot 0 is its alloy (the reference's queue runs one job at a time,
+    queue.rs:475-558).  Runs requests until budget_s of render time is spent;
+    returns their throughput and the time to final a serial server would give
+    them (each starts at its arrival or when the previous one ends)."""
+    from . import Film, render_pass, workloads
+    ps = workloads.path_settings(depth)
+    bases, looks, cams = {}, {}, {}
+    done, total_s, samples = [], 0.0, 0
+    for r in trace:
+        if total_s >= budget_s:
+            break
+        mi = r["mesh"] % len(pieces)
+        p = pieces[mi]
+        key = (mi, r["studio"], r["alloy"])
+        if key not in looks:
+            sc = Scene(ctx, p.scene(r["alloy"], 0.07, 0.8, r["studio"]), share_mesh_with=bases.get(mi))
+            bases.setdefault(mi, sc)
+            looks[key] = sc
+        sc = looks[key]
+        ck = (mi, r["width"], r["height"], r["studio"])
+        if ck not in cams:
+            f_stop = 0.0 if r["studio"] == "white" else 16.0
+            cams[ck] = p.shot("hero", 100.0, f_stop, r["width"] / r["height"], intersect=sc.intersect)
+        cam, turn = cams[ck]
+        sc.set_turntable(turn)
+        film = Film(ctx, r["width"], r["height"])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        render_pass(ctx, sc, cam, film, ps, r["spp"], r["seed"])
+        dt = time.perf_counter() - t0
+        film.close()
+        total_s += dt
+        samples += r["width"] * r["height"] * r["spp"]
+        done.append((r["t"], dt))
+    end, lat = 0.0, []
+    for t, dt in done:
+        end = max(end, t) + dt
+        lat.append((end - t) * 1e3)
+    base_ids = {id(sc) for sc in bases.values()}
+    for sc in looks.values():
+        if id(sc) not in base_ids:
+            sc.close()
+    for sc in bases.values():
+        sc.close()
+    pct = lambda v, q: float(np.percentile(v, q)) if v else None
+    log(f"serial plain: {len(done)} requests, {samples / 1e9:.2f} G samples in {total_s:.1f} s")
+    return {"requests": len(done), "samples": int(samples), "render_s": round(total_s, 2),
+            "msamples_per_s": round(samples / max(total_s, 1e-9) / 1e6, 2),
+            "final_ms_p50": pct(lat, 50), "final_ms_p99": pct(lat, 99),
+            "note": "the first requests of the trace in arrival order, one render_pass each at full spp; latency "
+                    "from arrival on a serial server (start at arrival or at the previous request's end)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--seconds", type=float, default=20.0)

@@ -706,6 +706,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the still's launch shape first (and persist it) instead of the stored choice")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default: the workload's)")
     ap.add_argument("--workload", choices=["signet", "torus", "gems"], default="signet",
@@ -719,10 +721,12 @@ def main():
     ap.add_argument("--live", action="store_true", help="the live view's first look / refinement loop instead")
     ap.add_argument("--bvh", action="store_true", help="scene creation with the device BVH build vs the host SAH")
     ap.add_argument("--new-mesh", action="store_true", help="a new STL's bytes to its first preview (serving)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
+    ap.add_argument("--serve-seconds", type=float, default=120.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
-    ap.add_argument("--serve-wall", type=float, default=45.0, help="stop serving after this many seconds")
+    ap.add_argument("--serve-wall", type=float, default=135.0, help="stop serving after this many seconds")
+    ap.add_argument("--plain-seconds", type=float, default=25.0,
+                    help="render time of the serial plain leg on the same trace (0 skips it)")
     ap.add_argument("--turntable", action="store_true", help="the 144-view turntable batch instead of the still")
     ap.add_argument("--spp-scale", type=float, default=1.0, help="turntable: scale every view's spp")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -783,6 +787,18 @@ def main():
     cam, turn = piece.shot("hero", 100.0, wl["f_stop"], w / h, intersect=scene.intersect)  # autofocus at f/16
     scene.set_turntable(turn)
     desc.d.turntable_deg = turn
+    from argent_amd import autotune
+    tune_info = None
+    if args.autotune:
+        log("autotune: trials at 64 spp ...")
+        tune_info = autotune.tune_render(ctx, scene, cam, w, h, 64, ps, repeats=2)
+        autotune.save("still", tune_info)
+        tuning_source = "autotuned in this run (render passes at 64 spp)"
+        log(f"autotune: {tune_info}")
+    else:
+        chosen, src = autotune.load("still")
+        autotune.apply(ctx, chosen)
+        tuning_source = src.split(":", 1)[0] + " autotuner choice" if src else "library defaults"
     stats = scene.stats()
     log(f"scene: {len(piece.triangles)} triangles, BVH {stats['nodes']} nodes depth {stats['depth']}, "
         f"{stats['bytes'] / 2**20:.0f} MiB on device, built in {time.time() - t0:.1f} s")
@@ -868,6 +884,7 @@ def main():
                       "ms_extend": [round(x, 2) for x in ds["ms_extend"]],
                       "ms_shade": [round(x, 2) for x in ds["ms_shade"]],
                       "ms_shadow": [round(x, 2) for x in ds["ms_shadow"]]},
+        "tuning": {"values": ctx.tuning, "source": tuning_source, "autotune": tune_info},
         "pipeline_hbm": {"bytes_alg_per_step": int(pipe_bytes / args.steps),
                          "achieved_GBs": round(pipe_bytes / elapsed / 1e9, 2),
                          "frac": round(pipe_bytes / elapsed / 1e9 / PEAK_HBM_GBS, 5)},
@@ -886,6 +903,10 @@ def main():
                         f"512x512, 1080x1920, 2000x2000; spp log-uniform 16..1024; 8 meshes "
                         f"{min(len(p.triangles) for p in pieces)}..{max(len(p.triangles) for p in pieces)} "
                         f"triangles; 6 alloys x 5 studios; depth 12; stopped after {args.serve_wall:g} s")
+        if args.plain_seconds > 0:
+            plain = serve.serial_plain(ctx, trace, pieces, depth=12, budget_s=args.plain_seconds, log=log)
+            res["serial_plain"] = plain
+            res["served_over_plain"] = round(res["served_msamples_per_s"] / max(plain["msamples_per_s"], 1e-9), 3)
         result["serving"] = res
         log("serving:", json.dumps(res))
     if not args.no_cpu:

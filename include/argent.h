@@ -628,6 +628,27 @@ typedef struct ag_tuning {
 int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
 int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);
 
+/* Autotuner (no counterpart in the reference, whose CPU renderer has no launch
+ * shape): times the caller's workload under candidate tunings and leaves the
+ * fastest set on the context.  Coordinate descent from the current tuning,
+ * each candidate the median of `repeats` timed runs after one untimed run; a
+ * change is kept when at least 1 % faster.  Every candidate renders the same
+ * film bit for bit.  ag_autotune_render times render passes of width x height
+ * at spp samples (paths_per_wave, blocks_per_cu, refill_min,
+ * shadow_refill_min, spread_min_lanes, sort_by_material); ag_autotune_rays
+ * times ag_intersect_device on a resident batch (batch_blocks_per_cu,
+ * batch_refill_min, spread_min_lanes). */
+typedef struct ag_autotune_result {
+    ag_tuning best;     /* also set on the context */
+    double ms_start;    /* the starting tuning's median time */
+    double ms_best;     /* the best tuning's median time */
+    uint32_t trials;    /* tunings timed */
+} ag_autotune_result;
+int ag_autotune_render(ag_ctx* ctx, const ag_scene* scene, const ag_camera* cam, uint32_t width, uint32_t height,
+                       uint32_t spp, const ag_path_settings* ps, uint32_t repeats, ag_autotune_result* out);
+int ag_autotune_rays(ag_ctx* ctx, const ag_scene* scene, const ag_rays_soa* rays_dev, uint32_t n,
+                     ag_hits_soa* hits_dev, uint32_t repeats, ag_autotune_result* out);
+
 /* Kernel timings of the last ag_render_pass / ag_render_views (HIP events on
  * the context stream), milliseconds, and ray counts. */
 typedef struct ag_stats {

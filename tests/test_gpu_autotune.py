@@ -1,0 +1,64 @@
+"""The autotuner (ag_autotune_render / ag_autotune_rays): it leaves the best
+tuning it found on the context, never slower than where it started, and every
+tuning renders the same film and the same hits bit for bit -- including
+shading in arrival order instead of material-sorted order."""
+import numpy as np
+import pytest
+
+import argent_amd as ag
+from argent_amd import autotune, synth, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _torus_scene(ctx):
+    piece = workloads.Piece(*synth.torus_ring(seed=1, nu=256, nv=64))
+    sc = ag.Scene(ctx, piece.scene("gold14k", 0.07, 0.8, "noir"))
+    cam, turn = piece.shot("hero", 100.0, 16.0, 1.0, intersect=sc.intersect)
+    sc.set_turntable(turn)
+    return piece, sc, cam
+
+
+def _film(ctx, sc, cam, ps, spp=8):
+    f = ag.Film(ctx, 96, 96)
+    ag.render_pass(ctx, sc, cam, f, ps, spp, 3)
+    s = f.sums()
+    f.close()
+    return s
+
+
+def test_autotune_render_keeps_the_film(ctx):
+    _, sc, cam = _torus_scene(ctx)
+    ps = workloads.path_settings(8)
+    start = ctx.tuning
+    want = _film(ctx, sc, cam, ps)
+    res = autotune.tune_render(ctx, sc, cam, 96, 96, 8, ps, repeats=1)
+    print("autotune:", res)
+    assert res["ms_best"] <= res["ms_start"] and res["trials"] > 10
+    assert ctx.tuning == {**start, **res["tuning"]}
+    assert np.array_equal(_film(ctx, sc, cam, ps), want)
+    # the extreme settings the search passes through render the same film
+    for kw in (dict(sort_by_material=0), dict(paths_per_wave=1 << 22, blocks_per_cu=2),
+               dict(refill_min=16, shadow_refill_min=8, spread_min_lanes=64)):
+        ctx.set_tuning(**{**start, **kw})
+        assert np.array_equal(_film(ctx, sc, cam, ps), want), kw
+    ctx.set_tuning(**start)
+    sc.close()
+
+
+def test_autotune_rays_keeps_the_hits(ctx):
+    piece, sc, _ = _torus_scene(ctx)
+    c = (piece.positions.min(0) + piece.positions.max(0)) * 0.5
+    o, d = synth.sphere_rays(1 << 18, c, seed=3)
+    rays = ag.DeviceRays(ctx, o, d)
+    start = ctx.tuning
+    ag.intersect_device(ctx, sc, rays)
+    want = rays.hits_host()
+    res = autotune.tune_rays(ctx, sc, rays, repeats=1)
+    assert res["ms_best"] <= res["ms_start"]
+    ag.intersect_device(ctx, sc, rays)
+    for a, b in zip(rays.hits_host(), want):
+        assert np.array_equal(a, b)
+    ctx.set_tuning(**start)
+    rays.close()
+    sc.close()
