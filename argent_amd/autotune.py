@@ -77,7 +77,7 @@ def apply(ctx, tuning):
 
 def main():
     """Tune the three classes on the benchmark workloads and write the choices:
-    still (the signet at 1080 x 1920, trials at 64 spp), preview (its live
+    still (the signet at 1080 x 1920, trials at its 256 spp), preview (its live
     first look: 540 x 960 halved, 2 spp, depth 32) and rays (the 2^24-ray
     batch)."""
     import argparse
@@ -102,7 +102,7 @@ def main():
                 w, h = wl["width"], wl["height"]
                 cam, turn = piece.shot("hero", 100.0, wl["f_stop"], w / h)
                 scene.set_turntable(turn)
-                res = tune_render(ctx, scene, cam, w, h, 64, workloads.path_settings(wl["depth"]), args.repeats)
+                res = tune_render(ctx, scene, cam, w, h, wl["spp"], workloads.path_settings(wl["depth"]), 1)
             elif cls == "preview":
                 w, h = 270, 480
                 cam, turn = piece.shot("hero", 100.0, wl["f_stop"], w / h)

@@ -10,10 +10,13 @@
 //   refill_min          idle lanes before a wavefront takes new extension rays
 //   shadow_refill_min   the same for shadow rays
 //   spread_min_lanes    how thinly a short queue is spread over the grid
-//   sort_by_material    shade in material-sorted order, or in arrival order
+//   sort_by_material    sort granularity: shade in material-sorted order, in
+//                       arrival order, or in runs ordered by material
 // (for ray batches: batch_blocks_per_cu, batch_refill_min, spread_min_lanes),
 // each candidate timed as the median of `repeats` runs; a change is kept when
 // it is at least 1 % faster, and rounds repeat until one changes nothing.
+#include <hip/hip_runtime.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -67,6 +70,23 @@ ag_tuning descend(const ag_tuning& start, const std::vector<Knob>& knobs,
     return best;
 }
 
+// Queue lengths worth trying: powers of two from 2^22 up to the first that
+// holds the whole pass in one wave, as far as the device's free memory allows
+// (about 284 bytes per path slot, with a quarter of the free memory kept back).
+std::vector<uint32_t> wave_sizes(uint32_t w, uint32_t h, uint32_t spp) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const double need = (double)w * h * spp;
+    std::vector<uint32_t> out;
+    for (uint32_t p = 1u << 22; p <= AG_MAX_PATHS_PER_WAVE; p <<= 1) {
+        if (p > (1u << 26) && (double)p * 284.0 > 0.75 * (double)free_b) break;
+        out.push_back(p);
+        if ((double)p >= need) break;
+        if (p == AG_MAX_PATHS_PER_WAVE) break;
+    }
+    return out;
+}
+
 }  // namespace
 
 extern "C" int ag_autotune_render(ag_ctx* ctx, const ag_scene* scene, const ag_camera* cam, uint32_t width,
@@ -103,12 +123,12 @@ extern "C" int ag_autotune_render(ag_ctx* ctx, const ag_scene* scene, const ag_c
         return median(ms);
     };
     std::vector<Knob> knobs = {
-        {&ag_tuning::paths_per_wave, {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26}},
+        {&ag_tuning::paths_per_wave, wave_sizes(width, height, spp)},
         {&ag_tuning::blocks_per_cu, {2, 3, 4, 5, 6, 7, 8}},
         {&ag_tuning::refill_min, {16, 32, 48, 64}},
         {&ag_tuning::shadow_refill_min, {8, 16, 32, 48, 64}},
         {&ag_tuning::spread_min_lanes, {1, 4, 16, 64}},
-        {&ag_tuning::sort_by_material, {0, 1}},
+        {&ag_tuning::sort_by_material, {0, 1, 2}},
     };
     out->best = descend(start, knobs, measure, &out->ms_start, &out->ms_best, &out->trials);
     ag_film_destroy(film);

@@ -147,7 +147,8 @@ struct DQueues {
     uint32_t* cursors;    // per depth: extension set, shadow set (CURSOR_SET u32 each)
     uint32_t cap;
     int32_t n_keys;
-    int32_t sort;         // 1: hits go to their material's bucket; 0: all to bucket 0 (shaded in arrival order)
+    int32_t sort;         // hits: 1 to their material's bucket; 0 all to bucket 0 in arrival order;
+                          // 2 to bucket 0 in runs (one traversal wavefront's) ordered by material
 };
 
 // Counters per depth slot: ray count, shadow count, one per bucket key.

@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
 // the piece in object space (ray turned by -turntable), then panels and floor
 // in world space.  Finished rays are staged per wavefront in LDS and appended
 // to the material buckets 64 or more at a time (one atomic per key per flush).
-constexpr int STAGE = 128;  // staging entries per wavefront: ray index | key << 26
+constexpr int STAGE = 128;  // staging entries per wavefront: ray index, and its key beside it
 
 struct ExtendJob {
     const DView* views;
@@ -1021,7 +1021,8 @@ struct ExtendJob {
     uint32_t* cnt;
     const float4* ro;
     const float4* rd;
-    uint32_t* stage;  // this wavefront's STAGE entries
+    uint32_t* stage;  // this wavefront's STAGE ray indices
+    uint8_t* skey;    // and their bucket keys
     uint32_t staged;  // uniform
     // Panels and floor first (world space, strict <: panels in index order,
     // then the floor), then the piece with t_max just above their nearest
@@ -1086,9 +1087,13 @@ struct ExtendJob {
                 Q.hit[i] = h;
             });
         }
-        if (!Q.sort && key > 0) key = 0;  // unsorted shading: one bucket in arrival order
+        if (Q.sort == 0 && key > 0) key = 0;  // unsorted shading: one bucket in arrival order
         uint64_t m = __ballot(key >= 0);
-        if (key >= 0) stage[staged + lane_prefix(m)] = i | ((uint32_t)key << 26);
+        if (key >= 0) {
+            const uint32_t e = staged + lane_prefix(m);
+            stage[e] = i;
+            skey[e] = (uint8_t)key;
+        }
         staged += (uint32_t)__popcll(m);
         if (staged >= 64) flush();
     }
@@ -1096,14 +1101,37 @@ struct ExtendJob {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (Q.sort == 2) {
+            // Local sort: the staged rays (neighbouring pixels, one traversal
+            // wavefront's) go to bucket 0 as one contiguous run ordered by key,
+            // so a shading wavefront mostly sees one material while the queue
+            // keeps its spatial order run by run.
+            uint32_t base = 0;
+            if (lane_id() == 0) base = atomicAdd(cnt + 2, staged);
+            base = __shfl(base, 0);
+            uint32_t off = 0;
+            for (int k = 0; k < Q.n_keys && off < staged; k++) {
+                for (uint32_t c = 0; c < staged; c += 64) {
+                    const uint32_t e = c + lane_id();
+                    const bool mine = e < staged && (int)skey[e] == k;
+                    const uint64_t m = __ballot(mine);
+                    if (mine) Q.bucket[base + off + lane_prefix(m)] = stage[e];
+                    off += (uint32_t)__popcll(m);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            staged = 0;
+            return;
+        }
         for (uint32_t base = 0; base < staged; base += 64) {
             uint32_t e = base + lane_id();
             int key = -1;
             uint32_t i = 0;
             if (e < staged) {
-                uint32_t v = stage[e];
-                i = v & 0x03FFFFFFu;
-                key = (int)(v >> 26);
+                i = stage[e];
+                key = (int)skey[e];
             }
             uint32_t slot = wave_append_keyed(cnt + 2, key);
             if (key >= 0) Q.bucket[(size_t)key * Q.cap + slot] = i;
@@ -1122,8 +1150,10 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* _
                                                        uint32_t refill, uint32_t spread, uint2* spill) {
     TRAV_STACK_DECL
     __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
+    __shared__ uint8_t s_skey[TRAV_BLOCK / 64][STAGE];
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
-    ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64], 0u};
+    ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64],
+                s_skey[threadIdx.x / 64], 0u};
     traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread);
 }
 
@@ -1506,6 +1536,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
                                                               uint2* spill) {
     TRAV_STACK_DECL
     __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
+    __shared__ uint8_t s_skey[TRAV_BLOCK / 64][STAGE];
     {
         uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
         ShadowJob J{views, P, Q};
@@ -1515,7 +1546,8 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
     {
         const int nd = depth + 1;
         uint32_t* cnt = Q.counters + nd * CNT_STRIDE;
-        ExtendJob J{views, P, Q, nd, cnt, Q.ray_o[nd & 1], Q.ray_d[nd & 1], s_stage[threadIdx.x / 64], 0u};
+        ExtendJob J{views, P, Q, nd, cnt, Q.ray_o[nd & 1], Q.ray_d[nd & 1], s_stage[threadIdx.x / 64],
+                    s_skey[threadIdx.x / 64], 0u};
         traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread);
     }
 }

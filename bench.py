@@ -149,7 +149,13 @@ def _pmc_rows(d, counter):
     return [(k, v) for _, k, v in rows]
 
 
-def reduce_traffic(fetch_dir, write_dir, mib):
+def _shipped_still_waves():
+    from argent_amd import autotune
+    t, src = autotune.load("still")
+    return (t or {}).get("paths_per_wave", 1 << 26), src
+
+
+def reduce_traffic(fetch_dir, write_dir, mib, spp):
     """HBM bytes per k_extend dispatch from two counter runs.  FETCH_SIZE and
     WRITE_SIZE read KiB per dispatch; each is scaled by known/counted bytes of
     the calibration copy in the same run."""
@@ -171,9 +177,10 @@ def reduce_traffic(fetch_dir, write_dir, mib):
                      "k_extend_kib_raw_mean": round(sum(ext) / len(ext), 1),
                      "k_extend_bytes_mean": round(sum(ext) / len(ext) * scale * 1024.0)}
     total = out["FETCH_SIZE"]["k_extend_bytes_mean"] + out["WRITE_SIZE"]["k_extend_bytes_mean"]
+    ppw, _ = _shipped_still_waves()
     res = {"kernel": "k_extend",
-           "workload": "signet still 1080x1920 64 spp depth 12: two waves of 2^26 paths, the same per-dispatch "
-                       "ray counts as the waves of the 256-spp still",
+           "workload": f"signet still 1080x1920 {spp} spp depth 12 with the bench's launch shape "
+                       f"({ppw} path slots per wave): the stage-timed pass's k_extend dispatches",
            "bytes_per_dispatch": int(total), "fetch": out["FETCH_SIZE"], "write": out["WRITE_SIZE"]}
     path = os.path.join(HERE, "profiles", "k_extend_signet_traffic.json")
     with open(path, "w") as fh:
@@ -735,7 +742,7 @@ def main():
                     help="turn two rocprofv3 --pmc runs into profiles/k_extend_signet_traffic.json")
     args = ap.parse_args()
     if args.reduce_traffic:
-        reduce_traffic(*args.reduce_traffic, args.copy_calibration or 1024)
+        reduce_traffic(*args.reduce_traffic, args.copy_calibration or 1024, args.spp or 256)
         return
     if args.turntable:
         turntable(args)
@@ -789,11 +796,12 @@ def main():
     desc.d.turntable_deg = turn
     from argent_amd import autotune
     tune_info = None
+    library_tuning = ctx.tuning
     if args.autotune:
-        log("autotune: trials at 64 spp ...")
-        tune_info = autotune.tune_render(ctx, scene, cam, w, h, 64, ps, repeats=2)
+        log(f"autotune: trials at {spp} spp ...")
+        tune_info = autotune.tune_render(ctx, scene, cam, w, h, spp, ps, repeats=1)
         autotune.save("still", tune_info)
-        tuning_source = "autotuned in this run (render passes at 64 spp)"
+        tuning_source = f"autotuned in this run (render passes at {spp} spp)"
         log(f"autotune: {tune_info}")
     else:
         chosen, src = autotune.load("still")
@@ -891,6 +899,9 @@ def main():
     }
     film.close()
     scene.close()
+    # the server runs with the library's launch shape (a still's tuning may
+    # hold a wave far larger than serving's memory budget allows)
+    ctx.set_tuning(**library_tuning)
     if args.serve_seconds > 0 and args.workload == "signet":
         from argent_amd import serve
         trace = serve.make_trace(args.serve_seconds, args.serve_rate)

@@ -29,6 +29,9 @@ extern "C" {
 #define AG_GRID_N 95   /* 5 nm grid, 360..830 nm */
 #define AG_MAX_PANELS 16
 #define AG_MAX_MATERIALS 32
+/* Largest ag_tuning.paths_per_wave: 2^29 path slots (about 150 GB of path
+ * state and queues) hold a whole 1080 x 1920 x 256-spp still in one wave. */
+#define AG_MAX_PATHS_PER_WAVE (1u << 29)
 
 enum {
     AG_OK = 0,
@@ -608,10 +611,14 @@ int ag_server_budget(const ag_server* s, size_t* used, size_t* budget);
 /* ------------------------------------------------------------ tuning
  * Launch shape of the wavefront kernels (the autotuner's knobs). */
 typedef struct ag_tuning {
-    uint32_t paths_per_wave;   /* path slots in flight */
+    uint32_t paths_per_wave;   /* path slots in flight (1024..AG_MAX_PATHS_PER_WAVE,
+                                  about 284 bytes each) */
     uint32_t block_traverse;   /* threads per block, traversal kernels */
     uint32_t blocks_per_cu;    /* persistent traversal blocks per CU, render passes */
-    uint32_t sort_by_material; /* 1 = shade in material-sorted order */
+    uint32_t sort_by_material; /* sort granularity of shading: 1 = material-sorted
+                                  order; 0 = arrival order; 2 = arrival order in
+                                  runs of one traversal wavefront's hits, each
+                                  run ordered by material */
     uint32_t refill_min;       /* render passes, extension rays: idle lanes (1..64)
                                   before a wavefront takes new rays; 64 = whole
                                   coherent batches */
