@@ -42,3 +42,25 @@ def assert_same_closest(pos, tris, o, d, got, ref, max_rate=1e-6):
               f"f64 barycentric min {m:.3g}")
         assert m < 0.0, ("a genuine closer hit was missed", i, p_close)
     return len(bad)
+
+
+def classify_rule_differences(pos, tris, o, d, ref, robust):
+    """Rays where the reference's traversal rules (bvh.rs:159-245: exact box
+    test, first hit found wins an equal t) and the robust rules (boxes widened
+    by 2^-16, equal t to the lower triangle id) disagree.  Each must be either
+    an exact tie -- both report the same t for two triangles, a ray through
+    their shared edge -- or a grazing ray whose closer answer is an f32
+    artefact outside its triangle in f64.  Returns (ties, artefacts)."""
+    xt, xp = ref
+    rt, rp = robust
+    ties = arts = 0
+    for i in np.nonzero((xp != rp) | (xt != rt))[0]:
+        if xt[i] == rt[i] and xp[i] != 0xFFFFFFFF and rp[i] != 0xFFFFFFFF:
+            ties += 1
+            continue
+        cands = [(float(t), int(p)) for t, p in ((xt[i], xp[i]), (rt[i], rp[i])) if p != 0xFFFFFFFF]
+        t_close, p_close = min(cands)
+        m = f64_barycentric_min(pos, tris, p_close, o[i], d[i])
+        assert m < 0.0, ("the rules differ on a ray whose closer hit is genuine", i, p_close, m)
+        arts += 1
+    return ties, arts

@@ -11,7 +11,7 @@ import pytest
 
 import argent_amd as ag
 from argent_amd import synth, workloads
-from hitcheck import assert_same_closest
+from hitcheck import assert_same_closest, classify_rule_differences
 
 pytestmark = pytest.mark.gpu
 
@@ -105,11 +105,12 @@ def test_signet_rays_bit_exact(ctx):
     hit = (rp != 0xFFFFFFFF) & same
     assert np.array_equal(gt[hit], rt[hit])
     assert np.array_equal(gu[hit], ru[hit]) and np.array_equal(gv[hit], rv[hit])
-    # the reference traversal rules agree except at exact ties / box-edge rounding
+    # the reference traversal rules agree except at exact ties and grazing
+    # f32 artefacts (79 rays of the full 2^24 batch, test_oracle.py)
     xt, _, _, xp = osc.intersect(o, d, mode=O.MODE_REFERENCE)
-    diff = xp != rp
-    assert diff.sum() <= 1e-4 * len(rp)
-    assert np.array_equal(xt[~diff], rt[~diff])
+    diff = (xp != rp) | (xt != rt)
+    ties, arts = classify_rule_differences(pos, tris, o, d, (xt, xp), (rt, rp))
+    assert ties + arts == diff.sum() <= 16, (ties, arts)
     occ = sc.occluded(o, d)
     assert np.array_equal(occ[same], (rp != 0xFFFFFFFF)[same])
     sc.close()
