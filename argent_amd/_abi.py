@@ -151,7 +151,8 @@ class Tuning(C.Structure):
     _fields_ = [("paths_per_wave", C.c_uint32), ("block_traverse", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("sort_by_material", C.c_uint32),
                 ("refill_min", C.c_uint32), ("batch_blocks_per_cu", C.c_uint32), ("batch_refill_min", C.c_uint32),
-                ("shadow_refill_min", C.c_uint32), ("spread_min_lanes", C.c_uint32)]
+                ("shadow_refill_min", C.c_uint32), ("spread_min_lanes", C.c_uint32),
+                ("ordered_emission", C.c_uint32), ("cursor_slices", C.c_uint32)]
 
 
 class AutotuneResult(C.Structure):

@@ -11,7 +11,10 @@
 //   shadow_refill_min   the same for shadow rays
 //   spread_min_lanes    how thinly a short queue is spread over the grid
 //   sort_by_material    sort granularity: shade in material-sorted order, in
-//                       arrival order, or in runs ordered by material
+//                       arrival order, in runs ordered by material, or with
+//                       misses and panel hits first
+//   ordered_emission    continuation rays in the hits' order or as appended
+//   cursor_slices       how many queue slices (each with a cursor) the grid pulls from
 // (for ray batches: batch_blocks_per_cu, batch_refill_min, spread_min_lanes),
 // each candidate timed as the median of `repeats` runs; a change is kept when
 // it is at least 1 % faster, and rounds repeat until one changes nothing.
@@ -128,7 +131,9 @@ extern "C" int ag_autotune_render(ag_ctx* ctx, const ag_scene* scene, const ag_c
         {&ag_tuning::refill_min, {16, 32, 48, 64}},
         {&ag_tuning::shadow_refill_min, {8, 16, 32, 48, 64}},
         {&ag_tuning::spread_min_lanes, {1, 4, 16, 64}},
-        {&ag_tuning::sort_by_material, {0, 1, 2}},
+        {&ag_tuning::sort_by_material, {0, 1, 2, 3}},
+        {&ag_tuning::ordered_emission, {0, 1}},
+        {&ag_tuning::cursor_slices, {1, 2, 4, 8}},
     };
     out->best = descend(start, knobs, measure, &out->ms_start, &out->ms_best, &out->trials);
     ag_film_destroy(film);

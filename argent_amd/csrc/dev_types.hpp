@@ -147,12 +147,22 @@ struct DQueues {
     uint32_t* cursors;    // per depth: extension set, shadow set (CURSOR_SET u32 each)
     uint32_t cap;
     int32_t n_keys;
+    uint32_t slices;      // render traversal: queue slices with a cursor each (1..CURSOR_SLICES)
+    int32_t ordered;      // 1: shading writes hit j's continuation ray to slot j of the next queue
+                          // (DEAD_PATH where the path ended), keeping the hit order exactly
     int32_t sort;         // hits: 1 to their material's bucket; 0 all to bucket 0 in arrival order;
-                          // 2 to bucket 0 in runs (one traversal wavefront's) ordered by material
+                          // 2 to bucket 0 in runs (one traversal wavefront's) ordered by material;
+                          // 3 misses and panel hits to buckets 0 and 1, surface hits to bucket 2 in arrival order
 };
 
 // Counters per depth slot: ray count, shadow count, one per bucket key.
 constexpr int CNT_STRIDE = 4 + MAX_KEYS;
+// counter slots per depth: [0] extension queue length, [1] shadow rays,
+// [2 .. 2 + MAX_KEYS) bucket sizes, [CNT_LIVE] live extension rays (the queue
+// length less its dead slots when shading emits in order)
+constexpr int CNT_LIVE = CNT_STRIDE - 1;
+// path id of a dead slot in an extension queue written in order
+constexpr uint32_t DEAD_PATH = 0xFFFFFFFFu;
 // Ray cursors of the persistent traversal kernels: a queue of n rays is cut
 // into CURSOR_SLICES equal slices, one per XCD, each with its own cursor on
 // its own 128-byte line; per depth slot there is one set for extension rays

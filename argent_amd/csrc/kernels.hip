@@ -282,16 +282,17 @@ struct RaySupply {
     uint32_t lo, hi;      // grabbed range not yet handed out
     uint32_t slice, tried;
     uint32_t grab;        // rays per cursor atomic (GRAB, or fewer for a spread queue)
+    uint32_t slices;      // 1..CURSOR_SLICES
 };
 
-AGD uint32_t slice_begin(uint32_t n, uint32_t s) { return (uint32_t)(((uint64_t)n * s) / CURSOR_SLICES); }
+AGD uint32_t slice_begin(uint32_t n, uint32_t s, uint32_t slices) { return (uint32_t)(((uint64_t)n * s) / slices); }
 
 // Up to `want` consecutive ray indices from *first; 0 once every slice is
 // spent.  Reached by all lanes of the wavefront.
 AGD uint32_t supply_take(RaySupply& R, uint32_t want, uint32_t* first) {
     while (R.lo == R.hi) {
-        if (R.tried >= CURSOR_SLICES) return 0;
-        uint32_t b = slice_begin(R.n, R.slice), e = slice_begin(R.n, R.slice + 1);
+        if (R.tried >= R.slices) return 0;
+        uint32_t b = slice_begin(R.n, R.slice, R.slices), e = slice_begin(R.n, R.slice + 1, R.slices);
         uint32_t got = e - b;  // spent unless the cursor says otherwise
         if (lane_id() == 0) {
             // Own slice: straight to the atomic.  Another wavefront's slice: a
@@ -307,7 +308,7 @@ AGD uint32_t supply_take(RaySupply& R, uint32_t want, uint32_t* first) {
             R.lo = b + got;
             R.hi = b + min(got + R.grab, e - b);
         } else {
-            R.slice = (R.slice + 1) % CURSOR_SLICES;
+            R.slice = (R.slice + 1) % R.slices;
             R.tried++;
         }
     }
@@ -471,7 +472,7 @@ AGD void step_any(TravLane& L, const TravStack& st) {
 // share a wavefront does not change any ray's result.
 template <bool ANY, class Job>
 AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t refill_min, const TravStack& st,
-                             uint32_t min_lanes = 64u) {
+                             uint32_t min_lanes = 64u, uint32_t slices = CURSOR_SLICES) {
     TravLane L;
     L.cur = TRAV_DONE;
     L.sp = 0;
@@ -482,7 +483,7 @@ AGD void traverse_persistent(Job& J, uint32_t n, uint32_t* cursors, uint32_t ref
     const uint32_t grab = lanes < 64u ? lanes : GRAB;
     const bool lane_ok = lane_id() < lanes;
     const uint32_t refill = min(refill_min, lanes);
-    RaySupply R{n, cursors, 0u, 0u, blockIdx.x % CURSOR_SLICES, 0u, grab};
+    RaySupply R{n, cursors, 0u, 0u, blockIdx.x % slices, 0u, grab, slices};
     // wavefronts beyond what n rays can occupy (`grab` each) leave at once
     uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
     // uniform across the wavefront; 64-bit so that n near 2^32 cannot wrap
@@ -1029,7 +1030,12 @@ struct ExtendJob {
     // hit, so the piece wins ties as it does when tested first.  Until the
     // piece is hit, L.u / L.v hold the panel-or-floor code and distance.
     AGD bool begin(uint32_t i, TravLane& L) {
-        float4 a = ro[i], b = rd[i];
+        float4 a = ro[i];
+        if (__float_as_uint(a.w) == DEAD_PATH) {  // a path that ended: nothing to trace or shade
+            L.aux = NO_RAY;
+            return false;
+        }
+        float4 b = rd[i];
         uint32_t vi = P.view[__float_as_uint(a.w)];
         per_uniform(vi, [&](uint32_t v) {
             const DScene* Sp = uload(&views[v].scene);
@@ -1072,7 +1078,7 @@ struct ExtendJob {
     }
     AGD void finish(uint32_t i, const TravLane& L, bool mine) {
         int key = -1;
-        if (mine) {
+        if (mine && L.aux != NO_RAY) {
             per_uniform(L.aux, [&](uint32_t v) {
                 const DScene* Sp = uload(&views[v].scene);
                 float4 h;
@@ -1088,6 +1094,7 @@ struct ExtendJob {
             });
         }
         if (Q.sort == 0 && key > 0) key = 0;  // unsorted shading: one bucket in arrival order
+        if (Q.sort == 3 && key > 2) key = 2;  // misses, panel hits, then every surface hit in arrival order
         uint64_t m = __ballot(key >= 0);
         if (key >= 0) {
             const uint32_t e = staged + lane_prefix(m);
@@ -1154,7 +1161,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* _
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64],
                 s_skey[threadIdx.x / 64], 0u};
-    traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread);
+    traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread, Q.slices);
 }
 
 struct ShadeArgs {
@@ -1173,6 +1180,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
     // count have nothing to shade (uniform per block, before any barrier)
     if (blockIdx.x * blockDim.x >= cnt[0]) return;
     __shared__ uint32_t pre[MAX_KEYS + 1];
+    // in-order emission: the next queue is as long as this depth's hit list
+    if (Q.ordered && blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int k = 0; k < Q.n_keys; k++) tot += cnt[2 + k];
+        cnt_next[0] = (uint32_t)depth + 1 < A.max_depth ? tot : 0u;
+    }
     // microfacet.rs:93-140 E(mu, sqrt(alpha)): read by every rough lobe's
     // multiple-scattering term, several times per hit
     __shared__ float s_energy[32 * 32];
@@ -1415,10 +1428,21 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
             P.flags[path] = (specular ? 1u : 0u) | (rough ? 2u : 0u) | ((uint32_t)medium << 8);
             P.rng[path] = rng;
         }
-        uint32_t slot = wave_append(&cnt_next[0], emit_ray);
-        if (emit_ray) {
-            no[slot] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(path));
-            nd[slot] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
+        if (Q.ordered) {
+            // hit j's continuation goes to slot j: the next queue keeps this
+            // depth's hit order whatever order the blocks run in
+            if (j < total && (uint32_t)depth + 1 < A.max_depth) {
+                no[j] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(emit_ray ? path : DEAD_PATH));
+                if (emit_ray) nd[j] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
+            }
+            (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
+        } else {
+            uint32_t slot = wave_append(&cnt_next[0], emit_ray);
+            if (emit_ray) {
+                no[slot] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(path));
+                nd[slot] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
+            }
+            (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
         }
     }
 }
@@ -1519,7 +1543,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow(const DView* _
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ShadowJob J{views, P, Q};
     traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st,
-                              spread);
+                              spread, Q.slices);
 }
 
 // The shadow rays of one depth, then the extension rays of the next, in one
@@ -1541,14 +1565,14 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
         uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
         ShadowJob J{views, P, Q};
         traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET,
-                                  refill_shadow, st, spread);
+                                  refill_shadow, st, spread, Q.slices);
     }
     {
         const int nd = depth + 1;
         uint32_t* cnt = Q.counters + nd * CNT_STRIDE;
         ExtendJob J{views, P, Q, nd, cnt, Q.ray_o[nd & 1], Q.ray_d[nd & 1], s_stage[threadIdx.x / 64],
                     s_skey[threadIdx.x / 64], 0u};
-        traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread);
+        traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread, Q.slices);
     }
 }
 
@@ -1611,7 +1635,10 @@ __global__ void k_tally(const uint32_t* __restrict__ counters, uint32_t depth_sl
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         unsigned long long e = 0, s = 0;
         for (uint32_t d = 0; d < depth_slots; d++) {
-            uint32_t ed = counters[d * CNT_STRIDE], sd = counters[d * CNT_STRIDE + 1];
+            // live rays: the queue length at depth 0 (camera rays), the
+            // continuation count after that (an in-order queue has dead slots)
+            uint32_t ed = d == 0 ? counters[0] : counters[d * CNT_STRIDE + CNT_LIVE];
+            uint32_t sd = counters[d * CNT_STRIDE + 1];
             e += ed;
             s += sd;
             if (d < (uint32_t)TALLY_DEPTHS) {

@@ -245,6 +245,8 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     // pass 766 -> 599 ms at 2^26 paths per wave); material-sorted shading
     // (1) and runs ordered by material (2) stay selectable.
     c->tuning.sort_by_material = 0;
+    c->tuning.ordered_emission = 0;
+    c->tuning.cursor_slices = CURSOR_SLICES;
     *out = c;
     return AG_OK;
 }
@@ -283,7 +285,10 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
     if (t->batch_blocks_per_cu < 1 || (int)t->batch_blocks_per_cu > c->max_batch_blocks)
         return fail(AG_E_INVALID, "batch_blocks_per_cu must be 1.." + std::to_string(c->max_batch_blocks) +
                                       " (resident ray-batch traversal blocks per CU)");
-    if (t->sort_by_material > 2) return fail(AG_E_INVALID, "sort_by_material must be 0, 1 or 2");
+    if (t->sort_by_material > 3) return fail(AG_E_INVALID, "sort_by_material must be 0..3");
+    if (t->ordered_emission > 1) return fail(AG_E_INVALID, "ordered_emission must be 0 or 1");
+    if (t->cursor_slices < 1 || t->cursor_slices > (uint32_t)CURSOR_SLICES)
+        return fail(AG_E_INVALID, "cursor_slices must be 1.." + std::to_string(CURSOR_SLICES));
     for (uint32_t r : {t->refill_min, t->batch_refill_min, t->shadow_refill_min, t->spread_min_lanes})
         if (r < 1 || r > 64)
             return fail(AG_E_INVALID,
@@ -395,6 +400,8 @@ static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) 
     c->Q.cap = cap;
     c->Q.n_keys = n_keys;
     c->Q.sort = (int32_t)c->tuning.sort_by_material;
+    c->Q.ordered = c->tuning.ordered_emission ? 1 : 0;
+    c->Q.slices = c->tuning.cursor_slices;
     if ((rc = c->seg_buf.ensure(MAX_SEGMENTS * sizeof(DSeg)))) return rc;
     return AG_OK;
 }

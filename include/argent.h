@@ -618,7 +618,9 @@ typedef struct ag_tuning {
     uint32_t sort_by_material; /* sort granularity of shading: 1 = material-sorted
                                   order; 0 = arrival order; 2 = arrival order in
                                   runs of one traversal wavefront's hits, each
-                                  run ordered by material */
+                                  run ordered by material; 3 = misses and panel
+                                  hits first, then every surface hit in arrival
+                                  order */
     uint32_t refill_min;       /* render passes, extension rays: idle lanes (1..64)
                                   before a wavefront takes new rays; 64 = whole
                                   coherent batches */
@@ -631,6 +633,15 @@ typedef struct ag_tuning {
                                      spread over all of them, ceil(n / wavefronts)
                                      lanes each but at least this many (1..64;
                                      64 = never spread; default 1) */
+    uint32_t ordered_emission;    /* 1: shading writes each hit's continuation ray
+                                     to the hit's own slot of the next queue
+                                     (dead slots where paths end), so the next
+                                     queue keeps the hits' order exactly; 0:
+                                     appended as shading wavefronts finish */
+    uint32_t cursor_slices;       /* render traversal: a queue is cut into this
+                                     many slices (1..8), each pulled through its
+                                     own cursor, block b starting on slice
+                                     b mod slices (8 = one per XCD) */
 } ag_tuning;
 int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
 int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);

@@ -38,7 +38,9 @@ def test_autotune_render_keeps_the_film(ctx):
     assert ctx.tuning == {**start, **res["tuning"]}
     assert np.array_equal(_film(ctx, sc, cam, ps), want)
     # the extreme settings the search passes through render the same film
-    for kw in (dict(sort_by_material=0), dict(sort_by_material=2), dict(paths_per_wave=1 << 22, blocks_per_cu=2),
+    for kw in (dict(sort_by_material=1), dict(sort_by_material=2), dict(sort_by_material=3),
+               dict(ordered_emission=1), dict(ordered_emission=1, sort_by_material=3), dict(cursor_slices=1),
+               dict(paths_per_wave=1 << 22, blocks_per_cu=2),
                dict(refill_min=16, shadow_refill_min=8, spread_min_lanes=64)):
         ctx.set_tuning(**{**start, **kw})
         assert np.array_equal(_film(ctx, sc, cam, ps), want), kw
