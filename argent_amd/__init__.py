@@ -442,6 +442,10 @@ class Live:
             A.lib().ag_live_destroy(self.h)
             self.h = None
 
+    def attach(self, server):
+        """Run this session's passes inside `server`'s steps (argent_amd.serve.Server); None detaches."""
+        check(A.lib().ag_live_attach(self.h, server.h if server is not None else None))
+
     @staticmethod
     def _now(now_ms):
         return time.monotonic() * 1e3 if now_ms is None else float(now_ms)

@@ -276,6 +276,7 @@ _SIGS = [
     ("ag_mesh_read_normals", C.c_int, [C.c_void_p, f32p]),
     ("ag_live_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("ag_live_destroy", None, [C.c_void_p]),
+    ("ag_live_attach", C.c_int, [C.c_void_p, C.c_void_p]),
     ("ag_live_set", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(LiveParams), C.c_double]),
     ("ag_live_set_look", C.c_int, [C.c_void_p, C.POINTER(Grade), C.c_int32, C.c_double]),
     ("ag_live_step", C.c_int, [C.c_void_p, C.c_double, C.POINTER(LiveFrame), u8p]),

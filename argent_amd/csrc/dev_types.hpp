@@ -108,6 +108,8 @@ struct DView {
     float aspect_sx, aspect_sy;
     float cam_m[9];
     float* acc;          // pass accumulators: colour, albedo, normal, each w*h*3
+    uint32_t max_depth, rr_depth;  // this view's PathSettings (integrator.rs:10-24): views of
+    float clamp_indirect;          // one schedule may differ (a live view beside served finals)
 };
 
 // A contiguous range of path ids: `ns` samples starting at `s0` for pixels

@@ -1167,8 +1167,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* _
 struct ShadeArgs {
     const DView* views;
     const DMat* vmats;
-    uint32_t max_depth, rr_depth;
-    float clamp_indirect;
+    uint32_t max_depth;   // the launch's deepest view (each path stops at its own view's)
     const float* energy;  // the GGX energy table (32 x 32), staged into LDS per block
 };
 
@@ -1230,7 +1229,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
             bool specular = flags & 1u, rough = (flags & 2u) != 0;
             int medium = (int)((flags >> 8) & 255u);
             const bool clamp = rough;
-            const float cm = A.clamp_indirect;
+            const float cm = uload(&V.clamp_indirect);
             bool alive = true;
             if (code != HIT_MISS && medium != 255) {
                 Spec sig = tab_sample(uload(&S.mat[medium].tb), lam);
@@ -1411,7 +1410,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                     if (sm.transmit) medium = (wo.z > 0.0f && M.kind == AG_MAT_GEM) ? mat_id : 255;
                     nro = offset_origin(p, ngu, wi_w);
                     nrd = wi_w;
-                    if ((uint32_t)depth >= A.rr_depth) {
+                    if ((uint32_t)depth >= uload(&V.rr_depth)) {
                         float survive = fminf(max_value(beta), 0.95f);
                         if (survive <= 0.0f || next_random(rng) >= survive)
                             ok = false;
@@ -1419,7 +1418,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                             beta = beta / survive;
                     }
                 }
-                if (ok && (uint32_t)depth + 1 < A.max_depth) emit_ray = true;
+                if (ok && (uint32_t)depth + 1 < uload(&V.max_depth)) emit_ray = true;
             }
             P.beta[path] = to4(beta);
             P.L[path] = to4(L);
@@ -1736,9 +1735,9 @@ void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueue
                    uint32_t spread, uint2* spill) {
     hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spread, spill);
 }
-void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
-                  float clamp_indirect, const float* energy, DPaths P, DQueues Q, int depth) {
-    ShadeArgs A{views, vmats, max_depth, rr_depth, clamp_indirect, energy};
+void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth,
+                  const float* energy, DPaths P, DQueues Q, int depth) {
+    ShadeArgs A{views, vmats, max_depth, energy};
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
 }
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,

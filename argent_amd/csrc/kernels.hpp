@@ -44,8 +44,8 @@ void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t
 // spread: the least lanes per wavefront when a short queue is spread over the grid (64: never)
 void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint32_t spread, uint2* spill);
-void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth, uint32_t rr_depth,
-                  float clamp_indirect, const float* energy, DPaths P, DQueues Q, int depth);
+void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth,
+                  const float* energy, DPaths P, DQueues Q, int depth);
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint32_t spread, uint2* spill);
 // k_shadow then k_extend of depth + 1 as one launch (same results)

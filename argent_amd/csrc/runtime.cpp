@@ -932,7 +932,7 @@ struct ViewJob {
 };
 
 void make_dview(DView& v, const ag_scene* sc, const ag_camera* cam, float turn_deg, int mat0, const ag_film* film,
-                uint32_t seed) {
+                uint32_t seed, const ag_path_settings* ps) {
     std::memset(&v, 0, sizeof(v));
     v.scene = sc->d_scene.as<DScene>();
     const float* cam_m_in = sc->cam_m;
@@ -963,6 +963,9 @@ void make_dview(DView& v, const ag_scene* sc, const ag_camera* cam, float turn_d
     v.aspect_sy = 2.0f / (float)film->h;
     for (int i = 0; i < 9; i++) v.cam_m[i] = cam_m_in[i];
     v.acc = film->acc.as<float>();
+    v.max_depth = ps->max_depth;
+    v.rr_depth = ps->rr_depth;
+    v.clamp_indirect = ps->clamp_indirect;
 }
 
 }  // namespace
@@ -994,9 +997,15 @@ static void set_stats(ag_ctx* c, const std::vector<unsigned long long>& tally, u
 
 // With `ev` the pass is only enqueued (no kernel timing); statistics come
 // with ag_event_wait.
-static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, const ag_path_settings* ps,
-                     const DMat* vmats_dev, ag_event* ev = nullptr) {
-    if (ps->max_depth > 64) return fail(AG_E_INVALID, "max_depth above 64");
+static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, const DMat* vmats_dev,
+                     ag_event* ev = nullptr) {
+    // every view carries its own path settings; the bounce loop runs to the deepest
+    ag_path_settings launch{0, 0.0f, 0};
+    for (auto& j : jobs) {
+        if (j.dv.max_depth > 64) return fail(AG_E_INVALID, "max_depth above 64");
+        launch.max_depth = std::max(launch.max_depth, j.dv.max_depth);
+    }
+    const ag_path_settings* ps = &launch;
     HIPCHK(hipSetDevice(c->device));
     uint32_t cap = c->tuning.paths_per_wave;
     int n_keys = s->n_keys;
@@ -1087,8 +1096,7 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
             if (!fuse || d == 0)
                 timed(0, (int)d, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, spread, c->spill_buf.as<uint2>()); });
             timed(1, (int)d, [&] {
-                launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, ps->rr_depth, ps->clamp_indirect,
-                             c->energy_buf.as<float>(), c->P,
+                launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, c->energy_buf.as<float>(), c->P,
                              c->Q, (int)d);
             });
             if (fuse && d + 1 < ps->max_depth)
@@ -1156,9 +1164,9 @@ extern "C" int ag_render_pass(ag_ctx* ctx, const ag_scene* scene, const ag_camer
     j.film = film;
     j.count = count;
     j.first = film->samples;
-    make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed);
+    make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed, ps);
     std::vector<ViewJob> jobs{j};
-    return run_views(ctx, scene, jobs, ps, nullptr);
+    return run_views(ctx, scene, jobs, nullptr);
 }
 
 extern "C" int ag_render_pass_async(ag_ctx* ctx, const ag_scene* scene, const ag_camera* cam, ag_film* film,
@@ -1174,9 +1182,9 @@ extern "C" int ag_render_pass_async(ag_ctx* ctx, const ag_scene* scene, const ag
         j.film = film;
         j.count = count;
         j.first = film->samples;
-        make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed);
+        make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed, ps);
         std::vector<ViewJob> jobs{j};
-        int rc = run_views(ctx, scene, jobs, ps, nullptr, ev.get());
+        int rc = run_views(ctx, scene, jobs, nullptr, ev.get());
         if (rc) return rc;
     } else {
         ev->finished = true;
@@ -1263,12 +1271,12 @@ extern "C" int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_mate
         j.first = v.film->samples;
         const ag_scene* vs = v.scene ? v.scene : scene;
         if (vs->ctx != ctx) return fail(AG_E_INVALID, "view scene from another context");
-        make_dview(j.dv, vs, &v.camera, v.turntable_deg, v.material0, v.film, v.seed);
+        make_dview(j.dv, vs, &v.camera, v.turntable_deg, v.material0, v.film, v.seed, ps);
         j.scene = vs;
         jobs.push_back(j);
     }
     if (jobs.empty()) return AG_OK;
-    return run_views(ctx, scene, jobs, ps, ctx->vmat_buf.as<DMat>());
+    return run_views(ctx, scene, jobs, ctx->vmat_buf.as<DMat>());
 }
 
 // ------------------------------------------------------------------ server
@@ -1375,10 +1383,11 @@ void server_fail(ag_server* s, uint64_t k, const std::string& why) {
 }
 
 // One shared wavefront schedule over the next chunk of each request in `set`.
-int server_render(ag_server* s, const std::vector<uint64_t>& set) {
+int server_render(ag_server* s, const std::vector<uint64_t>& set, const ViewJob* extra = nullptr) {
     ag_ctx* ctx = s->ctx;
     std::vector<ag_material> mats;
     std::vector<ViewJob> jobs;
+    if (extra) jobs.push_back(*extra);  // an attached live view's pass (its own look and path settings)
     for (uint64_t k : set) {
         auto& q = s->reqs[k];
         ViewJob j;
@@ -1390,13 +1399,14 @@ int server_render(ag_server* s, const std::vector<uint64_t>& set) {
             m0 = (int)mats.size();
             mats.push_back(q.r.material0);
         }
-        make_dview(j.dv, q.r.scene, &q.r.camera, q.r.turntable_deg, m0, q.film, q.r.seed);
+        make_dview(j.dv, q.r.scene, &q.r.camera, q.r.turntable_deg, m0, q.film, q.r.seed, &s->ps);
         j.scene = q.r.scene;
         jobs.push_back(j);
     }
+    if (jobs.empty()) return AG_OK;
     int rc = upload_view_materials(ctx, mats.data(), (int32_t)mats.size());
     if (rc) return rc;
-    return run_views(ctx, jobs[0].scene, jobs, &s->ps, ctx->vmat_buf.as<DMat>());
+    return run_views(ctx, jobs[0].scene, jobs, ctx->vmat_buf.as<DMat>());
 }
 
 // A failed pass leaves film sums and sample counts as they were (the commit
@@ -1480,7 +1490,12 @@ extern "C" int ag_server_submit(ag_server* s, const ag_request* r, uint64_t* id)
     return AG_OK;
 }
 
-extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queued) {
+// One scheduling round; `extra` (may be null) is an attached live view's pass,
+// rendered in the same schedule ahead of the finals, which share what is left
+// of the step budget.  *extra_rc: that pass's result; *extra_ms: its share of
+// the step's time (by samples).
+static int server_step(ag_server* s, const ViewJob* extra, uint32_t* n_active, uint32_t* n_queued, int* extra_rc,
+                       double* extra_ms) {
     if (!s) return fail(AG_E_INVALID, "null server");
     ag_ctx* ctx = s->ctx;
     // admit FIFO while the request's film and any scene it brings fit
@@ -1507,7 +1522,11 @@ extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queu
         s->active.push_back(k);
         s->queue.erase(s->queue.begin());
     }
-    if (!s->active.empty()) {
+    const uint64_t extra_samples =
+        extra ? (uint64_t)extra->count * extra->film->w * extra->film->h : 0;
+    if (extra_rc) *extra_rc = AG_OK;
+    if (extra_ms) *extra_ms = 0.0;
+    if (!s->active.empty() || extra) {
         s->steps++;
         // choose: no preview yet first, then least work left, within the step budget
         std::vector<uint64_t> order = s->active;
@@ -1527,7 +1546,7 @@ extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queu
         // sequence, and so its image, does not depend on the steps it skips.
         // After SERVER_PREVIEW_RUN_MAX such steps in a row the requests in
         // progress go first, least recently rendered first.
-        bool preview_step = waiting(order.front());
+        bool preview_step = !order.empty() && waiting(order.front());
         if (preview_step && s->preview_run >= SERVER_PREVIEW_RUN_MAX) {
             std::vector<uint64_t> prog;
             for (uint64_t k : order)
@@ -1543,26 +1562,42 @@ extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queu
         }
         s->preview_run = preview_step ? s->preview_run + 1 : 0;
         std::vector<uint64_t> chosen;
-        uint64_t room = s->step_samples;
+        uint64_t room = s->step_samples > extra_samples ? s->step_samples - extra_samples : 0;
         for (uint64_t k : order) {
             const auto& q = s->reqs[k];
             if (preview_step && !waiting(k)) break;
             uint64_t cost = (uint64_t)q.next_chunk * q.r.width * q.r.height;
-            if (!chosen.empty() && cost > room) continue;
+            // a live pass may fill the step; requests awaiting a first preview still join it
+            if ((!chosen.empty() || (extra && !waiting(k))) && cost > room) continue;
             chosen.push_back(k);
             room = cost >= room ? 0 : room - cost;
             if (room == 0) break;
         }
-        if (server_render(s, chosen) == AG_OK) {
+        uint64_t total = extra_samples;  // samples of this step, for the live pass's share of its time
+        for (uint64_t k : chosen) total += (uint64_t)s->reqs[k].next_chunk * s->reqs[k].r.width * s->reqs[k].r.height;
+        const auto t_step = std::chrono::steady_clock::now();
+        if (server_render(s, chosen, extra) == AG_OK) {
             server_progress(s, chosen);
+            if (extra_ms) {
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_step).count();
+                *extra_ms = total ? ms * (double)extra_samples / (double)total : ms;
+            }
         } else {
             // one request must not fail the others: retry each alone, fail
             // those that fail alone (with the pass's error), keep the rest
             const std::string first_error = ag_last_error();
             server_clear_acc(s, chosen);
+            if (extra) {
+                launch_zero_u32(s->ctx->stream, extra->film->acc.as<uint32_t>(),
+                                (uint32_t)((size_t)extra->film->w * extra->film->h * 9));
+                const auto t1 = std::chrono::steady_clock::now();
+                int rc = server_render(s, {}, extra);
+                if (extra_rc) *extra_rc = rc;
+                if (extra_ms) *extra_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+            }
             if (chosen.size() == 1) {
                 server_fail(s, chosen[0], first_error);
-            } else {
+            } else if (!chosen.empty()) {
                 for (uint64_t k : chosen) {
                     std::vector<uint64_t> one{k};
                     if (server_render(s, one) == AG_OK) {
@@ -1583,6 +1618,10 @@ extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queu
     if (n_active) *n_active = (uint32_t)s->active.size();
     if (n_queued) *n_queued = (uint32_t)s->queue.size();
     return AG_OK;
+}
+
+extern "C" int ag_server_step(ag_server* s, uint32_t* n_active, uint32_t* n_queued) {
+    return server_step(s, nullptr, n_active, n_queued, nullptr, nullptr);
 }
 
 extern "C" int ag_server_status(const ag_server* s, uint64_t id, ag_request_status* out) {
@@ -1909,6 +1948,7 @@ struct ag_live {
     uint64_t published_gen = 0;
     bool burst = false;
     bool settled = false;  // refinement stopped early for settled_max
+    ag_server* server = nullptr;  // attached: passes ride in this server's steps
     uint32_t settled_max = 0;
     double dn_ms_per_mpx = 0.0;
     // the last published linear image (live.rs:493-502: Linear)
@@ -2010,6 +2050,20 @@ int live_publish(ag_live* L, ag_film* f, bool is_half, bool converged, double no
 }
 
 int live_render(ag_live* L, ag_film* f, uint32_t count, double* ms) {
+    if (L->server) {
+        // one server step: this pass first, the finals' chunks in the rest of
+        // the step budget; the session is charged its samples' share of the time
+        if ((uint64_t)f->samples + count > 0xFFFFFFFFull) return fail(AG_E_INVALID, "sample index overflow");
+        ViewJob j;
+        j.scene = L->scene;
+        j.film = f;
+        j.count = count;
+        j.first = f->samples;
+        make_dview(j.dv, L->scene, &L->cam, L->scene->turntable_deg, -1, f, LIVE_SEED, &L->p.ps);
+        int pass_rc = AG_OK;
+        int rc = server_step(L->server, &j, nullptr, nullptr, &pass_rc, ms);
+        return rc ? rc : pass_rc;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     int rc = ag_render_pass(L->ctx, L->scene, &L->cam, f, &L->p.ps, count, LIVE_SEED);
     *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2048,6 +2102,13 @@ extern "C" int ag_live_create(ag_ctx* ctx, ag_live** out) {
 }
 
 extern "C" void ag_live_destroy(ag_live* L) { delete L; }
+
+extern "C" int ag_live_attach(ag_live* L, ag_server* srv) {
+    if (!L) return fail(AG_E_INVALID, "null live session");
+    if (srv && srv->ctx != L->ctx) return fail(AG_E_INVALID, "server belongs to another context");
+    L->server = srv;
+    return AG_OK;
+}
 
 extern "C" int ag_live_set(ag_live* L, const ag_scene* scene, const ag_camera* cam, const ag_live_params* p,
                            double now_ms) {

@@ -499,6 +499,14 @@ typedef struct ag_live_frame {
 
 int ag_live_create(ag_ctx* ctx, ag_live** out);
 void ag_live_destroy(ag_live* live);
+/* Attach the session to a continuous-batching server (NULL detaches): its
+ * passes then run inside that server's steps, in the same wavefront schedule
+ * as the queued requests' chunks, which take what the pass leaves of the step
+ * budget.  The reference instead pauses the live view while a queued job
+ * renders (live.rs:344-352, queue.rs:511-515).  The session's frames are the
+ * same either way; its pacing is charged the pass's share of the step's time
+ * by samples.  The caller drives both from one thread. */
+int ag_live_attach(ag_live* live, struct ag_server* server);
 /* A change of scene, camera or params: a new generation (accumulation restarts).
  * The session keeps a pointer to `scene`, which must outlive its use (until
  * the next ag_live_set or ag_live_destroy). */

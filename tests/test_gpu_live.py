@@ -161,3 +161,50 @@ def test_look_only_change_redevelops(ctx, piece_scene):
     assert f2["redeveloped"] and f2["published"] and f2["pass_spp"] == 0 and f2["spp"] == f["spp"]
     assert np.array_equal(img2, _hand_frame(ctx, sc, cam, W, H, passes, "khronos", 0.5))
     lv.close()
+
+
+def test_live_session_on_the_server_shares_steps_with_finals(ctx, piece_scene):
+    """A session attached to a continuous-batching server: its passes run in
+    the server's steps beside the queued requests' chunks (the reference
+    pauses the live view instead, live.rs:344-352).  Its frames equal the same
+    passes run by hand, the requests progress while it refines, and each
+    request's image equals its render alone."""
+    from argent_amd import _abi as A
+    from argent_amd.serve import Server
+    wl, sc, cam = piece_scene
+    ps = workloads.path_settings(8)
+    reqs = [dict(scene=sc, camera=cam, width=40, height=48, spp=24, seed=3),
+            dict(scene=sc, camera=cam, width=32, height=32, spp=40, seed=4,
+                 material0=ag.Material.metal("rose14k", 0.07, 0.8))]
+    srv = Server(ctx, 64 << 30, 1 << 16, ps)
+    ids = [srv.submit(**r) for r in reqs]
+    lv = ag.Live(ctx)
+    lv.attach(srv)
+    lv.set(sc, cam, W, H, max_spp=16, now_ms=0.0)
+    f, img = lv.step(now_ms=0.0)
+    assert f["published"] and f["half"]
+    assert np.array_equal(img, _hand_frame(ctx, sc, cam, W // 2, H // 2, [2]))
+    t, passes, progressed = 1000.0, [], False
+    while not f["converged"]:
+        f, img = lv.step(now_ms=t)
+        passes.append(f["pass_spp"])
+        progressed |= any(srv.status(i)["samples"] > 0 for i in ids)
+        t += 2000.0
+    assert progressed, "the requests sat out the live session's steps"
+    assert f["spp"] == 16 and np.array_equal(img, _hand_frame(ctx, sc, cam, W, H, passes))
+    lv.attach(None)
+    for _ in range(1000):
+        a, q = srv.step()
+        if a == 0 and q == 0:
+            break
+    imgs = [srv.read(i) for i in ids]
+    assert all(srv.status(i)["state"] == A.REQ_DONE for i in ids)
+    srv.close()
+    for r, img_mixed in zip(reqs, imgs):
+        alone = Server(ctx, 64 << 30, 1 << 16, ps)
+        i = alone.submit(**r)
+        while alone.step() != (0, 0):
+            pass
+        assert np.array_equal(alone.read(i), img_mixed)
+        alone.close()
+    lv.close()
