@@ -72,6 +72,9 @@ def lib():
             ("or_normals", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f, _f, _u32, _u32, _u32, _u64]),
             ("or_weld", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f, _u32, _u32, _u32, _u64, _f]),
             ("or_compact", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, _u32, _u32, _u32, _u64]),
+            ("or_concavity", C.c_int, [_f, _f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f]),
+            ("or_smooth", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, C.c_uint32, C.c_float]),
+            ("or_patina_finish", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, _f, C.c_float, C.c_uint32, _f]),
             ("or_load_stl", C.c_int, [C.c_char_p, C.c_size_t, C.c_int32, C.c_float, C.c_float, _f, _u32, _u32,
                                       _u32, _u64, _f]),
         ]
@@ -396,3 +399,55 @@ def normals(pos, tris, crease_angle_deg=50.0):
         raise IndexError("vertex index out of range")
     return (op[:nv.value].copy(), on[:nv.value].copy(), ot[:nt.value].copy(),
             dict(zip(NORMAL_FIELDS, (int(x) for x in rep))))
+
+
+def concavity(pos, nrm, tris, radius):
+    """cavity.rs:130-243: signed concavity per vertex, float32 [n_vertices]."""
+    pos, tris = _mesh_in(pos, tris)
+    nrm = np.ascontiguousarray(np.asarray(nrm, np.float32).reshape(-1, 3))
+    out = np.zeros(len(pos), np.float32)
+    lib().or_concavity(_p(pos), _p(nrm), len(pos), _p(tris, C.c_uint32), len(tris), radius, _p(out))
+    return out
+
+
+def smooth(pos, tris, values, iterations, lam=0.5):
+    """cavity.rs:245-296: Laplacian smoothing; returns the smoothed copy."""
+    pos, tris = _mesh_in(pos, tris)
+    v = np.ascontiguousarray(np.asarray(values, np.float32)).copy()
+    assert v.size == len(pos), "smooth: one value per vertex"
+    lib().or_smooth(_p(pos), len(pos), _p(tris, C.c_uint32), len(tris), _p(v), iterations, lam)
+    return v
+
+
+def patina_mask(pos, nrm, tris, ao_samples=48, ao_distance=1.2, curvature_radius=0.4, concavity_weight=0.35,
+                smooth_iterations=3, mode=MODE_ROBUST, ao=None):
+    """cavity.rs:52-80 with the mesh's own BVH as occluder (the scene's
+    Bvh::occluded); `ao` may carry a precomputed AO term.  float32 [n_vertices]."""
+    pos, tris = _mesh_in(pos, tris)
+    nrm = np.ascontiguousarray(np.asarray(nrm, np.float32).reshape(-1, 3))
+    nv = len(pos)
+    if nv == 0 or len(tris) == 0:
+        return np.zeros(nv, np.float32)
+    if ao is None:
+        if ao_samples > 0 and ao_distance > 0.0:
+            d = _piece_desc(pos, tris, nrm)
+            sc = OracleScene(d)
+            ao = sc.ambient_occlusion(nv, ao_samples, ao_distance, mode)
+            sc.close()
+        else:
+            ao = np.zeros(nv, np.float32)
+    ao = np.ascontiguousarray(np.asarray(ao, np.float32))
+    w = min(max(concavity_weight, 0.0), 1.0)
+    cav = concavity(pos, nrm, tris, curvature_radius) if w > 0.0 and curvature_radius > 0.0 else np.zeros(nv, np.float32)
+    out = np.zeros(nv, np.float32)
+    lib().or_patina_finish(_p(pos), nv, _p(tris, C.c_uint32), len(tris), _p(ao), _p(cav), concavity_weight,
+                           smooth_iterations, _p(out))
+    return out
+
+
+def _piece_desc(pos, tris, nrm):
+    """A scene holding only the piece (one sterling material), for AO."""
+    import argent_amd as ag
+    d = ag.SceneDesc(pos, tris, normals=nrm)
+    d.add_material(ag.Material.metal("sterling"))
+    return d

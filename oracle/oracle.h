@@ -113,6 +113,17 @@ int or_compact(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt,
 // non-manifold and inconsistent edges, degenerate faces, split vertices.
 int or_normals(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, float crease_deg, float* out_pos,
                float* out_nrm, uint32_t* out_nv, uint32_t* out_tris, uint32_t* out_nt, uint64_t* rep9);
+/* Patina mask (cavity.cpp): argent-mesh cavity.rs.  or_concavity: signed
+ * concavity per vertex (cavity.rs:130-243) from the given vertex normals;
+ * or_smooth: Laplacian smoothing in place with crease copies as one
+ * (cavity.rs:245-296); or_patina_finish: the blend of AO and concavity,
+ * smoothing and clamp of patina_mask (cavity.rs:64-79). */
+int or_concavity(const float* pos, const float* nrm, uint32_t nv, const uint32_t* tris, uint32_t nt, float radius,
+                 float* out);
+int or_smooth(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, float* values, uint32_t iterations,
+              float lambda);
+int or_patina_finish(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, const float* ao,
+                     const float* cav, float concavity_weight, uint32_t smooth_iterations, float* out);
 int or_load_stl(const uint8_t* b, size_t len, int32_t do_weld, float rel_tolerance, float unit_scale, float* out_pos,
                 uint32_t* out_nv, uint32_t* out_tris, uint32_t* out_nt, uint64_t* rep10, float* tol_mm);
 
