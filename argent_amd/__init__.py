@@ -307,7 +307,16 @@ class Scene:
         desc._bind()
         check(A.lib().ag_scene_create_from_mesh(ctx.h, mesh.h, C.byref(desc.d), 1 if gpu_bvh else 0, C.byref(h)))
         sc.h = h
+        sc._nv = mesh.counts()[0]
         return sc
+
+    @property
+    def n_vertices(self):
+        root = self
+        while root.base is not None:
+            root = root.base
+        nv = getattr(root, "_nv", None)
+        return nv if nv is not None else len(root.desc.positions)
 
     def stats(self):
         n, l, dp = C.c_uint32(), C.c_uint32(), C.c_uint32()
@@ -347,6 +356,39 @@ class Scene:
         out = np.zeros(len(root.desc.positions), np.float32)
         check(A.lib().ag_ambient_occlusion(self.ctx.h, self.h, samples, distance, _ptr(out)))
         return out
+
+    def patina_settings(self, **overrides):
+        """PatinaSettings sized by the piece as Prepared::build does
+        (shot.rs:133-141), with `overrides` applied: a dict."""
+        ps = A.PatinaSettings()
+        check(A.lib().ag_patina_for_scene(self.h, C.byref(ps)))
+        d = {k: getattr(ps, k) for k, _ in A.PatinaSettings._fields_}
+        d.update(overrides)
+        return d
+
+    def patina_mask(self, settings=None, store=False, parts=False):
+        """patina_mask (cavity.rs:52-80) on the GPU (ag_patina_mask): float32
+        [n_vertices] in [0, 1]; `settings` a dict (default: PatinaSettings::default()).
+        store: the mask becomes the piece's cavity for shading (Piece::cavity).
+        parts: also return the AO fraction and the concavity."""
+        ps = A.PatinaSettings()
+        check(A.lib().ag_patina_default(C.byref(ps)))
+        for k, v in (settings or {}).items():
+            setattr(ps, k, v)
+        nv = self.n_vertices
+        mask = np.zeros(nv, np.float32)
+        ao = np.zeros(nv, np.float32) if parts else None
+        cav = np.zeros(nv, np.float32) if parts else None
+        check(A.lib().ag_patina_mask(self.ctx.h, self.h, C.byref(ps), 1 if store else 0, _ptr(mask),
+                                     _ptr(ao) if parts else None, _ptr(cav) if parts else None))
+        return (mask, ao, cav) if parts else mask
+
+    def smooth_values(self, values, iterations, lam=0.5):
+        """smooth (cavity.rs:245-296) over the piece on the GPU: a smoothed copy."""
+        v = np.ascontiguousarray(np.asarray(values, np.float32)).copy()
+        assert v.size == self.n_vertices, "one value per vertex"
+        check(A.lib().ag_smooth_values(self.ctx.h, self.h, _ptr(v), iterations, lam))
+        return v
 
     def bsdf_probe(self, mat, lam, mask, u, wo, wi):
         mat = np.ascontiguousarray(np.asarray(mat, np.int32))

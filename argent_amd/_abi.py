@@ -155,6 +155,11 @@ class Tuning(C.Structure):
                 ("ordered_emission", C.c_uint32), ("cursor_slices", C.c_uint32)]
 
 
+class PatinaSettings(C.Structure):  # ag_patina_settings (cavity.rs:24-42)
+    _fields_ = [("ao_samples", C.c_uint32), ("ao_distance", C.c_float), ("curvature_radius", C.c_float),
+                ("concavity_weight", C.c_float), ("smooth_iterations", C.c_uint32)]
+
+
 class AutotuneResult(C.Structure):
     _fields_ = [("best", Tuning), ("ms_start", C.c_double), ("ms_best", C.c_double), ("trials", C.c_uint32)]
 
@@ -231,6 +236,10 @@ _SIGS = [
     ("ag_intersect", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, C.POINTER(HitsSoA)]),
     ("ag_occluded", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, u8p]),
     ("ag_ambient_occlusion", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, f32p]),
+    ("ag_patina_default", C.c_int, [C.POINTER(PatinaSettings)]),
+    ("ag_patina_for_scene", C.c_int, [C.c_void_p, C.POINTER(PatinaSettings)]),
+    ("ag_patina_mask", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PatinaSettings), C.c_int32, f32p, f32p, f32p]),
+    ("ag_smooth_values", C.c_int, [C.c_void_p, C.c_void_p, f32p, C.c_uint32, C.c_float]),
     ("ag_intersect_device", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32,
                                       C.POINTER(HitsSoA)]),
     ("ag_intersect_device_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32,

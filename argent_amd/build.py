@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libargent_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "post.hip", "mesh.hip", "bvh.hip", "normals.hip"]
+HIP_SOURCES = ["kernels.hip", "post.hip", "mesh.hip", "bvh.hip", "normals.hip", "cavity.hip"]
 CPP_SOURCES = ["runtime.cpp", "host_build.cpp", "host_spectra.cpp", "host_bvh.cpp", "host_post.cpp", "autotune.cpp"]
 
 

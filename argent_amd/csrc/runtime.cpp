@@ -18,6 +18,7 @@
 #include "host_spectra.hpp"
 #include "kernels.hpp"
 #include "mesh_internal.hpp"
+#include "patina.hpp"
 #include "post.hpp"
 #include "tables/spectral_data.h"
 
@@ -1736,6 +1737,29 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
     return AG_OK;
 }
 
+// The AO rays of cavity.rs:95-128 on the context stream: blocked-ray counts
+// per vertex into `counts` (n_vertices words).  Returns after the launch.
+static int ao_counts(ag_ctx* ctx, const Geometry& g, uint32_t n, float distance, DevBuf& counts) {
+    const uint32_t nv = g.n_vertices;
+    if ((uint64_t)nv * n > 0xFFFFFFFFull) return fail(AG_E_INVALID, "n_vertices * samples exceeds 2^32 - 1 rays");
+    int rc = ensure_spill(ctx);
+    if (rc) return rc;
+    if ((rc = ctx->cursor_buf.ensure(CURSOR_SET * sizeof(uint32_t)))) return rc;
+    // cavity.rs:87-90 and :97: eps from the diagonal of the vertex bounds, in f32
+    float dx = g.vhi[0] - g.vlo[0], dy = g.vhi[1] - g.vlo[1], dz = g.vhi[2] - g.vlo[2];
+    const float diag = std::sqrt((dx * dx + dy * dy) + dz * dz);
+    const float eps = std::fmax(2e-4f * diag, 1e-3f);
+    if ((rc = counts.ensure((size_t)nv * sizeof(uint32_t)))) return rc;
+    hipStream_t st = ctx->stream;
+    launch_zero_u32(st, counts.as<uint32_t>(), nv);
+    launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
+    launch_ao(st, ctx->grid_batch, g.nodes.as<float4>(), g.tris.as<float4>(), g.pos.as<float>(), g.nrm.as<float>(), n,
+              distance, eps, counts.as<uint32_t>(), nv * n, ctx->cursor_buf.as<uint32_t>(), ctx->tuning.batch_refill_min,
+              ctx->tuning.spread_min_lanes, ctx->spill_buf.as<uint2>());
+    HIPCHK(hipGetLastError());
+    return AG_OK;
+}
+
 extern "C" int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t samples, float distance, float* out) {
     if (!ctx || !s || !out) return fail(AG_E_INVALID, "null argument");
     const Geometry& g = *s->geo;
@@ -1747,32 +1771,127 @@ extern "C" int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t sam
     }
     if (!g.has_nrm) return fail(AG_E_INVALID, "ambient occlusion needs the scene's vertex normals");
     const uint32_t n = std::max(samples, 1u);
-    if ((uint64_t)nv * n > 0xFFFFFFFFull) return fail(AG_E_INVALID, "n_vertices * samples exceeds 2^32 - 1 rays");
     HIPCHK(hipSetDevice(ctx->device));
-    int rc = ensure_spill(ctx);
-    if (rc) return rc;
-    if ((rc = ctx->cursor_buf.ensure(CURSOR_SET * sizeof(uint32_t)))) return rc;
-    // cavity.rs:87-90 and :97: eps from the diagonal of the vertex bounds, in f32
-    float dx = g.vhi[0] - g.vlo[0], dy = g.vhi[1] - g.vlo[1], dz = g.vhi[2] - g.vlo[2];
-    const float diag = std::sqrt((dx * dx + dy * dy) + dz * dz);
-    const float eps = std::fmax(2e-4f * diag, 1e-3f);
-    DevBuf counts;
-    if ((rc = counts.ensure((size_t)nv * sizeof(uint32_t)))) return rc;
     hipStream_t st = ctx->stream;
-    launch_zero_u32(st, counts.as<uint32_t>(), nv);
-    launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
+    DevBuf counts;
     HIPCHK(hipEventRecord(ctx->ev[2], st));
-    launch_ao(st, ctx->grid_batch, g.nodes.as<float4>(), g.tris.as<float4>(), g.pos.as<float>(), g.nrm.as<float>(), n,
-              distance, eps, counts.as<uint32_t>(), nv * n, ctx->cursor_buf.as<uint32_t>(), ctx->tuning.batch_refill_min,
-              ctx->tuning.spread_min_lanes, ctx->spill_buf.as<uint2>());
+    int rc = ao_counts(ctx, g, n, distance, counts);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->ev[3], st));
-    HIPCHK(hipGetLastError());
     std::vector<uint32_t> h(nv);
     HIPCHK(hipMemcpyAsync(h.data(), counts.p, (size_t)nv * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->last_query_ms = ms;
     for (uint32_t v = 0; v < nv; v++) out[v] = (float)h[v] / (float)n;  // cavity.rs:126
+    return AG_OK;
+}
+
+// ------------------------------------------------------------------ patina mask
+extern "C" int ag_patina_default(ag_patina_settings* out) {
+    if (!out) return fail(AG_E_INVALID, "null argument");
+    *out = ag_patina_settings{48u, 1.2f, 0.4f, 0.35f, 3u};  // cavity.rs:38-42
+    return AG_OK;
+}
+
+extern "C" int ag_patina_for_scene(const ag_scene* s, ag_patina_settings* out) {
+    if (!s || !out) return fail(AG_E_INVALID, "null argument");
+    ag_patina_default(out);
+    // shot.rs:133-141: sized by the piece's vertex bounds, in f32
+    const Geometry& g = *s->geo;
+    const float hx = (g.vhi[0] - g.vlo[0]) * 0.5f, hy = (g.vhi[1] - g.vlo[1]) * 0.5f, hz = (g.vhi[2] - g.vlo[2]) * 0.5f;
+    const float len = std::sqrt(hx * hx + hy * hy + hz * hz);
+    const float radius = len > 1e-3f ? len : 1e-3f;  // f32::max(len, 1e-3): NaN gives 1e-3
+    auto clampf = [](float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); };
+    out->ao_distance = clampf(radius * 0.09f, 0.4f, 2.5f);
+    out->curvature_radius = clampf(radius * 0.03f, 0.15f, 1.0f);
+    return AG_OK;
+}
+
+static int patina_inputs(const ag_scene* s, PatinaInputs* in) {
+    const Geometry& g = *s->geo;
+    *in = PatinaInputs{};
+    in->pos = g.pos.as<float>();
+    in->nrm = g.has_nrm ? g.nrm.as<float>() : nullptr;
+    in->tris = g.idx.as<uint32_t>();
+    in->nv = g.n_vertices;
+    in->nt = g.n_triangles;
+    for (int k = 0; k < 3; k++) in->lo[k] = g.vlo[k];
+    in->lambda = 0.5f;
+    return AG_OK;
+}
+
+extern "C" int ag_patina_mask(ag_ctx* ctx, ag_scene* s, const ag_patina_settings* settings, int32_t store, float* mask,
+                              float* ao, float* concavity) {
+    if (!ctx || !s || !settings) return fail(AG_E_INVALID, "null argument");
+    if (s->ctx != ctx) return fail(AG_E_INVALID, "scene belongs to another context");
+    const ag_patina_settings& P = *settings;
+    if (!std::isfinite(P.ao_distance) || !std::isfinite(P.curvature_radius) || !std::isfinite(P.concavity_weight))
+        return fail(AG_E_INVALID, "patina settings must be finite");
+    Geometry& g = *s->geo;
+    const uint32_t nv = g.n_vertices;
+    if (nv == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DevBuf out, counts;
+    int rc = out.ensure(3 * 4 * (size_t)nv);
+    if (rc) return rc;
+    float* d_mask = out.as<float>();
+    float* d_ao = d_mask + nv;
+    float* d_cav = d_ao + nv;
+    const bool with_ao = P.ao_samples > 0 && P.ao_distance > 0.0f;
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    if (g.n_triangles == 0) {  // cavity.rs:54-56
+        HIPCHK(hipMemsetAsync(out.p, 0, 3 * 4 * (size_t)nv, st));
+    } else {
+        if (!g.has_nrm) return fail(AG_E_INVALID, "the patina mask needs the scene's vertex normals");
+        PatinaInputs in;
+        patina_inputs(s, &in);
+        if (with_ao) {
+            if ((rc = ao_counts(ctx, g, P.ao_samples, P.ao_distance, counts))) return rc;
+            in.ao_counts = counts.as<uint32_t>();
+            in.ao_samples = P.ao_samples;
+        }
+        in.clamp01 = 1;
+        if ((rc = patina_device(ctx, in, P, d_mask, d_ao, d_cav))) return rc;
+    }
+    HIPCHK(hipEventRecord(ctx->ev[3], st));
+    if (store) {
+        if ((rc = g.cavity.ensure(4 * (size_t)nv))) return rc;
+        HIPCHK(hipMemcpyAsync(g.cavity.p, d_mask, 4 * (size_t)nv, hipMemcpyDeviceToDevice, st));
+        g.has_cavity = true;
+        s->hs.cavity = g.cavity.as<float>();
+        HIPCHK(hipMemcpyAsync(s->d_scene.p, &s->hs, sizeof(DScene), hipMemcpyHostToDevice, st));
+    }
+    float* host[3] = {mask, ao, concavity};
+    for (int k = 0; k < 3; k++)
+        if (host[k]) HIPCHK(hipMemcpyAsync(host[k], d_mask + k * (size_t)nv, 4 * (size_t)nv, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->last_query_ms = ms;
+    return AG_OK;
+}
+
+extern "C" int ag_smooth_values(ag_ctx* ctx, const ag_scene* s, float* values, uint32_t iterations, float lambda) {
+    if (!ctx || !s || !values) return fail(AG_E_INVALID, "null argument");
+    if (s->ctx != ctx) return fail(AG_E_INVALID, "scene belongs to another context");
+    const uint32_t nv = s->geo->n_vertices;
+    if (nv == 0) return AG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DevBuf buf;
+    int rc = buf.ensure(2 * 4 * (size_t)nv);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(buf.p, values, 4 * (size_t)nv, hipMemcpyHostToDevice, st));
+    PatinaInputs in;
+    patina_inputs(s, &in);
+    in.values = buf.as<float>();
+    in.lambda = lambda;
+    ag_patina_settings P{};
+    P.smooth_iterations = iterations;
+    if ((rc = patina_device(ctx, in, P, buf.as<float>() + nv, nullptr, nullptr))) return rc;
+    HIPCHK(hipMemcpyAsync(values, buf.as<float>() + nv, 4 * (size_t)nv, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return AG_OK;
 }
 

@@ -337,9 +337,43 @@ int ag_intersect_device_noalloc(ag_ctx* ctx, const ag_scene* s, const ag_rays_so
  * p + n * max(2e-4 * |diagonal of the vertex bounds|, 1e-3), that are blocked
  * within `distance` (object space).  Needs the scene's vertex normals; a zero
  * normal gives 0.  out: n_vertices floats (host).  This is the ray-traced step
- * of the reference's patina mask (cavity.rs:52-79); its concavity and
- * smoothing steps are mesh processing on the host. */
+ * of the reference's patina mask (cavity.rs:52-79); ag_patina_mask runs the
+ * whole mask. */
 int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t samples, float distance, float* out);
+
+/* The patina mask: PatinaSettings (argent-mesh cavity.rs:24-42). */
+typedef struct ag_patina_settings {
+    uint32_t ao_samples;        /* AO rays per vertex (48); 0 = no AO term */
+    float ao_distance;          /* how far a recess sees, object units (1.2); <= 0 = no AO term */
+    float curvature_radius;     /* concavity neighbourhood radius (0.4) */
+    float concavity_weight;     /* blend of AO (0) and concavity (1), clamped to [0, 1] (0.35) */
+    uint32_t smooth_iterations; /* Laplacian passes over the mask (3) */
+} ag_patina_settings;
+/* PatinaSettings::default() (cavity.rs:38-42). */
+int ag_patina_default(ag_patina_settings* out);
+/* The settings Prepared::build sizes by the piece (shot.rs:133-141):
+ * radius = max(|half the vertex-bounds diagonal|, 1e-3), ao_distance =
+ * clamp(0.09 radius, 0.4, 2.5), curvature_radius = clamp(0.03 radius, 0.15,
+ * 1), the rest default. */
+int ag_patina_for_scene(const ag_scene* s, ag_patina_settings* out);
+/* patina_mask (cavity.rs:52-80) of the scene's piece with the piece itself as
+ * the Occluder (scene.rs:40-44), all on the device: one value in [0, 1] per
+ * vertex, 1 = deep recess.  The AO term is ag_ambient_occlusion's; the
+ * concavity (cavity.rs:130-243) is radius x mean curvature from a Gaussian
+ * neighbourhood over grid cells; the blend is smoothstep-shaped; the
+ * Laplacian smoothing (cavity.rs:245-296) treats copies of a vertex at one
+ * position (crease splits) as one.  Needs the scene's vertex normals.
+ * store != 0: the mask becomes the piece's cavity (Piece::cavity, shot.rs:142)
+ * that shading reads (the patina layer's weight, material.rs:113-121), for
+ * this scene and for scenes created over the same piece afterwards
+ * (ag_scene_create_shared); scenes created over it earlier keep the cavity
+ * they had.  mask / ao / concavity: host arrays of n_vertices floats (the
+ * mask, the AO fraction, the clamped concavity), each may be NULL. */
+int ag_patina_mask(ag_ctx* ctx, ag_scene* s, const ag_patina_settings* settings, int32_t store, float* mask,
+                   float* ao, float* concavity);
+/* smooth (cavity.rs:245-296) of per-vertex values over the scene's piece, in
+ * place: x <- x + lambda (mean of the neighbours - x), `iterations` times. */
+int ag_smooth_values(ag_ctx* ctx, const ag_scene* s, float* values, uint32_t iterations, float lambda);
 
 /* Preview development (argent-post), on the device.
  * Tone operators and looks of tonemap.rs:41-67 (Grade): */
