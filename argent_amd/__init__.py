@@ -307,7 +307,7 @@ class Scene:
         desc._bind()
         check(A.lib().ag_scene_create_from_mesh(ctx.h, mesh.h, C.byref(desc.d), 1 if gpu_bvh else 0, C.byref(h)))
         sc.h = h
-        sc._nv = mesh.counts()[0]
+        sc._nv = mesh.counts[0]
         return sc
 
     @property

@@ -95,6 +95,7 @@ def main():
     piece = wl["piece"]
     with Context(0) as ctx:
         scene = Scene(ctx, piece.scene(wl["metal"], 0.07, 0.8, wl["studio"]))
+        piece.prepare_patina(scene)
         defaults = ctx.tuning
         for cls in args.classes.split(","):
             t0 = time.time()

@@ -143,6 +143,8 @@ def run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, d
             p = pieces[key[0]]
             d = p.scene("sterling", 0.07, 0.8, key[1])
             sc = Scene(ctx, d, share_mesh_with=bases.get(key[0]))
+            if key[0] not in bases:
+                p.prepare_patina(sc)  # piece preparation, before any look of it
             bases.setdefault(key[0], sc)
             scenes[key] = (sc, p, d)
     mats = {a: Material.metal(a, 0.07, 0.8) for a in ALLOYS}
@@ -245,6 +247,8 @@ def serial_plain(ctx, trace, pieces, depth=12, budget_s=30.0, log=lambda *a: Non
         key = (mi, r["studio"], r["alloy"])
         if key not in looks:
             sc = Scene(ctx, p.scene(r["alloy"], 0.07, 0.8, r["studio"]), share_mesh_with=bases.get(mi))
+            if mi not in bases:
+                p.prepare_patina(sc)
             bases.setdefault(mi, sc)
             looks[key] = sc
         sc = looks[key]

@@ -7,10 +7,11 @@
                   1080 x 1920, 256 spp, depth 12, no depth of field
 
 The pieces are generated already posed (band around Z, head toward +Y), as
-`Prepared::build` with a given pose does (shot.rs:104-147); no cavity mask
-is computed, so the patina layer never shows (its blend weight is
-smoothstep of a zero mask).  Lens: 100 mm, f/16 outside the white studio,
-no depth of field in it (jobs.rs:403-406).
+`Prepared::build` with a given pose does (shot.rs:104-147); its patina step
+(shot.rs:133-143) is Piece.prepare_patina, which computes the cavity mask on
+the device and stores it on the piece, so the patina layer (strength 0.8)
+shows in the recesses.  Lens: 100 mm, f/16 outside the white studio, no
+depth of field in it (jobs.rs:403-406).
 """
 import numpy as np
 
@@ -50,6 +51,25 @@ class Piece:
         p.floor_y = float(lo[1])
         p.hull = p.positions.copy()
         return p
+
+    def patina_settings(self):
+        """PatinaSettings sized by the piece (shot.rs:133-141), in f32 as the
+        reference computes them: a dict for the oracle's patina_mask."""
+        lo = np.float32([self.positions[:, k].min() for k in range(3)])
+        hi = np.float32([self.positions[:, k].max() for k in range(3)])
+        h = (hi - lo) * np.float32(0.5)
+        r = max(np.sqrt(np.float32(h[0] * h[0] + h[1] * h[1] + h[2] * h[2])), np.float32(1e-3))
+        return dict(ao_samples=48, ao_distance=float(np.clip(r * np.float32(0.09), np.float32(0.4), np.float32(2.5))),
+                    curvature_radius=float(np.clip(r * np.float32(0.03), np.float32(0.15), np.float32(1.0))),
+                    concavity_weight=0.35, smooth_iterations=3)
+
+    def prepare_patina(self, scene):
+        """Prepared::build's patina step (shot.rs:133-143) on the device: the
+        mask with settings sized by the piece, stored as the piece's cavity
+        in `scene` (and in scenes created over it later); a host copy stays on
+        the piece for scenes described from it later and for the oracle."""
+        self.cavity = scene.patina_mask(scene.patina_settings(), store=True)
+        return self.cavity
 
     def scene(self, metal="sterling", roughness=0.07, patina=0.8, studio="noir", gems=()):
         """Prepared::scene (shot.rs:206-245): slot 0 metal, then gems, then
