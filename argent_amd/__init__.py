@@ -122,8 +122,14 @@ class Context:
         self.close()
 
     def last_query_ms(self):
-        """Kernel time of the last ray query or cavity AO call (HIP events), ms."""
+        """Kernel time of the last ray query, cavity AO or patina mask call (HIP events), ms."""
         return A.lib().ag_last_query_ms(self.h)
+
+    def graph_stats(self):
+        """(captures, replays) of render passes in HIP graphs (tuning launch_graph)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        check(A.lib().ag_ctx_graph_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def synchronize(self):
         check(A.lib().ag_ctx_synchronize(self.h))

@@ -152,7 +152,8 @@ class Tuning(C.Structure):
                 ("blocks_per_cu", C.c_uint32), ("sort_by_material", C.c_uint32),
                 ("refill_min", C.c_uint32), ("batch_blocks_per_cu", C.c_uint32), ("batch_refill_min", C.c_uint32),
                 ("shadow_refill_min", C.c_uint32), ("spread_min_lanes", C.c_uint32),
-                ("ordered_emission", C.c_uint32), ("cursor_slices", C.c_uint32)]
+                ("ordered_emission", C.c_uint32), ("cursor_slices", C.c_uint32),
+                ("launch_graph", C.c_uint32)]
 
 
 class PatinaSettings(C.Structure):  # ag_patina_settings (cavity.rs:24-42)
@@ -237,6 +238,7 @@ _SIGS = [
     ("ag_occluded", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RaysSoA), C.c_uint32, u8p]),
     ("ag_ambient_occlusion", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, f32p]),
     ("ag_patina_default", C.c_int, [C.POINTER(PatinaSettings)]),
+    ("ag_ctx_graph_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("ag_patina_for_scene", C.c_int, [C.c_void_p, C.POINTER(PatinaSettings)]),
     ("ag_patina_mask", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PatinaSettings), C.c_int32, f32p, f32p, f32p]),
     ("ag_smooth_values", C.c_int, [C.c_void_p, C.c_void_p, f32p, C.c_uint32, C.c_float]),

@@ -121,6 +121,16 @@ struct ag_ctx {
     DevBuf energy_buf;  // the library's GGX energy table, staged into LDS by the shading kernel
     double last_query_ms = 0.0;
     void* noalloc_scratch = nullptr;  // this device's address of `scratch`
+    // captured render passes (launch_graph): the bytes of every launch
+    // argument of a pass -> its executable graph, least recently used out
+    struct Graph {
+        std::vector<unsigned char> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    std::vector<Graph> graphs;
+    uint64_t graph_clock = 0;
+    uint64_t graph_captures = 0, graph_replays = 0;
     hipEvent_t next_event() {
         if (ev_used == evpool.size()) {
             hipEvent_t e;
@@ -248,6 +258,7 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     c->tuning.sort_by_material = 0;
     c->tuning.ordered_emission = 0;
     c->tuning.cursor_slices = CURSOR_SLICES;
+    c->tuning.launch_graph = 0;
     *out = c;
     return AG_OK;
 }
@@ -259,6 +270,7 @@ extern "C" void ag_ctx_destroy(ag_ctx* c) {
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->evpool) (void)hipEventDestroy(e);
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -272,6 +284,13 @@ extern "C" int ag_ctx_synchronize(ag_ctx* c) {
 extern "C" int ag_ctx_get_tuning(const ag_ctx* c, ag_tuning* t) {
     if (!c || !t) return fail(AG_E_INVALID, "null argument");
     *t = c->tuning;
+    return AG_OK;
+}
+
+extern "C" int ag_ctx_graph_stats(const ag_ctx* c, uint64_t* captures, uint64_t* replays) {
+    if (!c || !captures || !replays) return fail(AG_E_INVALID, "null argument");
+    *captures = c->graph_captures;
+    *replays = c->graph_replays;
     return AG_OK;
 }
 
@@ -290,6 +309,7 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
     if (t->ordered_emission > 1) return fail(AG_E_INVALID, "ordered_emission must be 0 or 1");
     if (t->cursor_slices < 1 || t->cursor_slices > (uint32_t)CURSOR_SLICES)
         return fail(AG_E_INVALID, "cursor_slices must be 1.." + std::to_string(CURSOR_SLICES));
+    if (t->launch_graph > 1) return fail(AG_E_INVALID, "launch_graph must be 0 or 1");
     for (uint32_t r : {t->refill_min, t->batch_refill_min, t->shadow_refill_min, t->spread_min_lanes})
         if (r < 1 || r > 64)
             return fail(AG_E_INVALID,
@@ -1065,7 +1085,6 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
     const size_t n_tally = 2 + 2 * (size_t)TALLY_DEPTHS;
     DevBuf& tally_buf = ev ? ev->tally : c->tally_buf;
     if ((rc = tally_buf.ensure(n_tally * sizeof(unsigned long long)))) return rc;
-    HIPCHK(hipMemsetAsync(tally_buf.p, 0, n_tally * sizeof(unsigned long long), st));
     auto timed = [&](int stage, int depth, auto&& launch) {
         if (!c->timing || ev) {
             launch();
@@ -1086,13 +1105,13 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
     const uint32_t spread = c->tuning.spread_min_lanes;
     auto t0 = std::chrono::steady_clock::now();
     uint32_t launches = 0;
-    for (auto& w : waves) {
-        uint32_t npaths = 0;
-        for (auto& sg : w) npaths = std::max(npaths, sg.begin + sg.npix * sg.ns);
-        HIPCHK(hipMemcpyAsync(c->seg_buf.p, w.data(), w.size() * sizeof(DSeg), hipMemcpyHostToDevice, st));
-        launch_zero_u32(st, c->Q.counters,
-                        (uint32_t)(counter_words(ps->max_depth) + (size_t)(ps->max_depth + 2) * CURSORS_PER_DEPTH));
-        timed(3, -1, [&] { launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), npaths, views, c->P, c->Q); });
+    const uint32_t n_counter_words =
+        (uint32_t)(counter_words(ps->max_depth) + (size_t)(ps->max_depth + 2) * CURSORS_PER_DEPTH);
+    // Every launch of one wave (its segments already in seg_buf).  Kernels
+    // only: no copies or memsets, so that the sequence can be captured.
+    auto enqueue_wave = [&](uint32_t nseg, uint32_t npaths) {
+        launch_zero_u32(st, c->Q.counters, n_counter_words);
+        timed(3, -1, [&] { launch_raygen(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)nseg, npaths, views, c->P, c->Q); });
         for (uint32_t d = 0; d < ps->max_depth; d++) {
             if (!fuse || d == 0)
                 timed(0, (int)d, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, spread, c->spill_buf.as<uint2>()); });
@@ -1105,15 +1124,86 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
                                      c->tuning.refill_min, spread, c->spill_buf.as<uint2>());
             else
                 timed(2, (int)d, [&] { launch_shadow(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.shadow_refill_min, spread, c->spill_buf.as<uint2>()); });
-            launches++;
         }
-        timed(4, -1, [&] { launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)w.size(), views, s->hs.cmf, c->P); });
+        timed(4, -1, [&] { launch_accumulate(st, c->grid_work, c->seg_buf.as<DSeg>(), (int)nseg, views, s->hs.cmf, c->P); });
         launch_tally(st, c->Q.counters, ps->max_depth, tally_buf.as<unsigned long long>());
+    };
+    auto enqueue_commits = [&]() {
+        for (auto& j : jobs) {
+            size_t n = (size_t)j.film->w * j.film->h * 9;
+            launch_film_commit(st, j.film->sums.as<float>(), j.film->acc.as<float>(), n);
+        }
+    };
+    const uint32_t tally_words = (uint32_t)(2 * n_tally);
+    if (c->tuning.launch_graph && fuse && !ev && waves.size() == 1) {
+        // one wave: replay the captured pass (the views and segments it reads
+        // are copied into device memory first, outside the graph)
+        auto& w = waves[0];
+        uint32_t npaths = 0;
+        for (auto& sg : w) npaths = std::max(npaths, sg.begin + sg.npix * sg.ns);
+        HIPCHK(hipMemcpyAsync(c->seg_buf.p, w.data(), w.size() * sizeof(DSeg), hipMemcpyHostToDevice, st));
+        std::vector<unsigned char> key;
+        auto put = [&](const void* p, size_t n) {
+            const unsigned char* b = static_cast<const unsigned char*>(p);
+            key.insert(key.end(), b, b + n);
+        };
+        const void* ptrs[] = {views, c->seg_buf.p, vmats_dev, c->energy_buf.p, s->hs.cmf, tally_buf.p, c->spill_buf.p};
+        const uint32_t dims[] = {(uint32_t)c->grid_work, (uint32_t)c->grid_trav, ps->max_depth, npaths,
+                                 (uint32_t)w.size(), n_counter_words, tally_words};
+        put(&c->P, sizeof c->P);
+        put(&c->Q, sizeof c->Q);
+        put(&c->tuning, sizeof c->tuning);
+        put(ptrs, sizeof ptrs);
+        put(dims, sizeof dims);
+        for (auto& j : jobs) {
+            const void* fp[] = {j.film->sums.p, j.film->acc.p};
+            const size_t n = (size_t)j.film->w * j.film->h * 9;
+            put(fp, sizeof fp);
+            put(&n, sizeof n);
+        }
+        ag_ctx::Graph* g = nullptr;
+        for (auto& e : c->graphs)
+            if (e.key == key) g = &e;
+        if (!g) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            launch_zero_u32(st, tally_buf.as<uint32_t>(), tally_words);
+            enqueue_wave((uint32_t)w.size(), npaths);
+            enqueue_commits();
+            const hipError_t ce = hipStreamEndCapture(st, &graph);
+            if (ce != hipSuccess) return fail(AG_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(AG_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            constexpr size_t MAX_GRAPHS = 8;
+            if (c->graphs.size() >= MAX_GRAPHS) {
+                auto old = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                            [](const ag_ctx::Graph& a, const ag_ctx::Graph& b) { return a.used < b.used; });
+                (void)hipGraphExecDestroy(old->exec);
+                c->graphs.erase(old);
+            }
+            c->graphs.push_back(ag_ctx::Graph{std::move(key), exec, 0});
+            g = &c->graphs.back();
+            c->graph_captures++;
+        } else {
+            c->graph_replays++;
+        }
+        g->used = ++c->graph_clock;
+        HIPCHK(hipGraphLaunch(g->exec, st));
+        launches += ps->max_depth;
         c->stats.samples += npaths;
-    }
-    for (auto& j : jobs) {
-        size_t n = (size_t)j.film->w * j.film->h * 9;
-        launch_film_commit(st, j.film->sums.as<float>(), j.film->acc.as<float>(), n);
+    } else {
+        launch_zero_u32(st, tally_buf.as<uint32_t>(), tally_words);
+        for (auto& w : waves) {
+            uint32_t npaths = 0;
+            for (auto& sg : w) npaths = std::max(npaths, sg.begin + sg.npix * sg.ns);
+            HIPCHK(hipMemcpyAsync(c->seg_buf.p, w.data(), w.size() * sizeof(DSeg), hipMemcpyHostToDevice, st));
+            enqueue_wave((uint32_t)w.size(), npaths);
+            launches += ps->max_depth;
+            c->stats.samples += npaths;
+        }
+        enqueue_commits();
     }
     if (ev) {
         HIPCHK(hipGetLastError());

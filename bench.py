@@ -753,6 +753,66 @@ def new_mesh(args):
     print(json.dumps(res), flush=True)
 
 
+def graph_launch(args):
+    """Host launch cost: render passes of the signet (patina mask stored) with
+    every kernel launched on its own against the same pass replayed from a
+    captured HIP graph (tuning launch_graph), wall clock per synchronous pass,
+    median of --steps x 10 passes after the capture.  Shapes: the live view's
+    first look (270x480, 2 spp, depth 32), a served preview step (135x240,
+    1 spp, depth 12), a 64x64 pass at 1 spp, and the 256-spp still (one wave
+    of 2^29 paths, shipped tuning)."""
+    import argent_amd as ag
+    from argent_amd import autotune, workloads
+    wl = workloads.signet_still()
+    piece = wl["piece"]
+    ctx = ag.Context(0)
+    sc = ag.Scene(ctx, piece.scene(wl["metal"], 0.07, 0.8, wl["studio"]))
+    piece.prepare_patina(sc)
+    library = ctx.tuning
+    shapes = [("first look 270x480 2spp depth 32", 270, 480, 2, ag.PathSettings(), 10),
+              ("preview step 135x240 1spp depth 12", 135, 240, 1, workloads.path_settings(12), 10),
+              ("64x64 1spp depth 12", 64, 64, 1, workloads.path_settings(12), 10),
+              ("still 1080x1920 256spp depth 12", 1080, 1920, 256, workloads.path_settings(12), 1)]
+    if args.graph_shape is not None:
+        shapes = [shapes[args.graph_shape]]
+    out = []
+    for name, w, h, spp, ps, reps in shapes:
+        ctx.set_tuning(**library)
+        if spp == 256:
+            chosen, _ = autotune.load("still")
+            autotune.apply(ctx, chosen)
+        cam, turn = piece.shot("hero", 100.0, 0.0, w / h)
+        sc.set_turntable(turn)
+        film = ag.Film(ctx, w, h)
+        row = {"shape": name}
+        for on in (0, 1):
+            ctx.set_tuning(launch_graph=on)
+            for k in range(2):  # warm-up; the first graph pass captures
+                ag.render_pass(ctx, sc, cam, film, ps, spp, k)
+            ms = []
+            for k in range(max(args.steps, 1) * reps):
+                t0 = time.perf_counter()
+                ag.render_pass(ctx, sc, cam, film, ps, spp, 10 + k)
+                ms.append((time.perf_counter() - t0) * 1e3)
+            row["graph_ms" if on else "eager_ms"] = round(float(np.median(ms)), 3)
+        row["eager_over_graph"] = round(row["eager_ms"] / row["graph_ms"], 3)
+        row["launches_per_pass"] = 2 * ps.max_depth + 5
+        log(f"{name}: {row}")
+        out.append(row)
+        film.close()
+    caps, reps_ = ctx.graph_stats()
+    sc.close()
+    ctx.close()
+    res = {"metric": "ms per render pass, kernels launched one by one vs one captured HIP graph, 1 GPU",
+           "value": out[0]["graph_ms"], "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 2,
+           "ms_per_step": out[0]["graph_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "signet ring (patina mask stored), sterling, white rig; shapes as listed"},
+           "shapes": out, "graph_captures": caps, "graph_replays": reps_,
+           "timing": "wall clock around ag_render_pass (synchronous: it waits for the pass's ray tallies)"}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -774,6 +834,9 @@ def main():
     ap.add_argument("--live", action="store_true", help="the live view's first look / refinement loop instead")
     ap.add_argument("--bvh", action="store_true", help="scene creation with the device BVH build vs the host SAH")
     ap.add_argument("--new-mesh", action="store_true", help="a new STL's bytes to its first preview (serving)")
+    ap.add_argument("--graphs", action="store_true", help="render passes launched one by one vs a captured HIP graph")
+    ap.add_argument("--graph-shape", type=int, default=None, choices=range(4),
+                    help="--graphs: only this shape (0 first look, 1 preview step, 2 64x64, 3 still)")
     ap.add_argument("--serve-seconds", type=float, default=120.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -813,6 +876,9 @@ def main():
         return
     if args.new_mesh:
         new_mesh(args)
+        return
+    if args.graphs:
+        graph_launch(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the render core runs on a single MI355X; --gpus {args.gpus} is not "

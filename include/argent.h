@@ -684,9 +684,18 @@ typedef struct ag_tuning {
                                      many slices (1..8), each pulled through its
                                      own cursor, block b starting on slice
                                      b mod slices (8 = one per XCD) */
+    uint32_t launch_graph;        /* 1: a render pass that fits one wave
+                                     replays a captured HIP graph of its
+                                     launches (captured again when the pass's
+                                     shape -- films, depth, wave, buffers --
+                                     changes; a few shapes are kept); 0: every
+                                     kernel is launched on its own */
 } ag_tuning;
 int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
 int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);
+/* Render passes captured into a HIP graph and passes replayed from one since
+ * the context was created (launch_graph); no counterpart in the reference. */
+int ag_ctx_graph_stats(const ag_ctx* ctx, uint64_t* captures, uint64_t* replays);
 
 /* Autotuner (no counterpart in the reference, whose CPU renderer has no launch
  * shape): times the caller's workload under candidate tunings and leaves the

@@ -250,6 +250,39 @@ def test_signet_white_render_matches_oracle(ctx):
     assert e < 0.01
 
 
+def test_signet_with_patina_mask_matches_oracle(ctx):
+    """The patina layer (material.rs:113-121, strength 0.8) weighted by the
+    device's mask of the piece (Prepared::build, shot.rs:133-143): the GPU
+    render against the oracle's with the same mask, and the mask visibly
+    darkens the recesses against the unmasked render."""
+    pos, tris, nrm = synth.signet(seed=2, band_nu=256, band_nv=64, plate_nt=100, plate_np=200)
+    piece = workloads.Piece(pos, tris, nrm)
+    cam, turn = piece.shot("hero", 100.0, 0.0, 1.0)
+    ps = workloads.path_settings(12)
+    plain = piece.scene("sterling", 0.07, 0.8, "noir")
+    sc = ag.Scene(ctx, plain)
+    try:
+        mask = piece.prepare_patina(sc)
+    finally:
+        sc.close()
+    assert mask.max() > 0.3 and 0.0 < mask.mean() < 0.5
+    desc = piece.scene("sterling", 0.07, 0.8, "noir")
+    gpu, ref = _render_both(ctx, desc, cam, 128, 128, ps, 16, 7, turn)
+    e = rel_rmse(gpu[0], ref[0])
+    print(f"relative RMSE {e:.2e}")
+    assert e < 0.01
+    assert rel_rmse(gpu[1], ref[1]) < 0.01
+    plain.d.turntable_deg = turn
+    sc = ag.Scene(ctx, plain)
+    film = ag.Film(ctx, 128, 128)
+    ag.render_pass(ctx, sc, cam, film, ps, 16, 7)
+    bare = film.sums()
+    film.close()
+    sc.close()
+    assert gpu[0].sum() < bare[0].sum() and gpu[1].sum() < bare[1].sum()
+    assert rel_rmse(gpu[1], bare[1]) > 0.02  # the albedo guide sees the tarnish
+
+
 def test_gems_hdri_render_matches_oracle(ctx):
     """Brilliants (diamond: Sellmeier dispersion; sapphire: absorption bands)
     set in the signet, lit by an importance-sampled HDRI together with the
