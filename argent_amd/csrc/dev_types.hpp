@@ -92,7 +92,13 @@ struct DScene {
     float floor_f0, floor_f1;
     const float* energy;     // 32 x 32
     const float* cmf;        // 471 x 3
+    // The table blob's first win_n floats (materials, panel emission, the
+    // gradient sky's spectra; at most SHADE_TAB_FLOATS), which the shading
+    // kernel stages into LDS.
+    const float* win;
+    uint32_t win_n;
 };
+constexpr uint32_t SHADE_TAB_FLOATS = 8192;
 
 // One view of a scene: camera, turntable, slot-0 material, film buffers.
 struct DView {

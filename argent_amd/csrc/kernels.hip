@@ -56,6 +56,23 @@ AGD T uload(const T* p) {
 // Field by field through uload: a whole-struct copy through the constant
 // space is lowered to flat loads, and the table pointers inside would lose
 // their global address space.
+// The spectral tables a shading block staged into LDS: a pointer into one of
+// the scene blob's staged ranges reads the LDS copy (through the flat
+// aperture, as every table pointer already is a flat one), any other the
+// original.  W null: no window.
+struct TabWin {
+    const float* g;  // the staged range [g, g + n) of the scene's table blob
+    uint32_t n;
+    const float* lds;
+};
+AGD const float* tw(const TabWin* W, const float* p) {
+    if (W) {
+        const size_t off = (size_t)(p - W->g);  // wraps for pointers below g
+        if (off < W->n) return W->lds + off;
+    }
+    return p;
+}
+
 AGD DMat load_mat(const DMat* p) {
     DMat m;
     m.kind = uload(&p->kind);
@@ -76,7 +93,7 @@ AGD DMat load_mat(const DMat* p) {
     m.opal_strength = uload(&p->opal_strength);
     return m;
 }
-AGD DPanel load_panel(const DPanel* p) {
+AGD DPanel load_panel(const DPanel* p, const TabWin* W = nullptr) {
     DPanel q;
     for (int i = 0; i < 3; i++) {
         q.c[i] = uload(&p->c[i]);
@@ -87,7 +104,7 @@ AGD DPanel load_panel(const DPanel* p) {
     q.area = uload(&p->area);
     q.hu2 = uload(&p->hu2);
     q.hv2 = uload(&p->hv2);
-    q.emit = uload(&p->emit);
+    q.emit = tw(W, uload(&p->emit));
     q.camera_visible = uload(&p->camera_visible);
     q.feather = uload(&p->feather);
     return q;
@@ -817,7 +834,7 @@ AGD f3 env_map_sample(const DScene& S, float u1, float u2, float* pdf) {
     return d;
 }
 // light.rs:110-127 (Env::eval)
-AGD Spec env_eval(const DScene& S, f3 dir, const Spec& lam) {
+AGD Spec env_eval(const DScene& S, f3 dir, const Spec& lam, const TabWin* W = nullptr) {
     if (uload(&S.env_kind) == AG_ENV_MAP) {
         float u, v;
         env_uv(S, dir, &u, &v);
@@ -833,10 +850,10 @@ AGD Spec env_eval(const DScene& S, f3 dir, const Spec& lam) {
     float y = fminf(fmaxf(dir.y, -1.0f), 1.0f);
     if (y >= 0.0f) {
         float t = powf(y, uload(&S.env_power));
-        return lerp(t, tab_sample(uload(&S.env_hor), lam), tab_sample(uload(&S.env_top), lam));
+        return lerp(t, tab_sample(tw(W, uload(&S.env_hor)), lam), tab_sample(tw(W, uload(&S.env_top)), lam));
     }
     float t = powf(-y, 0.5f);
-    return lerp(t, tab_sample(uload(&S.env_hor), lam), tab_sample(uload(&S.env_bot), lam));
+    return lerp(t, tab_sample(tw(W, uload(&S.env_hor)), lam), tab_sample(tw(W, uload(&S.env_bot)), lam));
 }
 
 AGD float light_pick_pdf(const DScene& S, int panel) {
@@ -846,9 +863,15 @@ AGD float light_pick_pdf(const DScene& S, int panel) {
     return 0.0f;
 }
 
-AGD DMat material_of(const DScene& S, const DMat* vmats, const DView& V, int m) {
+AGD DMat material_of(const DScene& S, const DMat* vmats, const DView& V, int m, const TabWin* W = nullptr) {
     const int m0 = uload(&V.mat0);
-    return load_mat(m == 0 && m0 >= 0 ? &vmats[m0] : &S.mat[m]);
+    DMat M = load_mat(m == 0 && m0 >= 0 ? &vmats[m0] : &S.mat[m]);
+    M.tn = tw(W, M.tn);
+    M.tk = tw(W, M.tk);
+    M.tpatina = tw(W, M.tpatina);
+    M.ta = tw(W, M.ta);
+    M.tb = tw(W, M.tb);
+    return M;
 }
 
 // material.rs:96-138
@@ -1171,8 +1194,11 @@ struct ShadeArgs {
     const float* energy;  // the GGX energy table (32 x 32), staged into LDS per block
 };
 
-// integrator.rs:50-221, one bounce
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q, int depth) {
+// integrator.rs:50-221, one bounce.  The body is compiled at two
+// occupancies (k_shade<W>, W waves per SIMD): at 2 it keeps every value in
+// registers; at 3 the compiler fits 168 registers and spills a few to
+// scratch, trading those reloads for more waves to hide memory latency.
+AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int depth) {
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     uint32_t* cnt_next = Q.counters + (depth + 1) * CNT_STRIDE;
     // every extension ray sits in exactly one bucket: blocks past the ray
@@ -1192,6 +1218,17 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
     __shared__ uint32_t s_sobol[SOBOL_TAB_WORDS];
     const AG_LDS uint32_t* sob = (AG_LDS uint32_t*)s_sobol;
     sobol_tables_fill((AG_LDS uint32_t*)s_sobol);
+    // the first view's scene tables (materials, panel emission, sky spectra)
+    // in LDS: the n, k and spectra lookups of every hit read them there
+    __shared__ float s_tab[SHADE_TAB_FLOATS];
+    TabWin W;
+    {
+        const DScene* S0 = uload(&A.views[0].scene);
+        W.g = uload(&S0->win);
+        W.n = uload(&S0->win_n);
+        W.lds = s_tab;
+        for (uint32_t q = threadIdx.x; q < W.n; q += blockDim.x) s_tab[q] = W.g[q];
+    }
     if (threadIdx.x == 0) {
         pre[0] = 0;
         for (int k = 0; k < Q.n_keys; k++) pre[k + 1] = pre[k] + cnt[2 + k];
@@ -1216,8 +1253,16 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
             f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
             float t = hh.x;
             uint32_t code = __float_as_uint(hh.w);
-            Spec beta = from4(P.beta[path]), L = from4(P.L[path]), lam = from4(P.lam[path]), pdf = from4(P.pdf[path]);
-            float prev_pdf = P.prev_pdf[path];
+            // Radiance and the previous bounce's pdf are read only by the lanes
+            // that may add emission (a miss, a light, the floor: every special
+            // code has the top bit), up front with the rest of the state; the
+            // hero pdfs only at a secondary-wavelength termination: a surface
+            // hit moves ~70 bytes less than with the whole state.
+            const bool lit = (code & HIT_PANEL) != 0u;
+            Spec beta = from4(P.beta[path]), lam = from4(P.lam[path]);
+            Spec L = lit ? from4(P.L[path]) : splat(0.0f);
+            const float prev_in = lit ? P.prev_pdf[path] : 0.0f;
+            float prev_pdf = 0.0f;
             uint32_t flags = P.flags[path];
             uint64_t rng = P.rng[path];
             const uint32_t ps = P.pix_seed[path], si = P.si[path];
@@ -1232,26 +1277,32 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
             const float cm = uload(&V.clamp_indirect);
             bool alive = true;
             if (code != HIT_MISS && medium != 255) {
-                Spec sig = tab_sample(uload(&S.mat[medium].tb), lam);
+                Spec sig = tab_sample(tw(&W, uload(&S.mat[medium].tb)), lam);
                 for (int i = 0; i < 4; i++) beta.v[i] = beta.v[i] * expf(-sig.v[i] * t);
             }
             auto env_weight = [&](f3 dir) -> float {
                 if (env_map && !specular) {
+                    const float pp = prev_in;
                     float pl = env_pick * env_map_pdf(S, dir);
-                    return prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl);
+                    return pp * pp / (pp * pp + pl * pl);
                 }
                 return 1.0f;
+            };
+            // emission seen by this bounce, added to the path's radiance
+            auto emit = [&](const Spec& c) {
+                add_clamped(L, c, clamp, cm);
+                P.L[path] = to4(L);
             };
             f3 p, p_obj, ng, ns;
             int mat_id = 0;
             float cavity = 0.0f;
             if (code == HIT_MISS) {
-                add_clamped(L, beta * env_eval(S, d, lam) * env_weight(d), clamp, cm);
+                emit(beta * env_eval(S, d, lam, &W) * env_weight(d));
                 alive = false;
             } else if (code == HIT_FLOOR) {
                 p = o + d * t;
                 if (next_random(rng) >= floor_presence(S, p)) {
-                    add_clamped(L, beta * env_eval(S, d, lam) * env_weight(d), clamp, cm);
+                    emit(beta * env_eval(S, d, lam, &W) * env_weight(d));
                     alive = false;
                 } else {
                     f3 n = d.y < 0.0f ? f3{0.0f, 1.0f, 0.0f} : f3{-0.0f, -1.0f, -0.0f};
@@ -1261,17 +1312,18 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                     mat_id = uload(&S.floor_mat);
                 }
             } else if (code & HIT_PANEL) {
-                const DPanel Pn = load_panel(&S.panel[code & 0xFFFFu]);
+                const DPanel Pn = load_panel(&S.panel[code & 0xFFFFu], &W);
                 Spec le = panel_le(Pn, o + d * t, d, lam);
                 if (!is_black(le)) {
                     if (specular) {
-                        add_clamped(L, beta * le, clamp, cm);
+                        emit(beta * le);
                     } else {
+                        const float pp = prev_in;
                         float cos_l = fmaxf(fabsf(dot(-d, ld3(Pn.n))), 1e-8f);
                         float p_light =
                             (1.0f - env_pick) * light_pick_pdf(S, (int)(code & 0xFFFFu)) * t * t / (cos_l * Pn.area);
-                        float w = prev_pdf * prev_pdf / (prev_pdf * prev_pdf + p_light * p_light);
-                        add_clamped(L, beta * le * w, clamp, cm);
+                        float w = pp * pp / (pp * pp + p_light * p_light);
+                        emit(beta * le * w);
                     }
                 }
                 alive = false;
@@ -1288,7 +1340,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                 float hu = hh.y, hv = hh.z;
                 float w = 1.0f - hu - hv;
                 mat_id = uload(&S.tmat)[prim];
-                const DMat Mh = material_of(S, A.vmats, V, mat_id);
+                const DMat Mh = material_of(S, A.vmats, V, mat_id, &W);
                 f3 nso = ngo;
                 if (Mh.kind != AG_MAT_GEM && nrm) {
                     f3 na = ld3(nrm + 3 * i0), nb = ld3(nrm + 3 * i1), nc = ld3(nrm + 3 * i2);
@@ -1308,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                 }
             }
             if (alive) {
-                const DMat M = material_of(S, A.vmats, V, mat_id);
+                const DMat M = material_of(S, A.vmats, V, mat_id, &W);
                 f3 wo_w = -d;
                 bool transmissive = M.kind == AG_MAT_GEM;
                 f3 nsu = ns, ngu = ng;
@@ -1341,7 +1393,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                                 float p_bsdf = bsdf_pdf(bs, wo, wi);
                                 float w = p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf);
                                 f3 org = offset_origin(p, ngu, dir);
-                                Spec le = env_eval(S, dir, lam);
+                                Spec le = env_eval(S, dir, lam, &W);
                                 Spec base = beta * f * le;
                                 // appended here (lanes in this branch only) so the
                                 // record does not stay live through the BSDF sample
@@ -1361,7 +1413,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                         float pick = uload(&S.light_cdf[k]) - (k == 0 ? 0.0f : uload(&S.light_cdf[k - 1]));
                         int li = uload(&S.lights[k]);
                         pick = pick * (1.0f - env_pick);
-                        const DPanel Pn = load_panel(&S.panel[li]);
+                        const DPanel Pn = load_panel(&S.panel[li], &W);
                         f3 lp = ld3(Pn.c) + ld3(Pn.hu) * (2.0f * u1 - 1.0f) + ld3(Pn.hv) * (2.0f * u2 - 1.0f);
                         f3 to = lp - p;
                         float d2 = length_sq(to);
@@ -1398,7 +1450,11 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                     if ((!sm.transmit && side <= 0.0f) || (sm.transmit && side >= 0.0f)) ok = false;
                 }
                 if (ok) {
-                    if (sm.term2) terminate_secondary(pdf);
+                    if (sm.term2) {  // kept whether or not the path goes on: the film weighs L by it
+                        Spec pdf = from4(P.pdf[path]);
+                        terminate_secondary(pdf);
+                        P.pdf[path] = to4(pdf);
+                    }
                     float sc = fabsf(sm.wi.z) / sm.pdf;
                     for (int i = 0; i < 4; i++) beta.v[i] = beta.v[i] * (sm.f.v[i] * sc);
                     if (!is_finite(beta)) ok = false;
@@ -1421,8 +1477,6 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
                 if (ok && (uint32_t)depth + 1 < uload(&V.max_depth)) emit_ray = true;
             }
             P.beta[path] = to4(beta);
-            P.L[path] = to4(L);
-            P.pdf[path] = to4(pdf);
             P.prev_pdf[path] = prev_pdf;
             P.flags[path] = (specular ? 1u : 0u) | (rough ? 2u : 0u) | ((uint32_t)medium << 8);
             P.rng[path] = rng;
@@ -1444,6 +1498,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A, DPaths P, DQueues Q,
             (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
         }
     }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void k_shade(ShadeArgs A, DPaths P,
+                                                                                        DQueues Q, int depth) {
+    shade_bounce(A, P, Q, depth);
 }
 
 // scene.rs:120-140 (sky_visibility) and scene.rs:220-243 (visible): the
@@ -1736,9 +1796,10 @@ void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueue
     hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spread, spill);
 }
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth,
-                  const float* energy, DPaths P, DQueues Q, int depth) {
+                  const float* energy, DPaths P, DQueues Q, int depth, int waves) {
     ShadeArgs A{views, vmats, max_depth, energy};
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
+    if (waves >= 3) hipLaunchKernelGGL(k_shade<3>, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
+    else hipLaunchKernelGGL(k_shade<2>, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
 }
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint32_t spread, uint2* spill) {

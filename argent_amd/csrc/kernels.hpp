@@ -44,8 +44,9 @@ void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t
 // spread: the least lanes per wavefront when a short queue is spread over the grid (64: never)
 void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint32_t spread, uint2* spill);
+// waves: the k_shade instance compiled for 2 or 3 waves per SIMD
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth,
-                  const float* energy, DPaths P, DQueues Q, int depth);
+                  const float* energy, DPaths P, DQueues Q, int depth, int waves);
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint32_t spread, uint2* spill);
 // k_shadow then k_extend of depth + 1 as one launch (same results)

@@ -259,6 +259,9 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     c->tuning.ordered_emission = 0;
     c->tuning.cursor_slices = CURSOR_SLICES;
     c->tuning.launch_graph = 0;
+    // shading at 3 waves per SIMD (a few values spilled): with the spectral
+    // tables in LDS, the signet still's shading 229 -> 222 ms per 256-spp pass
+    c->tuning.shade_waves = 3;
     *out = c;
     return AG_OK;
 }
@@ -310,6 +313,7 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
     if (t->cursor_slices < 1 || t->cursor_slices > (uint32_t)CURSOR_SLICES)
         return fail(AG_E_INVALID, "cursor_slices must be 1.." + std::to_string(CURSOR_SLICES));
     if (t->launch_graph > 1) return fail(AG_E_INVALID, "launch_graph must be 0 or 1");
+    if (t->shade_waves < 2 || t->shade_waves > 3) return fail(AG_E_INVALID, "shade_waves must be 2 or 3");
     for (uint32_t r : {t->refill_min, t->batch_refill_min, t->shadow_refill_min, t->spread_min_lanes})
         if (r < 1 || r > 64)
             return fail(AG_E_INVALID,
@@ -580,9 +584,13 @@ static int scene_over(ag_ctx* ctx, std::shared_ptr<Geometry> geo, const ag_scene
     s->ctx = ctx;
     s->geo = std::move(geo);
     hipStream_t st = ctx->stream;
-    // table blob: materials, panels, env spectra + basis, energy, CMF
+    // table blob: the tables shading reads per hit first and contiguous (the
+    // materials', the panels' emission, the gradient sky's three spectra: the
+    // prefix the shading kernel stages into LDS), then the env basis, the
+    // energy table and the CMF
     s->mat_stride = tables_per_material();
-    size_t n_tab = AG_MAX_MATERIALS * s->mat_stride + AG_MAX_PANELS * AG_SPEC_N + 6 * AG_SPEC_N + 1024 + 3 * AG_SPEC_N;
+    size_t n_tab = (size_t)d->n_materials * s->mat_stride + (size_t)d->n_panels * AG_SPEC_N + 6 * AG_SPEC_N + 1024 +
+                   3 * AG_SPEC_N;
     std::vector<float> tab(n_tab, 0.0f);
     if ((rc = s->tab.ensure(n_tab * 4))) return rc;
     const float* dbase = s->tab.as<float>();
@@ -593,7 +601,7 @@ static int scene_over(ag_ctx* ctx, std::shared_ptr<Geometry> geo, const ag_scene
         pack_material(d->materials[m], &tab[m * s->mat_stride]);
         fill_dmat(S.mat[m], d->materials[m], dbase + m * s->mat_stride);
     }
-    size_t off = AG_MAX_MATERIALS * s->mat_stride;
+    size_t off = (size_t)d->n_materials * s->mat_stride;
     S.n_panels = d->n_panels;
     for (int i = 0; i < d->n_panels; i++) {
         const ag_panel& p = d->panels[i];
@@ -619,7 +627,7 @@ static int scene_over(ag_ctx* ctx, std::shared_ptr<Geometry> geo, const ag_scene
             q.emit = nullptr;
         }
     }
-    off += AG_MAX_PANELS * AG_SPEC_N;
+    off += (size_t)d->n_panels * AG_SPEC_N;
     std::memcpy(&tab[off], d->env.top, AG_SPEC_N * 4);
     std::memcpy(&tab[off + AG_SPEC_N], d->env.horizon, AG_SPEC_N * 4);
     std::memcpy(&tab[off + 2 * AG_SPEC_N], d->env.bottom, AG_SPEC_N * 4);
@@ -634,6 +642,23 @@ static int scene_over(ag_ctx* ctx, std::shared_ptr<Geometry> geo, const ag_scene
     off += 1024;
     std::memcpy(&tab[off], ag_data::CMF_1931_2DEG, 3 * AG_SPEC_N * 4);
     S.cmf = dbase + off;
+    // the prefix the shading kernel stages into LDS: whole tables only (a
+    // material's, a panel's, a sky spectrum), in blob order, while they fit,
+    // so that every table is either staged entire or read from the blob
+    {
+        size_t n = 0;
+        bool full = false;
+        auto take = [&](size_t len) {
+            if (!full && n + len <= SHADE_TAB_FLOATS) n += len;
+            else full = true;
+        };
+        for (int m = 0; m < d->n_materials; m++) take(s->mat_stride);
+        for (int i = 0; i < d->n_panels; i++) take(AG_SPEC_N);
+        if (d->env.kind != AG_ENV_MAP)
+            for (int i = 0; i < 3; i++) take(AG_SPEC_N);
+        S.win = dbase;
+        S.win_n = (uint32_t)n;
+    }
     HIPCHK(hipMemcpyAsync(s->tab.p, tab.data(), n_tab * 4, hipMemcpyHostToDevice, st));
     S.env_kind = d->env.kind;
     S.env_power = d->env.power;
@@ -1117,7 +1142,7 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
                 timed(0, (int)d, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, spread, c->spill_buf.as<uint2>()); });
             timed(1, (int)d, [&] {
                 launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, c->energy_buf.as<float>(), c->P,
-                             c->Q, (int)d);
+                             c->Q, (int)d, (int)c->tuning.shade_waves);
             });
             if (fuse && d + 1 < ps->max_depth)
                 launch_shadow_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.shadow_refill_min,

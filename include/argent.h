@@ -690,6 +690,9 @@ typedef struct ag_tuning {
                                      shape -- films, depth, wave, buffers --
                                      changes; a few shapes are kept); 0: every
                                      kernel is launched on its own */
+    uint32_t shade_waves;         /* shading kernel compiled for this many waves
+                                     per SIMD (2: every value in registers; 3:
+                                     168 registers, a few values spilled) */
 } ag_tuning;
 int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
 int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);
