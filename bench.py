@@ -190,19 +190,23 @@ def reduce_traffic(fetch_dir, write_dir, mib, spp):
 
 
 def turntable(args):
-    """144 views (36 turntable steps of 10 degrees x 4 alloys) of the signet,
-    1080 x 1920, spp log-uniform in [64, 512] (seeded) times --spp-scale:
-    rendered as one batched schedule (ag_render_views: every view's samples
-    share waves) and as one render_pass per view; equal samples, and every
-    view's film must come out bit-identical both ways."""
+    """144 views (36 turntable steps of 10 degrees x 4 alloys) of the signet
+    (patina mask stored), --tt-size (default 1080 x 1920), spp log-uniform in
+    [64, 512] (seeded) times --spp-scale: rendered as one batched schedule
+    (ag_render_views: every view's samples share waves) and as one
+    render_pass per view; equal samples, and every view's film must come out
+    bit-identical both ways, so the two schedules reach equal error and their
+    times compare directly."""
     import argent_amd as ag
     from argent_amd import workloads
     wl = workloads.signet_still()
     piece = wl["piece"]
-    w, h, depth = wl["width"], wl["height"], wl["depth"]
+    w, h = args.tt_size if args.tt_size else (wl["width"], wl["height"])
+    depth = wl["depth"]
     ps = workloads.path_settings(depth)
     ctx = ag.Context(0)
     scene = ag.Scene(ctx, piece.scene("sterling", 0.07, 0.8, "white"))
+    piece.prepare_patina(scene)
     alloys = ["sterling", "gold14k", "rose14k", "bronze"]
     mats = [ag.Material.metal(a, 0.07, 0.8) for a in alloys]
     rng = np.random.default_rng(44)
@@ -236,9 +240,11 @@ def turntable(args):
            "value": round(total / t_batch / 1e6, 2), "unit": "Msamples/s", "n_gpus": 1,
            "per_view_msamples_per_s": round(total / t_each / 1e6, 2), "batch_s": round(t_batch, 2),
            "per_view_s": round(t_each, 2), "bit_identical": bool(same), "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"signet turntable 36 x 10 deg x 4 alloys at 1080x1920, spp log-uniform "
+           "speedup": round(t_each / t_batch, 3),
+           "config": {"workload": f"signet turntable 36 x 10 deg x 4 alloys at {w}x{h}, spp log-uniform "
                                   f"64..512 x {args.spp_scale:g}, depth {depth}, white studio",
-                      "views": 144, "samples": int(total)}}
+                      "views": 144, "samples": int(total), "width": w, "height": h,
+                      "paths_per_wave": ctx.tuning["paths_per_wave"]}}
     for f in films:
         f.close()
     scene.close()
@@ -845,6 +851,8 @@ def main():
                     help="render time of the serial plain leg on the same trace (0 skips it)")
     ap.add_argument("--turntable", action="store_true", help="the 144-view turntable batch instead of the still")
     ap.add_argument("--spp-scale", type=float, default=1.0, help="turntable: scale every view's spp")
+    ap.add_argument("--tt-size", type=int, nargs=2, metavar=("W", "H"), default=None,
+                    help="turntable: view size (default 1080 1920)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="run a streaming copy of this size first (counter calibration)")
     ap.add_argument("--reduce-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
