@@ -1239,16 +1239,38 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
     const float4* rd = Q.ray_d[depth & 1];
     float4* no = Q.ray_o[(depth + 1) & 1];
     float4* nd = Q.ray_d[(depth + 1) & 1];
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    // Software pipeline over the grid-stride loop: each iteration issues the
+    // next iteration's ray and hit loads and the bucket entry of the one after,
+    // so two of a hit's dependent memory round trips (bucket -> ray -> path
+    // state) overlap the previous hit's shading.
+    const uint32_t stride = gridDim.x * blockDim.x;
+    auto slot_of = [&](uint32_t jj) -> uint32_t {
+        int key = 0;
+        while (key + 1 < Q.n_keys && pre[key + 1] <= jj) key++;
+        return Q.bucket[(size_t)key * Q.cap + (jj - pre[key])];
+    };
+    const uint32_t j0 = blockIdx.x * blockDim.x + threadIdx.x;
+    float4 pa = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pb = pa, ph = pa;
+    if (j0 < total) {
+        const uint32_t r0 = slot_of(j0);
+        pa = ro[r0];
+        pb = rd[r0];
+        ph = Q.hit[r0];
+    }
+    uint32_t r_next = j0 + stride < total ? slot_of(j0 + stride) : 0u;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         uint32_t j = base + threadIdx.x;
         bool emit_ray = false;
         f3 nro, nrd;
         uint32_t path = 0;
+        const float4 a = pa, b = pb, hh = ph;
+        if (j + stride < total) {
+            pa = ro[r_next];
+            pb = rd[r_next];
+            ph = Q.hit[r_next];
+        }
+        if (j + 2 * stride < total) r_next = slot_of(j + 2 * stride);
         if (j < total) {
-            int key = 0;
-            while (key + 1 < Q.n_keys && pre[key + 1] <= j) key++;
-            uint32_t r = Q.bucket[(size_t)key * Q.cap + (j - pre[key])];
-            float4 a = ro[r], b = rd[r], hh = Q.hit[r];
             path = __float_as_uint(a.w);
             f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
             float t = hh.x;

@@ -259,9 +259,10 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     c->tuning.ordered_emission = 0;
     c->tuning.cursor_slices = CURSOR_SLICES;
     c->tuning.launch_graph = 0;
-    // shading at 3 waves per SIMD (a few values spilled): with the spectral
-    // tables in LDS, the signet still's shading 229 -> 222 ms per 256-spp pass
-    c->tuning.shade_waves = 3;
+    // shading at 2 waves per SIMD, every value in registers: with the loop's
+    // loads pipelined it matches the 3-wave instance (which spills ~70
+    // registers) on the signet still, 219 vs 223 ms of shading per 256-spp pass
+    c->tuning.shade_waves = 2;
     *out = c;
     return AG_OK;
 }
