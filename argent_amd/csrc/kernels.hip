@@ -1659,9 +1659,14 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
 
 // render.rs:96-111: per pixel, samples in order, into the pass accumulators
 __global__ __launch_bounds__(256) void k_accumulate(const DSeg* __restrict__ segs, int nseg,
-                                                    const DView* __restrict__ views, const float* __restrict__ cmf,
+                                                    const DView* __restrict__ views, const float* __restrict__ cmf_g,
                                                     DPaths P) {
     const float s = 1.0f / (4.0f * 106.85692f);
+    // the CIE 1931 table (471 x 3) in LDS: every path reads it for 4 wavelengths
+    __shared__ float s_cmf[3 * AG_SPEC_N];
+    for (int i = threadIdx.x; i < 3 * AG_SPEC_N; i += blockDim.x) s_cmf[i] = cmf_g[i];
+    __syncthreads();
+    const float* cmf = s_cmf;
     for (int g = 0; g < nseg; g++) {
         const DSeg sg = segs[g];
         const DView& V = views[sg.view];
