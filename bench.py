@@ -846,6 +846,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=120.0,
                     help="length of the served trace (Poisson arrivals at --serve-rate); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-step-log2", type=int, default=26, help="serving: log2 of the samples per server step")
     ap.add_argument("--serve-wall", type=float, default=135.0, help="stop serving after this many seconds")
     ap.add_argument("--plain-seconds", type=float, default=25.0,
                     help="render time of the serial plain leg on the same trace (0 skips it)")
@@ -1038,8 +1039,9 @@ def main():
         log(f"serving: {len(trace)} requests over {args.serve_seconds:.0f} s, building meshes ...")
         pieces = serve.build_pieces(8)
         log("serving meshes (triangles):", [len(p.triangles) for p in pieces])
-        res = serve.run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << 25, depth=12,
-                              max_wall=args.serve_wall, log=log)
+        res = serve.run_trace(ctx, trace, pieces, budget_bytes=96 << 30, step_samples=1 << args.serve_step_log2,
+                              depth=12, max_wall=args.serve_wall, log=log)
+        res["step_samples"] = 1 << args.serve_step_log2
         res["trace"] = (f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g} s; resolution uniform over "
                         f"512x512, 1080x1920, 2000x2000; spp log-uniform 16..1024; 8 meshes "
                         f"{min(len(p.triangles) for p in pieces)}..{max(len(p.triangles) for p in pieces)} "
