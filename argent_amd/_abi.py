@@ -153,7 +153,7 @@ class Tuning(C.Structure):
                 ("refill_min", C.c_uint32), ("batch_blocks_per_cu", C.c_uint32), ("batch_refill_min", C.c_uint32),
                 ("shadow_refill_min", C.c_uint32), ("spread_min_lanes", C.c_uint32),
                 ("ordered_emission", C.c_uint32), ("cursor_slices", C.c_uint32),
-                ("launch_graph", C.c_uint32), ("shade_waves", C.c_uint32)]
+                ("launch_graph", C.c_uint32), ("shade_waves", C.c_uint32), ("raygen_bands", C.c_uint32)]
 
 
 class PatinaSettings(C.Structure):  # ag_patina_settings (cavity.rs:24-42)

@@ -16,6 +16,7 @@
 //   ordered_emission    continuation rays in the hits' order or as appended
 //   cursor_slices       how many queue slices (each with a cursor) the grid pulls from
 //   shade_waves         the shading kernel's occupancy (2 or 3 waves per SIMD)
+//   raygen_bands        camera rays queued by image band (one per XCD) or in path order
 // (for ray batches: batch_blocks_per_cu, batch_refill_min, spread_min_lanes),
 // each candidate timed as the median of `repeats` runs; a change is kept when
 // it is at least 1 % faster, and rounds repeat until one changes nothing.
@@ -136,6 +137,7 @@ extern "C" int ag_autotune_render(ag_ctx* ctx, const ag_scene* scene, const ag_c
         {&ag_tuning::ordered_emission, {0, 1}},
         {&ag_tuning::cursor_slices, {1, 2, 4, 8}},
         {&ag_tuning::shade_waves, {2, 3}},
+        {&ag_tuning::raygen_bands, {1, 8}},
     };
     out->best = descend(start, knobs, measure, &out->ms_start, &out->ms_best, &out->trials);
     ag_film_destroy(film);

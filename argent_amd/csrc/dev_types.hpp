@@ -156,6 +156,8 @@ struct DQueues {
     uint32_t cap;
     int32_t n_keys;
     uint32_t slices;      // render traversal: queue slices with a cursor each (1..CURSOR_SLICES)
+    uint32_t bands;       // ray generation: a segment's camera rays queued in this many bands of
+                          // its pixel range (1..CURSOR_SLICES), band-major
     int32_t ordered;      // 1: shading writes hit j's continuation ray to slot j of the next queue
                           // (DEAD_PATH where the path ended), keeping the hit order exactly
     int32_t sort;         // hits: 1 to their material's bucket; 0 all to bucket 0 in arrival order;

@@ -989,8 +989,22 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
     for (uint32_t path = blockIdx.x * blockDim.x + threadIdx.x; path < npaths; path += gridDim.x * blockDim.x) {
         const DSeg& sg = segs[find_segment(segs, nseg, path)];
         uint32_t local = path - sg.begin;
-        uint32_t pix = sg.p0 + local % sg.npix;
-        uint32_t si = sg.s0 + local / sg.npix;
+        const uint32_t q = local % sg.npix, k = local / sg.npix;
+        uint32_t pix = sg.p0 + q;
+        uint32_t si = sg.s0 + k;
+        // The camera ray's queue slot: the segment's pixel range cut into
+        // Q.bands bands of rows, band-major, samples then pixels inside a band.
+        // One band per XCD slice of the queue: each XCD's traversal then walks
+        // the part of the tree one strip of the image sees, through its own L2.
+        uint32_t slot = path;
+        if (Q.bands > 1u) {
+            const uint64_t nb = Q.bands, np = sg.npix;
+            uint32_t bd = (uint32_t)min<uint64_t>(nb - 1, (uint64_t)q * nb / np);
+            while (bd + 1 < nb && (uint64_t)(bd + 1) * np / nb <= q) bd++;
+            while (bd > 0 && (uint64_t)bd * np / nb > q) bd--;
+            const uint32_t lo = (uint32_t)((uint64_t)bd * np / nb), hi = (uint32_t)((uint64_t)(bd + 1) * np / nb);
+            slot = sg.begin + sg.ns * lo + k * (hi - lo) + (q - lo);
+        }
         const DView& V = views[sg.view];
         uint32_t x = pix % V.width, y = pix / V.width;
         uint32_t ps = hash32((y * V.width + x) ^ V.seed_h);
@@ -1026,8 +1040,8 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
         P.view[path] = sg.view;
         P.galb[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         P.gnrm[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        Q.ray_o[0][path] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
-        Q.ray_d[0][path] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
+        Q.ray_o[0][slot] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
+        Q.ray_d[0][slot] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
     }
 }
 

@@ -263,6 +263,10 @@ extern "C" int ag_ctx_create(int device, ag_ctx** out) {
     // loads pipelined it matches the 3-wave instance (which spills ~70
     // registers) on the signet still, 219 vs 223 ms of shading per 256-spp pass
     c->tuning.shade_waves = 2;
+    // camera rays in path order: queueing them by image band, one band per XCD
+    // slice, made the signet still's first traversal slower (96.4 -> 101.5 ms:
+    // the piece fills the middle bands, whose slices the other XCDs then drain)
+    c->tuning.raygen_bands = 1;
     *out = c;
     return AG_OK;
 }
@@ -315,6 +319,8 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
         return fail(AG_E_INVALID, "cursor_slices must be 1.." + std::to_string(CURSOR_SLICES));
     if (t->launch_graph > 1) return fail(AG_E_INVALID, "launch_graph must be 0 or 1");
     if (t->shade_waves < 2 || t->shade_waves > 3) return fail(AG_E_INVALID, "shade_waves must be 2 or 3");
+    if (t->raygen_bands < 1 || t->raygen_bands > (uint32_t)CURSOR_SLICES)
+        return fail(AG_E_INVALID, "raygen_bands must be 1.." + std::to_string(CURSOR_SLICES));
     for (uint32_t r : {t->refill_min, t->batch_refill_min, t->shadow_refill_min, t->spread_min_lanes})
         if (r < 1 || r > 64)
             return fail(AG_E_INVALID,
@@ -428,6 +434,7 @@ static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) 
     c->Q.sort = (int32_t)c->tuning.sort_by_material;
     c->Q.ordered = c->tuning.ordered_emission ? 1 : 0;
     c->Q.slices = c->tuning.cursor_slices;
+    c->Q.bands = c->tuning.raygen_bands;
     if ((rc = c->seg_buf.ensure(MAX_SEGMENTS * sizeof(DSeg)))) return rc;
     return AG_OK;
 }

@@ -693,6 +693,9 @@ typedef struct ag_tuning {
     uint32_t shade_waves;         /* shading kernel compiled for this many waves
                                      per SIMD (2: every value in registers; 3:
                                      168 registers, a few values spilled) */
+    uint32_t raygen_bands;        /* camera rays queued in this many horizontal
+                                     bands of the image, band-major (1..8; with
+                                     cursor_slices 8, one band per XCD) */
 } ag_tuning;
 int ag_ctx_get_tuning(const ag_ctx* ctx, ag_tuning* out);
 int ag_ctx_set_tuning(ag_ctx* ctx, const ag_tuning* t);
