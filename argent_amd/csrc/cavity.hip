@@ -356,6 +356,46 @@ __global__ __launch_bounds__(BLK) void k_scatter_clamp(const float* x, const uin
 }
 
 
+// ---------------------------------------------------------------- smooth normals
+// util.rs:39-50: the unit face normal in f64, rounded to f32 (zero if degenerate)
+__global__ __launch_bounds__(BLK) void k_face_nrm(const float* pos, const uint32_t* tris, uint32_t nt, float* fn) {
+    const uint32_t f = blockIdx.x * BLK + threadIdx.x;
+    if (f >= nt) return;
+    const uint32_t* t = tris + 3 * (size_t)f;
+    const D3 a = ld3(pos + 3 * (size_t)t[0]), b = ld3(pos + 3 * (size_t)t[1]), c = ld3(pos + 3 * (size_t)t[2]);
+    const D3 x = cross(sub(b, a), sub(c, a));
+    const double l = sqrt(dot(x, x));
+    const bool ok = l > 0.0 && isfinite(l);
+    fn[3 * (size_t)f] = ok ? (float)(x.x / l) : 0.0f;
+    fn[3 * (size_t)f + 1] = ok ? (float)(x.y / l) : 0.0f;
+    fn[3 * (size_t)f + 2] = ok ? (float)(x.z / l) : 0.0f;
+}
+// normals.rs:226-247: a vertex's corners in corner order, each adding its
+// face normal times the corner angle (f64); normalized, or +z
+__global__ __launch_bounds__(BLK) void k_smooth_nrm(const float* pos, const uint32_t* tris, const uint32_t* off,
+                                                    const uint32_t* corners, const float* fn, uint32_t nv,
+                                                    float* out) {
+    const uint32_t v = blockIdx.x * BLK + threadIdx.x;
+    if (v >= nv) return;
+    D3 acc{0.0, 0.0, 0.0};
+    const D3 p = ld3(pos + 3 * (size_t)v);
+    for (uint32_t i = off[v]; i < off[v + 1]; i++) {
+        const uint32_t c = corners[i], f = c / 3u, k = c - 3u * f;
+        const uint32_t* t = tris + 3 * (size_t)f;
+        const D3 ea = sub(ld3(pos + 3 * (size_t)t[(k + 1) % 3]), p);
+        const D3 eb = sub(ld3(pos + 3 * (size_t)t[(k + 2) % 3]), p);
+        const D3 x = cross(ea, eb);
+        const double ang = atan2(sqrt(dot(x, x)), dot(ea, eb));
+        acc = add(acc, D3{ang * (double)fn[3 * (size_t)f], ang * (double)fn[3 * (size_t)f + 1],
+                          ang * (double)fn[3 * (size_t)f + 2]});
+    }
+    const double l = sqrt(dot(acc, acc));
+    const bool ok = l > 1e-300;
+    out[3 * (size_t)v] = ok ? (float)(acc.x / l) : 0.0f;
+    out[3 * (size_t)v + 1] = ok ? (float)(acc.y / l) : 0.0f;
+    out[3 * (size_t)v + 2] = ok ? (float)(acc.z / l) : 1.0f;
+}
+
 // Stable radix sort of (key, value) pairs; keys_out may be scratch.
 template <class K>
 int sort_pairs(void* w, size_t bytes, const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint32_t n,
@@ -365,6 +405,45 @@ int sort_pairs(void* w, size_t bytes, const K* kin, K* kout, const uint32_t* vin
 }
 
 }  // namespace
+
+int smooth_normals_device(ag_ctx* ctx, const float* pos, const uint32_t* tris, uint32_t nv, uint32_t nt,
+                          float* out) {
+    hipStream_t st = ctx_stream(ctx);
+    if (nv == 0) return AG_OK;
+    if ((uint64_t)3 * nt >= 0xFFFFFFFFull) return fail(AG_E_INVALID, "too many triangles for smooth normals");
+    const uint32_t n3 = 3 * nt;
+    unsigned vbits = 1;
+    while (vbits < 32 && (1ull << vbits) <= nv) vbits++;
+    const size_t NV = (size_t)nv + 1, N3 = (size_t)n3 + 1;
+    size_t sort_bytes = 0;
+    MCHK(rocprim::radix_sort_pairs(nullptr, sort_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                   (const uint32_t*)nullptr, (uint32_t*)nullptr, n3, 0, vbits, st));
+    Arena a;
+    const size_t o_sort = a.add(sort_bytes), o_scan = a.add(4 * device_scan_work_words(std::max(NV, N3))),
+                 o_tot = a.add(256), o_cnt = a.add(4 * NV), o_off = a.add(4 * NV), o_cidx = a.add(4 * N3),
+                 o_corner = a.add(4 * N3), o_ckey = a.add(4 * N3), o_fn = a.add(12 * (size_t)std::max(nt, 1u));
+    int rc = a.bind(ctx);
+    if (rc) return rc;
+    uint32_t *cnt = a.at<uint32_t>(o_cnt), *off = a.at<uint32_t>(o_off), *cidx = a.at<uint32_t>(o_cidx),
+             *corner = a.at<uint32_t>(o_corner);
+    float* fn = a.at<float>(o_fn);
+    MCHK(hipMemsetAsync(cnt, 0, 4 * NV, st));
+    if (nt) {
+        k_face_nrm<<<grid_of(nt), BLK, 0, st>>>(pos, tris, nt, fn);
+        k_corner_count<<<grid_of(n3), BLK, 0, st>>>(tris, n3, cnt);
+    }
+    device_scan_u32(st, cnt, off, nv + 1, a.at<uint32_t>(o_scan), a.at<uint32_t>(o_tot));
+    if (nt) {
+        k_iota<<<grid_of(n3), BLK, 0, st>>>(cidx, n3);
+        if ((rc = sort_pairs(a.at<void>(o_sort), sort_bytes, tris, a.at<uint32_t>(o_ckey), cidx, corner, n3, vbits,
+                             st)))
+            return rc;
+    }
+    k_smooth_nrm<<<grid_of(nv), BLK, 0, st>>>(pos, tris, off, corner, fn, nv, out);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(st));
+    return AG_OK;
+}
 
 int patina_device(ag_ctx* ctx, const PatinaInputs& in, const ag_patina_settings& s, float* mask, float* ao_out,
                   float* cav_out) {

@@ -19,6 +19,11 @@ struct PatinaInputs {
     int clamp01;                  // clamp the result to [0, 1] (the mask does)
 };
 
+// normals::smooth_vertex_normals (normals.rs:226-242) into `out` (device,
+// 3 * nv floats), the normals cavity.rs:82-87 falls back to.  Synchronizes.
+int smooth_normals_device(ag_ctx* ctx, const float* pos, const uint32_t* tris, uint32_t nv, uint32_t nt,
+                          float* out);
+
 // Writes nv floats to `mask` (device); ao_out / cav_out (device, or null)
 // get the AO fraction and the concavity per vertex.  Synchronizes.
 int patina_device(ag_ctx* ctx, const PatinaInputs& in, const ag_patina_settings& s, float* mask, float* ao_out,

@@ -1862,7 +1862,7 @@ static int ray_batch(ag_ctx* ctx, const ag_scene* s, const ag_rays_soa* rays, ui
 
 // The AO rays of cavity.rs:95-128 on the context stream: blocked-ray counts
 // per vertex into `counts` (n_vertices words).  Returns after the launch.
-static int ao_counts(ag_ctx* ctx, const Geometry& g, uint32_t n, float distance, DevBuf& counts) {
+static int ao_counts(ag_ctx* ctx, const Geometry& g, const float* nrm, uint32_t n, float distance, DevBuf& counts) {
     const uint32_t nv = g.n_vertices;
     if ((uint64_t)nv * n > 0xFFFFFFFFull) return fail(AG_E_INVALID, "n_vertices * samples exceeds 2^32 - 1 rays");
     int rc = ensure_spill(ctx);
@@ -1876,10 +1876,26 @@ static int ao_counts(ag_ctx* ctx, const Geometry& g, uint32_t n, float distance,
     hipStream_t st = ctx->stream;
     launch_zero_u32(st, counts.as<uint32_t>(), nv);
     launch_zero_u32(st, ctx->cursor_buf.as<uint32_t>(), CURSOR_SET);
-    launch_ao(st, ctx->grid_batch, g.nodes.as<float4>(), g.tris.as<float4>(), g.pos.as<float>(), g.nrm.as<float>(), n,
+    launch_ao(st, ctx->grid_batch, g.nodes.as<float4>(), g.tris.as<float4>(), g.pos.as<float>(), nrm, n,
               distance, eps, counts.as<uint32_t>(), nv * n, ctx->cursor_buf.as<uint32_t>(), ctx->tuning.batch_refill_min,
               ctx->tuning.spread_min_lanes, ctx->spill_buf.as<uint2>());
     HIPCHK(hipGetLastError());
+    return AG_OK;
+}
+
+// cavity.rs:82-87 (vertex_normals): the piece's normals, else
+// smooth_vertex_normals computed into `tmp`
+static int vertex_normals(ag_ctx* ctx, const Geometry& g, DevBuf& tmp, const float** out) {
+    if (g.has_nrm) {
+        *out = g.nrm.as<float>();
+        return AG_OK;
+    }
+    int rc = tmp.ensure(12 * (size_t)std::max(g.n_vertices, 1u));
+    if (rc) return rc;
+    if ((rc = smooth_normals_device(ctx, g.pos.as<float>(), g.idx.as<uint32_t>(), g.n_vertices, g.n_triangles,
+                                    tmp.as<float>())))
+        return rc;
+    *out = tmp.as<float>();
     return AG_OK;
 }
 
@@ -1892,14 +1908,15 @@ extern "C" int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t sam
         std::fill(out, out + nv, 0.0f);
         return AG_OK;
     }
-    if (!g.has_nrm) return fail(AG_E_INVALID, "ambient occlusion needs the scene's vertex normals");
     const uint32_t n = std::max(samples, 1u);
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    DevBuf counts;
-    HIPCHK(hipEventRecord(ctx->ev[2], st));
-    int rc = ao_counts(ctx, g, n, distance, counts);
+    DevBuf counts, smooth;
+    const float* nrm = nullptr;
+    int rc = vertex_normals(ctx, g, smooth, &nrm);
     if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    if ((rc = ao_counts(ctx, g, nrm, n, distance, counts))) return rc;
     HIPCHK(hipEventRecord(ctx->ev[3], st));
     std::vector<uint32_t> h(nv);
     HIPCHK(hipMemcpyAsync(h.data(), counts.p, (size_t)nv * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -1967,11 +1984,14 @@ extern "C" int ag_patina_mask(ag_ctx* ctx, ag_scene* s, const ag_patina_settings
     if (g.n_triangles == 0) {  // cavity.rs:54-56
         HIPCHK(hipMemsetAsync(out.p, 0, 3 * 4 * (size_t)nv, st));
     } else {
-        if (!g.has_nrm) return fail(AG_E_INVALID, "the patina mask needs the scene's vertex normals");
+        DevBuf smooth;
+        const float* nrm = nullptr;
+        if ((rc = vertex_normals(ctx, g, smooth, &nrm))) return rc;
         PatinaInputs in;
         patina_inputs(s, &in);
+        in.nrm = nrm;
         if (with_ao) {
-            if ((rc = ao_counts(ctx, g, P.ao_samples, P.ao_distance, counts))) return rc;
+            if ((rc = ao_counts(ctx, g, nrm, P.ao_samples, P.ao_distance, counts))) return rc;
             in.ao_counts = counts.as<uint32_t>();
             in.ao_samples = P.ao_samples;
         }

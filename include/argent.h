@@ -335,8 +335,9 @@ int ag_intersect_device_noalloc(ag_ctx* ctx, const ag_scene* s, const ag_rays_so
  * scene.rs:40-44, i.e. Bvh::occluded).  For each vertex, the fraction of
  * `samples` (at least 1) cosine-weighted rays about its normal, leaving
  * p + n * max(2e-4 * |diagonal of the vertex bounds|, 1e-3), that are blocked
- * within `distance` (object space).  Needs the scene's vertex normals; a zero
- * normal gives 0.  out: n_vertices floats (host).  This is the ray-traced step
+ * within `distance` (object space).  The scene's vertex normals, or without
+ * them the angle-weighted smooth normals (normals.rs:226-242) as
+ * cavity.rs:82-87 does; a zero normal gives 0.  out: n_vertices floats (host).  This is the ray-traced step
  * of the reference's patina mask (cavity.rs:52-79); ag_patina_mask runs the
  * whole mask. */
 int ag_ambient_occlusion(ag_ctx* ctx, const ag_scene* s, uint32_t samples, float distance, float* out);
@@ -362,7 +363,8 @@ int ag_patina_for_scene(const ag_scene* s, ag_patina_settings* out);
  * concavity (cavity.rs:130-243) is radius x mean curvature from a Gaussian
  * neighbourhood over grid cells; the blend is smoothstep-shaped; the
  * Laplacian smoothing (cavity.rs:245-296) treats copies of a vertex at one
- * position (crease splits) as one.  Needs the scene's vertex normals.
+ * position (crease splits) as one.  Without the scene's vertex normals it
+ * uses the angle-weighted smooth normals, as ag_ambient_occlusion does.
  * store != 0: the mask becomes the piece's cavity (Piece::cavity, shot.rs:142)
  * that shading reads (the patina layer's weight, material.rs:113-121), for
  * this scene and for scenes created over the same piece afterwards

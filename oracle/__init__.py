@@ -73,6 +73,7 @@ def lib():
             ("or_weld", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f, _u32, _u32, _u32, _u64, _f]),
             ("or_compact", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, _u32, _u32, _u32, _u64]),
             ("or_concavity", C.c_int, [_f, _f, C.c_uint32, _u32, C.c_uint32, C.c_float, _f]),
+            ("or_smooth_vertex_normals", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f]),
             ("or_smooth", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, C.c_uint32, C.c_float]),
             ("or_patina_finish", C.c_int, [_f, C.c_uint32, _u32, C.c_uint32, _f, _f, C.c_float, C.c_uint32, _f]),
             ("or_load_stl", C.c_int, [C.c_char_p, C.c_size_t, C.c_int32, C.c_float, C.c_float, _f, _u32, _u32,
@@ -154,7 +155,7 @@ class OracleScene:
         out = np.zeros(n_vertices, np.float32)
         rc = lib().or_ambient_occlusion(self.h, samples, distance, mode, threads, _p(out))
         if rc:
-            raise ValueError("or_ambient_occlusion: the scene has no vertex normals" if rc == 2 else f"rc {rc}")
+            raise ValueError(f"or_ambient_occlusion: rc {rc}")
         return out
 
     def render_pass(self, cam, width, height, sums, first, ps, count, seed, threads=0, rows=None):
@@ -424,10 +425,12 @@ def patina_mask(pos, nrm, tris, ao_samples=48, ao_distance=1.2, curvature_radius
     """cavity.rs:52-80 with the mesh's own BVH as occluder (the scene's
     Bvh::occluded); `ao` may carry a precomputed AO term.  float32 [n_vertices]."""
     pos, tris = _mesh_in(pos, tris)
-    nrm = np.ascontiguousarray(np.asarray(nrm, np.float32).reshape(-1, 3))
     nv = len(pos)
     if nv == 0 or len(tris) == 0:
         return np.zeros(nv, np.float32)
+    # cavity.rs:82-87: the mesh's normals, else smooth_vertex_normals
+    nrm = smooth_vertex_normals(pos, tris) if nrm is None else np.ascontiguousarray(
+        np.asarray(nrm, np.float32).reshape(-1, 3))
     if ao is None:
         if ao_samples > 0 and ao_distance > 0.0:
             d = _piece_desc(pos, tris, nrm)
@@ -451,3 +454,12 @@ def _piece_desc(pos, tris, nrm):
     d = ag.SceneDesc(pos, tris, normals=nrm)
     d.add_material(ag.Material.metal("sterling"))
     return d
+
+
+def smooth_vertex_normals(pos, tris):
+    """normals.rs:226-242: angle-weighted vertex normals, float32 (n_vertices, 3)."""
+    pos, tris = _mesh_in(pos, tris)
+    out = np.zeros_like(pos)
+    if lib().or_smooth_vertex_normals(_p(pos), len(pos), _p(tris, C.c_uint32), len(tris), _p(out)):
+        raise IndexError("vertex index out of range")
+    return out

@@ -577,3 +577,29 @@ extern "C" int or_normals(const float* pos, uint32_t nv, const uint32_t* tris, u
     std::memcpy(rep9, v, sizeof v);
     return 0;
 }
+
+// normals::smooth_vertex_normals (normals.rs:226-242): per vertex, the face
+// normals of its triangles weighted by the corner angle, summed in f64 in
+// triangle then corner order, normalized (or +z).  out: 3 * nv floats.
+extern "C" int or_smooth_vertex_normals(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out) {
+    const std::vector<float> P(pos, pos + 3 * (size_t)nv);
+    for (size_t i = 0; i < 3 * (size_t)nt; i++)
+        if (tris[i] >= nv) return 1;
+    std::vector<double> acc(3 * (size_t)nv, 0.0);
+    for (uint32_t f = 0; f < nt; f++) {
+        const uint32_t* t = tris + 3 * (size_t)f;
+        float fn[3], area;
+        tri_normal_area(P, t, fn, &area);
+        for (int k = 0; k < 3; k++) {
+            const uint32_t v = t[k];
+            const D3 p = d3(&P[3 * (size_t)v]);
+            const D3 ea = sub(d3(&P[3 * (size_t)t[(k + 1) % 3]]), p);
+            const D3 eb = sub(d3(&P[3 * (size_t)t[(k + 2) % 3]]), p);
+            const double ang = std::atan2(len(cross(ea, eb)), dot(ea, eb));
+            for (int i = 0; i < 3; i++) acc[3 * (size_t)v + i] += ang * (double)fn[i];
+        }
+    }
+    for (uint32_t v = 0; v < nv; v++)
+        normalize_or_z(D3{acc[3 * (size_t)v], acc[3 * (size_t)v + 1], acc[3 * (size_t)v + 2]}, out + 3 * (size_t)v);
+    return 0;
+}

@@ -1977,7 +1977,15 @@ extern "C" int or_ambient_occlusion(const or_scene* s, uint32_t samples, float d
         std::fill(out, out + nv, 0.0f);
         return 0;
     }
-    if (S.normals.size() != S.positions.size()) return 2;
+    // cavity.rs:82-87 (vertex_normals): the mesh's own, else smooth_vertex_normals
+    std::vector<float> smooth;
+    const float* N = S.normals.data();
+    if (S.normals.size() != S.positions.size()) {
+        smooth.resize(3 * (size_t)nv);
+        or_smooth_vertex_normals(S.positions.data(), nv, S.triangles.data(), (uint32_t)(S.triangles.size() / 3),
+                                 smooth.data());
+        N = smooth.data();
+    }
     float lo[3] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
                    std::numeric_limits<float>::max()};
     float hi[3] = {std::numeric_limits<float>::lowest(), std::numeric_limits<float>::lowest(),
@@ -1995,7 +2003,7 @@ extern "C" int or_ambient_occlusion(const or_scene* s, uint32_t samples, float d
     const bool robust = mode == OR_MODE_ROBUST;
     parallel_for(nv, threads, [&](uint32_t a, uint32_t b) {
         for (uint32_t v = a; v < b; v++) {
-            V3 nrm = ld(&S.normals[3 * (size_t)v]);
+            V3 nrm = ld(&N[3 * (size_t)v]);
             // Vec3::normalize_or_zero: times the reciprocal length when finite and positive
             const float rcp = 1.0f / nrm.length();
             if (!(std::isfinite(rcp) && rcp > 0.0f)) {

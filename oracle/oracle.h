@@ -47,7 +47,8 @@ int or_intersect(const or_scene* s, const float* ox, const float* oy, const floa
 // reference-tree work of closest-hit queries: totals = {Node4 visits, triangle tests}
 int or_intersect_stats(const or_scene* s, const float* ox, const float* oy, const float* oz, const float* dx,
                        const float* dy, const float* dz, const float* tmax, uint32_t n, int32_t mode, uint64_t* totals);
-// cavity AO per vertex (argent-mesh cavity.rs:95-128); 2 when the scene has no normals
+// cavity AO per vertex (argent-mesh cavity.rs:95-128); without the scene's
+// normals, smooth_vertex_normals' (cavity.rs:82-87)
 int or_ambient_occlusion(const or_scene* s, uint32_t samples, float distance, int32_t mode, int32_t threads,
                          float* out);
 int or_occluded(const or_scene* s, const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
@@ -124,6 +125,8 @@ int or_smooth(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, 
               float lambda);
 int or_patina_finish(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, const float* ao,
                      const float* cav, float concavity_weight, uint32_t smooth_iterations, float* out);
+/* normals::smooth_vertex_normals (normals.rs:226-242), normals.cpp */
+int or_smooth_vertex_normals(const float* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out);
 int or_load_stl(const uint8_t* b, size_t len, int32_t do_weld, float rel_tolerance, float unit_scale, float* out_pos,
                 uint32_t* out_nv, uint32_t* out_tris, uint32_t* out_nt, uint64_t* rep10, float* tol_mm);
 

@@ -38,11 +38,12 @@ def test_groove_occluded_flat_and_ridge_open():
     assert np.all((ao >= 0.0) & (ao <= 1.0))
 
 
-def test_no_normals_is_an_error_and_zero_normal_gives_zero():
+def test_no_normals_uses_smooth_normals_and_zero_normal_gives_zero():
+    """cavity.rs:82-87: without the mesh's normals, smooth_vertex_normals."""
     pos, tris, nrm = floor_and_ceiling(0.5)
-    d = desc(pos, tris, None)
-    with pytest.raises(ValueError):
-        O.OracleScene(d).ambient_occlusion(len(pos), 8, 1.0)
+    a = O.OracleScene(desc(pos, tris, None)).ambient_occlusion(len(pos), 8, 1.0)
+    b = O.OracleScene(desc(pos, tris, O.smooth_vertex_normals(pos, tris))).ambient_occlusion(len(pos), 8, 1.0)
+    assert np.array_equal(a, b) and a.max() > 0.0
     nrm = nrm.copy()
     nrm[0] = 0.0
     ao = O.OracleScene(desc(pos, tris, nrm)).ambient_occlusion(len(pos), 8, 1.0)

@@ -179,15 +179,18 @@ def test_edge_cases(ctx):
     ref = O.patina_mask(p, n, t, ao_samples=0, concavity_weight=1.0, smooth_iterations=2)
     assert np.abs(m - ref).max() <= 1e-5
     assert m.max() > 0.5
-    # no normals: an error, as for the AO
+    # no normals: the smooth vertex normals (cavity.rs:82-87), on the device as in the oracle
     d = ag.SceneDesc(pos, tris)
     d.add_material(ag.Material.metal("sterling"))
     sc = ag.Scene(ctx, d)
     try:
-        with pytest.raises(ag.ArgentError):
-            sc.patina_mask()
+        m, ao, cav = sc.patina_mask(parts=True)
     finally:
         sc.close()
+    sn = O.smooth_vertex_normals(pos, tris)
+    assert np.abs(cav - O.concavity(pos, sn, tris, 0.4)).max() <= CAV_TOL
+    assert np.abs(m - O.patina_mask(pos, None, tris, ao=ao)).max() <= 1e-5
+    assert value_near(pos, m, -3.0, 0.0) > 0.5
 
 
 def test_stored_mask_is_the_cavity_shading_reads(ctx):

@@ -147,3 +147,32 @@ def test_patina_open_plane_and_enclosure():
     m = O.patina_mask(pos, nrm, tris, concavity_weight=0.0, smooth_iterations=0, ao=np.ones(len(pos), np.float32))
     assert (m == 1.0).all()
     assert O.patina_mask(pos, nrm, np.zeros((0, 3), np.uint32)).sum() == 0.0
+
+
+def test_smooth_vertex_normals_closed_forms():
+    """normals.rs:226-242, the fallback of cavity.rs:82-87: a triangulated
+    cube's corners read (+-1, +-1, +-1)/sqrt(3) (each face's corner angles sum
+    to 90 degrees whatever its diagonal), a flat grid reads +z exactly, an
+    unreferenced vertex +z, and a fine icosphere's normals are radial."""
+    p, t = synth.cuboid([2.0, 2.0, 2.0])
+    n = O.smooth_vertex_normals(p, t)
+    want = np.sign(p) / np.sqrt(3.0)
+    assert np.abs(n - want).max() < 1e-6
+    g, gt = height_field(8, lambda x: np.zeros_like(x))
+    g = np.concatenate([g, [[9.0, 9.0, 9.0]]]).astype(np.float32)
+    ng = O.smooth_vertex_normals(g, gt)
+    assert (ng == np.float32([0, 0, 1])).all()
+    s, st = synth.icosphere(3.0, 4)
+    ns = O.smooth_vertex_normals(s, st)
+    radial = s / np.linalg.norm(s, axis=1, keepdims=True)
+    assert (np.sum(ns * radial, 1) > 0.9995).all()
+
+
+def test_patina_without_normals_uses_smooth_normals():
+    """patina_mask on a mesh without normals equals the mask given its smooth
+    vertex normals (cavity.rs:82-87)."""
+    pos, tris = height_field(60, groove_ridge_profile)
+    a = O.patina_mask(pos, None, tris)
+    b = O.patina_mask(pos, O.smooth_vertex_normals(pos, tris), tris)
+    assert np.array_equal(a, b)
+    assert value_near(pos, a, -3.0, 0.0) > 0.5
