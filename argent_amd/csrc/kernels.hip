@@ -136,7 +136,7 @@ AGD uint32_t wave_append_keyed(uint32_t* counters, int key) {
         uint64_t mask = __ballot(mine);
         uint32_t prefix = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
         uint32_t base = 0;
-        if ((int)lane_id() == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(mask));
+        if ((int)lane_id() == leader) base = atomicAdd(&counters[k * CNT_PAD], (uint32_t)__popcll(mask));
         base = __shfl(base, leader);
         if (mine) slot = base + prefix;
         todo &= ~mask;
@@ -985,7 +985,7 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
     const AG_LDS uint32_t* sob = (AG_LDS uint32_t*)s_sobol;
     sobol_tables_fill((AG_LDS uint32_t*)s_sobol);
     __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[0] = npaths;
+    if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[CNT_RAYS] = npaths;
     for (uint32_t path = blockIdx.x * blockDim.x + threadIdx.x; path < npaths; path += gridDim.x * blockDim.x) {
         const DSeg& sg = segs[find_segment(segs, nseg, path)];
         uint32_t local = path - sg.begin;
@@ -1151,7 +1151,7 @@ struct ExtendJob {
             // so a shading wavefront mostly sees one material while the queue
             // keeps its spatial order run by run.
             uint32_t base = 0;
-            if (lane_id() == 0) base = atomicAdd(cnt + 2, staged);
+            if (lane_id() == 0) base = atomicAdd(cnt + CNT_KEY, staged);
             base = __shfl(base, 0);
             uint32_t off = 0;
             for (int k = 0; k < Q.n_keys && off < staged; k++) {
@@ -1177,7 +1177,7 @@ struct ExtendJob {
                 i = stage[e];
                 key = (int)skey[e];
             }
-            uint32_t slot = wave_append_keyed(cnt + 2, key);
+            uint32_t slot = wave_append_keyed(cnt + CNT_KEY, key);
             if (key >= 0) Q.bucket[(size_t)key * Q.cap + slot] = i;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1198,7 +1198,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* _
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64],
                 s_skey[threadIdx.x / 64], 0u};
-    traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread, Q.slices);
+    traverse_persistent<false>(J, cnt[CNT_RAYS], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread, Q.slices);
 }
 
 struct ShadeArgs {
@@ -1217,13 +1217,13 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
     uint32_t* cnt_next = Q.counters + (depth + 1) * CNT_STRIDE;
     // every extension ray sits in exactly one bucket: blocks past the ray
     // count have nothing to shade (uniform per block, before any barrier)
-    if (blockIdx.x * blockDim.x >= cnt[0]) return;
+    if (blockIdx.x * blockDim.x >= cnt[CNT_RAYS]) return;
     __shared__ uint32_t pre[MAX_KEYS + 1];
     // in-order emission: the next queue is as long as this depth's hit list
     if (Q.ordered && blockIdx.x == 0 && threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (int k = 0; k < Q.n_keys; k++) tot += cnt[2 + k];
-        cnt_next[0] = (uint32_t)depth + 1 < A.max_depth ? tot : 0u;
+        for (int k = 0; k < Q.n_keys; k++) tot += cnt[CNT_KEY + k * CNT_PAD];
+        cnt_next[CNT_RAYS] = (uint32_t)depth + 1 < A.max_depth ? tot : 0u;
     }
     // microfacet.rs:93-140 E(mu, sqrt(alpha)): read by every rough lobe's
     // multiple-scattering term, several times per hit
@@ -1245,7 +1245,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
     }
     if (threadIdx.x == 0) {
         pre[0] = 0;
-        for (int k = 0; k < Q.n_keys; k++) pre[k + 1] = pre[k] + cnt[2 + k];
+        for (int k = 0; k < Q.n_keys; k++) pre[k + 1] = pre[k] + cnt[CNT_KEY + k * CNT_PAD];
     }
     __syncthreads();
     const uint32_t total = pre[Q.n_keys];
@@ -1433,7 +1433,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                                 Spec base = beta * f * le;
                                 // appended here (lanes in this branch only) so the
                                 // record does not stay live through the BSDF sample
-                                uint32_t ss = wave_append(&cnt[1], true);
+                                uint32_t ss = wave_append(&cnt[CNT_SHADOW], true);
                                 Q.sh_o[ss] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
                                 Q.sh_d[ss] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0xFFFFFFFFu));
                                 Q.sh_c[ss] = to4(base);
@@ -1465,7 +1465,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                                 f3 org = offset_origin(p, ngu, dir);
                                 Spec le = panel_le(Pn, lp, dir, lam);
                                 Spec c = beta * f * le * (fabsf(wi.z) * w / p_light);
-                                uint32_t ss = wave_append(&cnt[1], true);
+                                uint32_t ss = wave_append(&cnt[CNT_SHADOW], true);
                                 Q.sh_o[ss] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
                                 Q.sh_d[ss] = make_float4(lp.x, lp.y, lp.z, __uint_as_float((uint32_t)li));
                                 Q.sh_c[ss] = to4(c);
@@ -1526,7 +1526,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
             }
             (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
         } else {
-            uint32_t slot = wave_append(&cnt_next[0], emit_ray);
+            uint32_t slot = wave_append(&cnt_next[CNT_RAYS], emit_ray);
             if (emit_ray) {
                 no[slot] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(path));
                 nd[slot] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
@@ -1637,7 +1637,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow(const DView* _
     TRAV_STACK_DECL
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ShadowJob J{views, P, Q};
-    traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st,
+    traverse_persistent<true>(J, cnt[CNT_SHADOW], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st,
                               spread, Q.slices);
 }
 
@@ -1659,7 +1659,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
     {
         uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
         ShadowJob J{views, P, Q};
-        traverse_persistent<true>(J, cnt[1], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET,
+        traverse_persistent<true>(J, cnt[CNT_SHADOW], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET,
                                   refill_shadow, st, spread, Q.slices);
     }
     {
@@ -1667,7 +1667,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
         uint32_t* cnt = Q.counters + nd * CNT_STRIDE;
         ExtendJob J{views, P, Q, nd, cnt, Q.ray_o[nd & 1], Q.ray_d[nd & 1], s_stage[threadIdx.x / 64],
                     s_skey[threadIdx.x / 64], 0u};
-        traverse_persistent<false>(J, cnt[0], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread, Q.slices);
+        traverse_persistent<false>(J, cnt[CNT_RAYS], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread, Q.slices);
     }
 }
 
@@ -1737,8 +1737,8 @@ __global__ void k_tally(const uint32_t* __restrict__ counters, uint32_t depth_sl
         for (uint32_t d = 0; d < depth_slots; d++) {
             // live rays: the queue length at depth 0 (camera rays), the
             // continuation count after that (an in-order queue has dead slots)
-            uint32_t ed = d == 0 ? counters[0] : counters[d * CNT_STRIDE + CNT_LIVE];
-            uint32_t sd = counters[d * CNT_STRIDE + 1];
+            uint32_t ed = d == 0 ? counters[CNT_RAYS] : counters[d * CNT_STRIDE + CNT_LIVE];
+            uint32_t sd = counters[d * CNT_STRIDE + CNT_SHADOW];
             e += ed;
             s += sd;
             if (d < (uint32_t)TALLY_DEPTHS) {
