@@ -11,8 +11,9 @@
 //   shadow_refill_min   the same for shadow rays
 //   spread_min_lanes    how thinly a short queue is spread over the grid
 //   sort_by_material    sort granularity: shade in material-sorted order, in
-//                       arrival order, in runs ordered by material, or with
-//                       misses and panel hits first
+//                       arrival order, in runs ordered by material, with
+//                       misses and panel hits first, or by shading family
+//                       with one kernel per family
 //   ordered_emission    continuation rays in the hits' order or as appended
 //   cursor_slices       how many queue slices (each with a cursor) the grid pulls from
 //   shade_waves         the shading kernel's occupancy (2 or 3 waves per SIMD)
@@ -133,7 +134,7 @@ extern "C" int ag_autotune_render(ag_ctx* ctx, const ag_scene* scene, const ag_c
         {&ag_tuning::refill_min, {16, 32, 48, 64}},
         {&ag_tuning::shadow_refill_min, {8, 16, 32, 48, 64}},
         {&ag_tuning::spread_min_lanes, {1, 4, 16, 64}},
-        {&ag_tuning::sort_by_material, {0, 1, 2, 3}},
+        {&ag_tuning::sort_by_material, {0, 1, 2, 3, 4}},
         {&ag_tuning::ordered_emission, {0, 1}},
         {&ag_tuning::cursor_slices, {1, 2, 4, 8}},
         {&ag_tuning::shade_waves, {2, 3}},

@@ -116,6 +116,7 @@ struct DView {
     float* acc;          // pass accumulators: colour, albedo, normal, each w*h*3
     uint32_t max_depth, rr_depth;  // this view's PathSettings (integrator.rs:10-24): views of
     float clamp_indirect;          // one schedule may differ (a live view beside served finals)
+    uint64_t fam;        // 2 bits per material index: its shading family (0 metal, 1 diffuse, 2 other)
 };
 
 // A contiguous range of path ids: `ns` samples starting at `s0` for pixels
@@ -162,7 +163,8 @@ struct DQueues {
                           // (DEAD_PATH where the path ended), keeping the hit order exactly
     int32_t sort;         // hits: 1 to their material's bucket; 0 all to bucket 0 in arrival order;
                           // 2 to bucket 0 in runs (one traversal wavefront's) ordered by material;
-                          // 3 misses and panel hits to buckets 0 and 1, surface hits to bucket 2 in arrival order
+                          // 3 misses and panel hits to buckets 0 and 1, surface hits to bucket 2 in arrival order;
+                          // 4 to the buckets of their shading family (KEY_*), one shading kernel per family
 };
 
 // Counters per depth slot, each on its own 128-byte line: device-scope
@@ -178,6 +180,11 @@ constexpr int CNT_SHADOW = 1 * CNT_PAD;
 constexpr int CNT_KEY = 2 * CNT_PAD;
 constexpr int CNT_LIVE = (2 + MAX_KEYS) * CNT_PAD;
 constexpr int CNT_STRIDE = (3 + MAX_KEYS) * CNT_PAD;  // u32 per depth slot
+// Bucket keys of shading by family (sort mode 4): misses, panel hits, metal
+// surfaces, diffuse surfaces (the floor when its material is diffuse), every
+// other surface.
+constexpr int KEY_MISS = 0, KEY_PANEL = 1, KEY_METAL = 2, KEY_DIFFUSE = 3, KEY_OTHER = 4;
+constexpr int FAM_KEYS = 5;
 // path id of a dead slot in an extension queue written in order
 constexpr uint32_t DEAD_PATH = 0xFFFFFFFFu;
 // Ray cursors of the persistent traversal kernels: a queue of n rays is cut

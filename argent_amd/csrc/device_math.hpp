@@ -400,6 +400,12 @@ AGD float ggx_e(const float* energy, float mu, float alpha) {
 // ------------------------------------------------------------------ BSDF
 // bsdf.rs:10-27
 enum { B_CONDUCTOR = 0, B_COATED = 1, B_DIFFUSE = 2, B_DIELECTRIC = 3, B_BLACK = 4, B_OPAL = 5 };
+// The BSDF kinds a caller can meet, as a bit mask over B_*: a shading kernel
+// specialised to one family of materials compiles only that family's lobes
+// (a kind outside the mask evaluates as black).
+constexpr uint32_t KIND_ALL = 0x3Fu;
+constexpr uint32_t kind_bit(int k) { return 1u << k; }
+constexpr bool has_kind(uint32_t km, int k) { return ((km >> k) & 1u) != 0u; }
 struct Bsdf {
     int kind;
     float alpha;
@@ -415,8 +421,10 @@ struct BSample {
     bool delta, transmit, term2;
 };
 
+template <uint32_t KM = KIND_ALL>
 AGD bool bsdf_is_delta(const Bsdf& b) {
-    return b.kind == B_DIELECTRIC || (b.kind == B_CONDUCTOR && ggx_smooth(b.alpha));
+    return (has_kind(KM, B_DIELECTRIC) && b.kind == B_DIELECTRIC) ||
+           (has_kind(KM, B_CONDUCTOR) && b.kind == B_CONDUCTOR && ggx_smooth(b.alpha));
 }
 
 AGD float coat_spec_prob(float cos_o, float eta, const Spec& base) {
@@ -470,9 +478,11 @@ AGD float opal_flake_pdf(const Bsdf& b, f3 wo, f3 wi) {
 constexpr float OPAL_BASE_ALPHA = 0.12f * 0.12f;  // Ggx::from_roughness(0.12)
 
 // bsdf.rs:53-93
+template <uint32_t KM = KIND_ALL>
 AGD Spec bsdf_eval(const Bsdf& b, const float* energy, f3 wo, f3 wi) {
     switch (b.kind) {
         case B_CONDUCTOR: {
+            if constexpr (!has_kind(KM, B_CONDUCTOR)) break;
             if (wo.z <= 0.0f || wi.z <= 0.0f || ggx_smooth(b.alpha)) return splat(0.0f);
             f3 h = normalize(wo + wi);
             Spec f = fresnel_conductor(dot(wo, h), b.a, b.b);
@@ -484,39 +494,51 @@ AGD Spec bsdf_eval(const Bsdf& b, const float* energy, f3 wo, f3 wi) {
             for (int i = 0; i < 4; i++) r.v[i] = (f.v[i] * base) * (1.0f + f0.v[i] * q);
             return r;
         }
-        case B_COATED: return coated_eval(b.alpha, b.eta1, b.a, wo, wi);
+        case B_COATED:
+            if constexpr (!has_kind(KM, B_COATED)) break;
+            return coated_eval(b.alpha, b.eta1, b.a, wo, wi);
         case B_DIFFUSE:
+            if constexpr (!has_kind(KM, B_DIFFUSE)) break;
             if (wo.z <= 0.0f || wi.z <= 0.0f) return splat(0.0f);
             return b.a * FRAC_1_PI_F;
         case B_OPAL: {
+            if constexpr (!has_kind(KM, B_OPAL)) break;
             if (wo.z <= 0.0f || wi.z <= 0.0f) return splat(0.0f);
             Spec base = coated_eval(OPAL_BASE_ALPHA, 1.45f, b.a, wo, wi);
             Spec flake = opal_flake_eval(b, wo, wi);
             return base * (1.0f - b.w) + flake * b.w;
         }
-        default: return splat(0.0f);
+        default: break;
     }
+    return splat(0.0f);
 }
 
 // bsdf.rs:95-129
+template <uint32_t KM = KIND_ALL>
 AGD float bsdf_pdf(const Bsdf& b, f3 wo, f3 wi) {
     switch (b.kind) {
         case B_CONDUCTOR: {
+            if constexpr (!has_kind(KM, B_CONDUCTOR)) break;
             if (wo.z <= 0.0f || wi.z <= 0.0f || ggx_smooth(b.alpha)) return 0.0f;
             f3 h = normalize(wo + wi);
             return ggx_d_visible(b.alpha, wo, h) / (4.0f * fmaxf(dot(wo, h), 1e-8f));
         }
-        case B_COATED: return coated_pdf(b.alpha, b.eta1, b.a, wo, wi);
+        case B_COATED:
+            if constexpr (!has_kind(KM, B_COATED)) break;
+            return coated_pdf(b.alpha, b.eta1, b.a, wo, wi);
         case B_DIFFUSE:
+            if constexpr (!has_kind(KM, B_DIFFUSE)) break;
             if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
             return wi.z * FRAC_1_PI_F;
         case B_OPAL: {
+            if constexpr (!has_kind(KM, B_OPAL)) break;
             if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
             float pb = coated_pdf(OPAL_BASE_ALPHA, 1.45f, b.a, wo, wi);
             return pb * (1.0f - b.w) + opal_flake_pdf(b, wo, wi) * b.w;
         }
-        default: return 0.0f;
+        default: break;
     }
+    return 0.0f;
 }
 
 AGD bool coated_sample(float alpha, float eta, const Spec& base, f3 wo, float u0, float u1, float u2, BSample* s) {
@@ -538,9 +560,11 @@ AGD bool coated_sample(float alpha, float eta, const Spec& base, f3 wo, float u0
 }
 
 // bsdf.rs:132-239
+template <uint32_t KM = KIND_ALL>
 AGD bool bsdf_sample(const Bsdf& b, const float* energy, f3 wo, float u0, float u1, float u2, BSample* s) {
     switch (b.kind) {
         case B_CONDUCTOR: {
+            if constexpr (!has_kind(KM, B_CONDUCTOR)) break;
             if (wo.z <= 0.0f) return false;
             if (ggx_smooth(b.alpha)) {
                 f3 wi{-wo.x, -wo.y, wo.z};
@@ -556,13 +580,16 @@ AGD bool bsdf_sample(const Bsdf& b, const float* energy, f3 wo, float u0, float 
             f3 wi = reflect(wo, h);
             if (wi.z <= 0.0f) return false;
             s->pdf = ggx_d_visible(b.alpha, wo, h) / (4.0f * fmaxf(dot(wo, h), 1e-8f));
-            s->f = bsdf_eval(b, energy, wo, wi);
+            s->f = bsdf_eval<KM>(b, energy, wo, wi);
             s->wi = wi;
             s->delta = s->transmit = s->term2 = false;
             return true;
         }
-        case B_COATED: return coated_sample(b.alpha, b.eta1, b.a, wo, u0, u1, u2, s);
+        case B_COATED:
+            if constexpr (!has_kind(KM, B_COATED)) break;
+            return coated_sample(b.alpha, b.eta1, b.a, wo, u0, u1, u2, s);
         case B_DIFFUSE: {
+            if constexpr (!has_kind(KM, B_DIFFUSE)) break;
             if (wo.z <= 0.0f) return false;
             f3 wi = cosine_hemisphere(u1, u2);
             float pdf = wi.z * FRAC_1_PI_F;
@@ -574,6 +601,7 @@ AGD bool bsdf_sample(const Bsdf& b, const float* energy, f3 wo, float u0, float 
             return true;
         }
         case B_DIELECTRIC: {
+            if constexpr (!has_kind(KM, B_DIELECTRIC)) break;
             bool dispersive = (max_value(b.a) - min_value(b.a)) > 1e-5f;
             float e0 = b.a.v[0];
             float cos_o = wo.z;
@@ -606,6 +634,7 @@ AGD bool bsdf_sample(const Bsdf& b, const float* energy, f3 wo, float u0, float 
             return true;
         }
         case B_OPAL: {
+            if constexpr (!has_kind(KM, B_OPAL)) break;
             if (wo.z <= 0.0f) return false;
             f3 wi;
             if (u0 < b.w) {
@@ -621,16 +650,17 @@ AGD bool bsdf_sample(const Bsdf& b, const float* energy, f3 wo, float u0, float 
                 wi = bs.wi;
             }
             if (wi.z <= 0.0f) return false;
-            float pdf = bsdf_pdf(b, wo, wi);
+            float pdf = bsdf_pdf<KM>(b, wo, wi);
             if (pdf <= 0.0f) return false;
             s->wi = wi;
-            s->f = bsdf_eval(b, energy, wo, wi);
+            s->f = bsdf_eval<KM>(b, energy, wo, wi);
             s->pdf = pdf;
             s->delta = s->transmit = s->term2 = false;
             return true;
         }
-        default: return false;
+        default: break;
     }
+    return false;
 }
 
 // ------------------------------------------------------------------ opal domains

@@ -875,15 +875,20 @@ AGD DMat material_of(const DScene& S, const DMat* vmats, const DView& V, int m, 
 }
 
 // material.rs:96-138
+// KM: the BSDF kinds the caller can meet (a material whose lobes lie
+// outside it comes out black).
+template <uint32_t KM = KIND_ALL>
 AGD Bsdf material_bsdf(const DMat& M, const Spec& lam, float mask, float u, f3 p_obj, f3 n_world, const Frame& fr,
                        const DView& V) {
     Bsdf b;
     b.alpha = M.alpha;
     b.eta1 = M.eta;
     b.w = 0.0f;
+    b.kind = B_BLACK;
     switch (M.kind) {
         case AG_MAT_METAL: {
-            if (M.has_patina) {
+            if constexpr (!has_kind(KM, B_CONDUCTOR)) break;
+            if (has_kind(KM, B_COATED) && M.has_patina) {
                 float t = fminf(fmaxf((mask - M.p_lo) / fmaxf(M.p_hi - M.p_lo, 1e-4f), 0.0f), 1.0f);
                 float w = M.p_strength * t * t * (3.0f - 2.0f * t);
                 if (u < w) {
@@ -902,19 +907,23 @@ AGD Bsdf material_bsdf(const DMat& M, const Spec& lam, float mask, float u, f3 p
             return b;
         }
         case AG_MAT_GEM:
+            if constexpr (!has_kind(KM, B_DIELECTRIC)) break;
             b.kind = B_DIELECTRIC;
             b.a = tab_sample(M.ta, lam);
             return b;
         case AG_MAT_GEM_OPAQUE:
         case AG_MAT_PLASTIC:
+            if constexpr (!has_kind(KM, B_COATED)) break;
             b.kind = B_COATED;
             b.a = tab_sample(M.ta, lam);
             return b;
         case AG_MAT_DIFFUSE:
+            if constexpr (!has_kind(KM, B_DIFFUSE)) break;
             b.kind = B_DIFFUSE;
             b.a = tab_sample(M.ta, lam);
             return b;
         case AG_MAT_OPAL: {
+            if constexpr (!has_kind(KM, B_OPAL)) break;
             f3 dir;
             float spacing, brightness;
             opal_domain(p_obj / M.opal_cell, &dir, &spacing, &brightness);
@@ -931,8 +940,10 @@ AGD Bsdf material_bsdf(const DMat& M, const Spec& lam, float mask, float u, f3 p
             b.w = 0.6f;
             return b;
         }
-        default: b.kind = B_BLACK; return b;
+        default: break;
     }
+    b.kind = B_BLACK;
+    return b;
 }
 
 // material.rs:157-177
@@ -1119,13 +1130,26 @@ struct ExtendJob {
             per_uniform(L.aux, [&](uint32_t v) {
                 const DScene* Sp = uload(&views[v].scene);
                 float4 h;
+                int m = -1;  // the surface's material
+                bool floor = false;
                 if (L.prim != NO_RAY) {
                     h = make_float4(L.t, L.u, L.v, __uint_as_float(L.prim));
-                    key = 2 + (int)uload(&Sp->tmat)[L.prim];
+                    m = (int)uload(&Sp->tmat)[L.prim];
                 } else {
                     uint32_t code = __float_as_uint(L.u);
                     h = make_float4(L.v, 0.0f, 0.0f, L.u);
-                    key = code == HIT_MISS ? 0 : code == HIT_FLOOR ? 2 + uload(&Sp->floor_mat) : 1;
+                    floor = code == HIT_FLOOR;
+                    if (floor) m = uload(&Sp->floor_mat);
+                    key = code == HIT_MISS ? KEY_MISS : KEY_PANEL;
+                }
+                if (m >= 0) {
+                    key = 2 + m;
+                    if (Q.sort == 4) {
+                        // the shading family of the view's material m; floor
+                        // hits go to the diffuse kernel or the general one
+                        const int f = (int)((uload(&views[v].fam) >> (2 * m)) & 3u);
+                        key = f == 0 && !floor ? KEY_METAL : f == 1 ? KEY_DIFFUSE : KEY_OTHER;
+                    }
                 }
                 Q.hit[i] = h;
             });
@@ -1206,49 +1230,85 @@ struct ShadeArgs {
     const DMat* vmats;
     uint32_t max_depth;   // the launch's deepest view (each path stops at its own view's)
     const float* energy;  // the GGX energy table (32 x 32), staged into LDS per block
+    uint32_t tab_cap;     // floats of dynamic LDS for the first view's scene tables
+    int32_t k0, k1;       // the bucket keys this launch shades
 };
 
-// integrator.rs:50-221, one bounce.  The body is compiled at two
-// occupancies (k_shade<W>, W waves per SIMD): at 2 it keeps every value in
+// Shading families (sort mode 4, keys KEY_MISS .. KEY_OTHER): each family's
+// buckets are shaded by a kernel compiled for that family alone, with fewer
+// registers than the kernel that handles every material.
+enum ShadeFam { FAM_ALL = 0, FAM_SKY = 1, FAM_METAL = 2, FAM_DIFFUSE = 3 };
+constexpr uint32_t fam_kinds(int fam) {
+    return fam == FAM_ALL       ? KIND_ALL
+           : fam == FAM_METAL   ? kind_bit(B_CONDUCTOR) | kind_bit(B_COATED)
+           : fam == FAM_DIFFUSE ? kind_bit(B_DIFFUSE)
+                                : 0u;
+}
+
+// integrator.rs:50-221, one bounce, over the hits in buckets [A.k0, A.k1).
+// FAM_ALL shades every material and is compiled at two occupancies
+// (k_shade<FAM_ALL, W>, W waves per SIMD): at 2 it keeps every value in
 // registers; at 3 the compiler fits 168 registers and spills a few to
-// scratch, trading those reloads for more waves to hide memory latency.
+// scratch, trading those reloads for more waves to hide memory latency.  The
+// family instances compile only their own hit codes and lobes.
+template <int FAM>
 AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int depth) {
+    constexpr uint32_t KM = fam_kinds(FAM);
+    constexpr bool SKY = FAM == FAM_ALL || FAM == FAM_SKY;         // misses and panel hits
+    constexpr bool FLOOR = FAM == FAM_ALL || FAM == FAM_DIFFUSE;   // floor hits
+    constexpr bool SURF = FAM != FAM_SKY;                          // triangle hits
+    constexpr bool EMITS = SKY || FLOOR;  // a miss, a panel or an absent floor adds emission
+    constexpr bool ENERGY = has_kind(KM, B_CONDUCTOR);  // E(mu, alpha) of the conductor lobe
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     uint32_t* cnt_next = Q.counters + (depth + 1) * CNT_STRIDE;
-    // every extension ray sits in exactly one bucket: blocks past the ray
-    // count have nothing to shade (uniform per block, before any barrier)
-    if (blockIdx.x * blockDim.x >= cnt[CNT_RAYS]) return;
-    __shared__ uint32_t pre[MAX_KEYS + 1];
+    const int k0 = A.k0, k1 = A.k1;
     // in-order emission: the next queue is as long as this depth's hit list
-    if (Q.ordered && blockIdx.x == 0 && threadIdx.x == 0) {
+    // (written by the launch over the first bucket)
+    if (Q.ordered && k0 == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         uint32_t tot = 0;
         for (int k = 0; k < Q.n_keys; k++) tot += cnt[CNT_KEY + k * CNT_PAD];
         cnt_next[CNT_RAYS] = (uint32_t)depth + 1 < A.max_depth ? tot : 0u;
     }
+    // blocks past this launch's hit count have nothing to shade (uniform per
+    // block, before any barrier)
+    {
+        uint32_t n = 0;
+        for (int k = k0; k < k1; k++) n += cnt[CNT_KEY + k * CNT_PAD];
+        if (blockIdx.x * blockDim.x >= n) return;
+    }
+    __shared__ uint32_t pre[MAX_KEYS + 1];
+    __shared__ uint32_t s_first;  // hits in the buckets before k0 (in-order emission's slots)
     // microfacet.rs:93-140 E(mu, sqrt(alpha)): read by every rough lobe's
     // multiple-scattering term, several times per hit
-    __shared__ float s_energy[32 * 32];
-    for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x) s_energy[i] = A.energy[i];
-    __shared__ uint32_t s_sobol[SOBOL_TAB_WORDS];
+    __shared__ float s_energy[ENERGY ? 32 * 32 : 1];
+    if (ENERGY)
+        for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x) s_energy[i] = A.energy[i];
+    __shared__ uint32_t s_sobol[SURF ? SOBOL_TAB_WORDS : 1];
     const AG_LDS uint32_t* sob = (AG_LDS uint32_t*)s_sobol;
-    sobol_tables_fill((AG_LDS uint32_t*)s_sobol);
+    if (SURF) sobol_tables_fill((AG_LDS uint32_t*)s_sobol);
     // the first view's scene tables (materials, panel emission, sky spectra)
-    // in LDS: the n, k and spectra lookups of every hit read them there
-    __shared__ float s_tab[SHADE_TAB_FLOATS];
+    // in LDS, as many as the launch gave room for: the n, k and spectra
+    // lookups of every hit read them there
+    extern __shared__ float s_tab[];
     TabWin W;
     {
         const DScene* S0 = uload(&A.views[0].scene);
         W.g = uload(&S0->win);
-        W.n = uload(&S0->win_n);
+        W.n = min(uload(&S0->win_n), A.tab_cap);
         W.lds = s_tab;
         for (uint32_t q = threadIdx.x; q < W.n; q += blockDim.x) s_tab[q] = W.g[q];
     }
     if (threadIdx.x == 0) {
+        uint32_t first = 0;
+        for (int k = 0; k < k0; k++) first += cnt[CNT_KEY + k * CNT_PAD];
+        s_first = first;
         pre[0] = 0;
-        for (int k = 0; k < Q.n_keys; k++) pre[k + 1] = pre[k] + cnt[CNT_KEY + k * CNT_PAD];
+        for (int k = k0; k < k1; k++) pre[k - k0 + 1] = pre[k - k0] + cnt[CNT_KEY + k * CNT_PAD];
     }
     __syncthreads();
-    const uint32_t total = pre[Q.n_keys];
+    const int nk = k1 - k0;
+    const uint32_t total = pre[nk];
+    const uint32_t jfirst = s_first;
     const float4* ro = Q.ray_o[depth & 1];
     const float4* rd = Q.ray_d[depth & 1];
     float4* no = Q.ray_o[(depth + 1) & 1];
@@ -1260,8 +1320,8 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
     const uint32_t stride = gridDim.x * blockDim.x;
     auto slot_of = [&](uint32_t jj) -> uint32_t {
         int key = 0;
-        while (key + 1 < Q.n_keys && pre[key + 1] <= jj) key++;
-        return Q.bucket[(size_t)key * Q.cap + (jj - pre[key])];
+        while (key + 1 < nk && pre[key + 1] <= jj) key++;
+        return Q.bucket[(size_t)(k0 + key) * Q.cap + (jj - pre[key])];
     };
     const uint32_t j0 = blockIdx.x * blockDim.x + threadIdx.x;
     float4 pa = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pb = pa, ph = pa;
@@ -1294,14 +1354,14 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
             // code has the top bit), up front with the rest of the state; the
             // hero pdfs only at a secondary-wavelength termination: a surface
             // hit moves ~70 bytes less than with the whole state.
-            const bool lit = (code & HIT_PANEL) != 0u;
+            const bool lit = EMITS && (code & HIT_PANEL) != 0u;
             Spec beta = from4(P.beta[path]), lam = from4(P.lam[path]);
             Spec L = lit ? from4(P.L[path]) : splat(0.0f);
             const float prev_in = lit ? P.prev_pdf[path] : 0.0f;
             float prev_pdf = 0.0f;
             uint32_t flags = P.flags[path];
-            uint64_t rng = P.rng[path];
-            const uint32_t ps = P.pix_seed[path], si = P.si[path];
+            uint64_t rng = SURF ? P.rng[path] : 0u;
+            const uint32_t ps = SURF ? P.pix_seed[path] : 0u, si = SURF ? P.si[path] : 0u;
             const DView& V = A.views[P.view[path]];  // fields read below with uload
             const DScene& S = *uload(&V.scene);
             const float turn = uload(&V.turn_deg), vts = uload(&V.ts), vtc = uload(&V.tc);
@@ -1332,10 +1392,10 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
             f3 p, p_obj, ng, ns;
             int mat_id = 0;
             float cavity = 0.0f;
-            if (code == HIT_MISS) {
+            if (SKY && code == HIT_MISS) {
                 emit(beta * env_eval(S, d, lam, &W) * env_weight(d));
                 alive = false;
-            } else if (code == HIT_FLOOR) {
+            } else if (FLOOR && code == HIT_FLOOR) {
                 p = o + d * t;
                 if (next_random(rng) >= floor_presence(S, p)) {
                     emit(beta * env_eval(S, d, lam, &W) * env_weight(d));
@@ -1347,7 +1407,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                     ns = n;
                     mat_id = uload(&S.floor_mat);
                 }
-            } else if (code & HIT_PANEL) {
+            } else if (SKY && (code & HIT_PANEL)) {
                 const DPanel Pn = load_panel(&S.panel[code & 0xFFFFu], &W);
                 Spec le = panel_le(Pn, o + d * t, d, lam);
                 if (!is_black(le)) {
@@ -1363,7 +1423,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                     }
                 }
                 alive = false;
-            } else {
+            } else if (SURF) {
                 // scene.rs:173-201: attributes in object space, then to world
                 uint32_t prim = code;
                 const uint32_t* idx = uload(&S.idx);
@@ -1395,7 +1455,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                     ns = nso;
                 }
             }
-            if (alive) {
+            if (SURF && alive) {
                 const DMat M = material_of(S, A.vmats, V, mat_id, &W);
                 f3 wo_w = -d;
                 bool transmissive = M.kind == AG_MAT_GEM;
@@ -1408,7 +1468,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                 Frame fr = make_frame(nsu);
                 f3 wo = to_local(fr, wo_w);
                 const uint32_t dim = 8u + 8u * (uint32_t)depth;
-                Bsdf bs = material_bsdf(M, lam, cavity, rng_dim(sob, ps, si, dim), p_obj, nsu, fr, V);
+                Bsdf bs = material_bsdf<KM>(M, lam, cavity, rng_dim(sob, ps, si, dim), p_obj, nsu, fr, V);
                 if (depth == 0) {
                     float g[3];
                     guide_rgb(M, cavity, g);
@@ -1416,17 +1476,17 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                     P.gnrm[path] = make_float4(nsu.x, nsu.y, nsu.z, 0.0f);
                 }
                 // next-event estimation (integrator.rs:130-178)
-                if (!bsdf_is_delta(bs)) {
+                if (!bsdf_is_delta<KM>(bs)) {
                     float ul = rng_dim(sob, ps, si, dim + 1), u1 = rng_dim(sob, ps, si, dim + 2), u2 = rng_dim(sob, ps, si, dim + 3);
                     if (env_map && ul < env_pick) {
                         float pdf_dir;
                         f3 dir = env_map_sample(S, u1, u2, &pdf_dir);
                         if (pdf_dir > 0.0f && dot(dir, ngu) > 0.0f) {
                             f3 wi = to_local(fr, dir);
-                            Spec f = bsdf_eval(bs, s_energy, wo, wi);
+                            Spec f = bsdf_eval<KM>(bs, s_energy, wo, wi);
                             if (!is_black(f)) {
                                 float p_light = env_pick * pdf_dir;
-                                float p_bsdf = bsdf_pdf(bs, wo, wi);
+                                float p_bsdf = bsdf_pdf<KM>(bs, wo, wi);
                                 float w = p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf);
                                 f3 org = offset_origin(p, ngu, dir);
                                 Spec le = env_eval(S, dir, lam, &W);
@@ -1457,10 +1517,10 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                         float cos_l = dot(-dir, ld3(Pn.n));
                         if (cos_l > 0.0f && dot(dir, ngu) > 0.0f) {
                             f3 wi = to_local(fr, dir);
-                            Spec f = bsdf_eval(bs, s_energy, wo, wi);
+                            Spec f = bsdf_eval<KM>(bs, s_energy, wo, wi);
                             if (!is_black(f)) {
                                 float p_light = pick * d2 / (cos_l * Pn.area);
-                                float p_bsdf = bsdf_pdf(bs, wo, wi);
+                                float p_bsdf = bsdf_pdf<KM>(bs, wo, wi);
                                 float w = p_light * p_light / (p_light * p_light + p_bsdf * p_bsdf);
                                 f3 org = offset_origin(p, ngu, dir);
                                 Spec le = panel_le(Pn, lp, dir, lam);
@@ -1477,7 +1537,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                 // continue the path (integrator.rs:180-218)
                 BSample sm;
                 float u0 = rng_dim(sob, ps, si, dim + 4), u1 = rng_dim(sob, ps, si, dim + 5), u2 = rng_dim(sob, ps, si, dim + 6);
-                bool ok = bsdf_sample(bs, s_energy, wo, u0, u1, u2, &sm);
+                bool ok = bsdf_sample<KM>(bs, s_energy, wo, u0, u1, u2, &sm);
                 if (ok && (sm.pdf <= 0.0f || is_black(sm.f))) ok = false;
                 f3 wi_w;
                 if (ok) {
@@ -1512,17 +1572,21 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                 }
                 if (ok && (uint32_t)depth + 1 < uload(&V.max_depth)) emit_ray = true;
             }
-            P.beta[path] = to4(beta);
-            P.prev_pdf[path] = prev_pdf;
-            P.flags[path] = (specular ? 1u : 0u) | (rough ? 2u : 0u) | ((uint32_t)medium << 8);
-            P.rng[path] = rng;
+            // the state the next bounce reads (a path that ended is read
+            // again only for its radiance, hero pdfs and guides)
+            if (emit_ray) {
+                P.beta[path] = to4(beta);
+                P.prev_pdf[path] = prev_pdf;
+                P.flags[path] = (specular ? 1u : 0u) | (rough ? 2u : 0u) | ((uint32_t)medium << 8);
+                P.rng[path] = rng;
+            }
         }
         if (Q.ordered) {
             // hit j's continuation goes to slot j: the next queue keeps this
             // depth's hit order whatever order the blocks run in
             if (j < total && (uint32_t)depth + 1 < A.max_depth) {
-                no[j] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(emit_ray ? path : DEAD_PATH));
-                if (emit_ray) nd[j] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
+                no[jfirst + j] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(emit_ray ? path : DEAD_PATH));
+                if (emit_ray) nd[jfirst + j] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
             }
             (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
         } else {
@@ -1536,10 +1600,10 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
     }
 }
 
-template <int W>
+template <int FAM, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void k_shade(ShadeArgs A, DPaths P,
                                                                                         DQueues Q, int depth) {
-    shade_bounce(A, P, Q, depth);
+    shade_bounce<FAM>(A, P, Q, depth);
 }
 
 // scene.rs:120-140 (sky_visibility) and scene.rs:220-243 (visible): the
@@ -1837,10 +1901,32 @@ void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueue
     hipLaunchKernelGGL(k_extend, dim3(grid), dim3(TRAV_BLOCK), 0, s, views, P, Q, depth, refill, spread, spill);
 }
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth,
-                  const float* energy, DPaths P, DQueues Q, int depth, int waves) {
-    ShadeArgs A{views, vmats, max_depth, energy};
-    if (waves >= 3) hipLaunchKernelGGL(k_shade<3>, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
-    else hipLaunchKernelGGL(k_shade<2>, dim3(grid), dim3(256), 0, s, A, P, Q, depth);
+                  const float* energy, DPaths P, DQueues Q, int depth, int waves, uint32_t tab_floats) {
+    const uint32_t cap = std::min(tab_floats, SHADE_TAB_FLOATS);
+    const size_t lds = cap * sizeof(float);
+    ShadeArgs A{views, vmats, max_depth, energy, cap, 0, Q.n_keys};
+    if (Q.sort == 4) {
+        // one launch per family, in key order (misses and panel hits, metal,
+        // diffuse, the rest); the metal kernel at the shading occupancy knob,
+        // the others at the occupancy their registers allow without spills
+        auto keys = [&](int k0, int k1) {
+            A.k0 = k0;
+            A.k1 = k1;
+        };
+        keys(KEY_MISS, KEY_PANEL + 1);
+        hipLaunchKernelGGL((k_shade<FAM_SKY, 6>), dim3(grid), dim3(256), lds, s, A, P, Q, depth);
+        keys(KEY_METAL, KEY_METAL + 1);
+        if (waves >= 3) hipLaunchKernelGGL((k_shade<FAM_METAL, 3>), dim3(grid), dim3(256), lds, s, A, P, Q, depth);
+        else hipLaunchKernelGGL((k_shade<FAM_METAL, 2>), dim3(grid), dim3(256), lds, s, A, P, Q, depth);
+        keys(KEY_DIFFUSE, KEY_DIFFUSE + 1);
+        hipLaunchKernelGGL((k_shade<FAM_DIFFUSE, 3>), dim3(grid), dim3(256), lds, s, A, P, Q, depth);
+        keys(KEY_OTHER, Q.n_keys);
+        if (waves >= 3) hipLaunchKernelGGL((k_shade<FAM_ALL, 3>), dim3(grid), dim3(256), lds, s, A, P, Q, depth);
+        else hipLaunchKernelGGL((k_shade<FAM_ALL, 2>), dim3(grid), dim3(256), lds, s, A, P, Q, depth);
+        return;
+    }
+    if (waves >= 3) hipLaunchKernelGGL((k_shade<FAM_ALL, 3>), dim3(grid), dim3(256), lds, s, A, P, Q, depth);
+    else hipLaunchKernelGGL((k_shade<FAM_ALL, 2>), dim3(grid), dim3(256), lds, s, A, P, Q, depth);
 }
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint32_t spread, uint2* spill) {

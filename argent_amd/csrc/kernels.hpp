@@ -44,9 +44,11 @@ void launch_raygen(hipStream_t s, int grid, const DSeg* segs, int nseg, uint32_t
 // spread: the least lanes per wavefront when a short queue is spread over the grid (64: never)
 void launch_extend(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint32_t spread, uint2* spill);
-// waves: the k_shade instance compiled for 2 or 3 waves per SIMD
+// waves: the k_shade instance compiled for 2 or 3 waves per SIMD; tab_floats:
+// the first view's scene table floats to stage in LDS (dynamic LDS, at most
+// SHADE_TAB_FLOATS).  Q.sort 4 launches one kernel per shading family.
 void launch_shade(hipStream_t s, int grid, const DView* views, const DMat* vmats, uint32_t max_depth,
-                  const float* energy, DPaths P, DQueues Q, int depth, int waves);
+                  const float* energy, DPaths P, DQueues Q, int depth, int waves, uint32_t tab_floats);
 void launch_shadow(hipStream_t s, int grid, const DView* views, DPaths P, DQueues Q, int depth, uint32_t refill,
                    uint32_t spread, uint2* spill);
 // k_shadow then k_extend of depth + 1 as one launch (same results)

@@ -313,7 +313,7 @@ extern "C" int ag_ctx_set_tuning(ag_ctx* c, const ag_tuning* t) {
     if (t->batch_blocks_per_cu < 1 || (int)t->batch_blocks_per_cu > c->max_batch_blocks)
         return fail(AG_E_INVALID, "batch_blocks_per_cu must be 1.." + std::to_string(c->max_batch_blocks) +
                                       " (resident ray-batch traversal blocks per CU)");
-    if (t->sort_by_material > 3) return fail(AG_E_INVALID, "sort_by_material must be 0..3");
+    if (t->sort_by_material > 4) return fail(AG_E_INVALID, "sort_by_material must be 0..4");
     if (t->ordered_emission > 1) return fail(AG_E_INVALID, "ordered_emission must be 0 or 1");
     if (t->cursor_slices < 1 || t->cursor_slices > (uint32_t)CURSOR_SLICES)
         return fail(AG_E_INVALID, "cursor_slices must be 1.." + std::to_string(CURSOR_SLICES));
@@ -374,6 +374,12 @@ static int ensure_spill(ag_ctx* c) {
 static size_t counter_words(uint32_t max_depth) {
     size_t n = (size_t)(max_depth + 2) * CNT_STRIDE;
     return (n + CURSOR_STRIDE - 1) / CURSOR_STRIDE * CURSOR_STRIDE;
+}
+
+// Bucket keys a wave needs: the materials', or the shading families' when
+// shading by family
+static int wave_keys(const ag_ctx* c, int n_keys) {
+    return c->tuning.sort_by_material == 4 ? std::max(n_keys, FAM_KEYS) : n_keys;
 }
 
 static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) {
@@ -985,8 +991,10 @@ struct ViewJob {
     uint32_t first;
 };
 
-void make_dview(DView& v, const ag_scene* sc, const ag_camera* cam, float turn_deg, int mat0, const ag_film* film,
-                uint32_t seed, const ag_path_settings* ps) {
+// mat0: the view-material block index that replaces material 0 (-1 none),
+// mat0_kind that material's kind
+void make_dview(DView& v, const ag_scene* sc, const ag_camera* cam, float turn_deg, int mat0, int mat0_kind,
+                const ag_film* film, uint32_t seed, const ag_path_settings* ps) {
     std::memset(&v, 0, sizeof(v));
     v.scene = sc->d_scene.as<DScene>();
     const float* cam_m_in = sc->cam_m;
@@ -1020,6 +1028,15 @@ void make_dview(DView& v, const ag_scene* sc, const ag_camera* cam, float turn_d
     v.max_depth = ps->max_depth;
     v.rr_depth = ps->rr_depth;
     v.clamp_indirect = ps->clamp_indirect;
+    // each material's shading family (metal, diffuse, other); indices no
+    // surface carries stay "other", which the general kernel shades
+    v.fam = 0;
+    for (int m = 0; m < AG_MAX_MATERIALS; m++) {
+        int kind = m < (int)sc->mats.size() ? (int)sc->mats[m].kind : -1;
+        if (m == 0 && mat0 >= 0) kind = mat0_kind;
+        const uint64_t f = kind == AG_MAT_METAL ? 0u : kind == AG_MAT_DIFFUSE ? 1u : 2u;
+        v.fam |= f << (2 * m);
+    }
 }
 
 }  // namespace
@@ -1064,7 +1081,7 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
     uint32_t cap = c->tuning.paths_per_wave;
     int n_keys = s->n_keys;
     for (auto& j : jobs) n_keys = std::max(n_keys, j.scene->n_keys);
-    int rc = ensure_wave(c, cap, n_keys, ps->max_depth);
+    int rc = ensure_wave(c, cap, wave_keys(c, n_keys), ps->max_depth);
     if (rc) return rc;
     hipStream_t st = c->stream;
     // upload views
@@ -1150,7 +1167,7 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
                 timed(0, (int)d, [&] { launch_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.refill_min, spread, c->spill_buf.as<uint2>()); });
             timed(1, (int)d, [&] {
                 launch_shade(st, c->grid_work, views, vmats_dev, ps->max_depth, c->energy_buf.as<float>(), c->P,
-                             c->Q, (int)d, (int)c->tuning.shade_waves);
+                             c->Q, (int)d, (int)c->tuning.shade_waves, jobs[0].scene->hs.win_n);
             });
             if (fuse && d + 1 < ps->max_depth)
                 launch_shadow_extend(st, c->grid_trav, views, c->P, c->Q, (int)d, c->tuning.shadow_refill_min,
@@ -1288,7 +1305,7 @@ extern "C" int ag_render_pass(ag_ctx* ctx, const ag_scene* scene, const ag_camer
     j.film = film;
     j.count = count;
     j.first = film->samples;
-    make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed, ps);
+    make_dview(j.dv, scene, cam, scene->turntable_deg, -1, -1, film, seed, ps);
     std::vector<ViewJob> jobs{j};
     return run_views(ctx, scene, jobs, nullptr);
 }
@@ -1306,7 +1323,7 @@ extern "C" int ag_render_pass_async(ag_ctx* ctx, const ag_scene* scene, const ag
         j.film = film;
         j.count = count;
         j.first = film->samples;
-        make_dview(j.dv, scene, cam, scene->turntable_deg, -1, film, seed, ps);
+        make_dview(j.dv, scene, cam, scene->turntable_deg, -1, -1, film, seed, ps);
         std::vector<ViewJob> jobs{j};
         int rc = run_views(ctx, scene, jobs, nullptr, ev.get());
         if (rc) return rc;
@@ -1395,7 +1412,8 @@ extern "C" int ag_render_views(ag_ctx* ctx, const ag_scene* scene, const ag_mate
         j.first = v.film->samples;
         const ag_scene* vs = v.scene ? v.scene : scene;
         if (vs->ctx != ctx) return fail(AG_E_INVALID, "view scene from another context");
-        make_dview(j.dv, vs, &v.camera, v.turntable_deg, v.material0, v.film, v.seed, ps);
+        make_dview(j.dv, vs, &v.camera, v.turntable_deg, v.material0, v.material0 >= 0 ? (int)materials[v.material0].kind : -1,
+                   v.film, v.seed, ps);
         j.scene = vs;
         jobs.push_back(j);
     }
@@ -1446,6 +1464,7 @@ constexpr int SERVER_PREVIEW_RUN_MAX = 4;
 // n_keys material keys: path state, ray / hit / shadow queues, buckets,
 // counters, segments and the traversal spill stacks.
 size_t wave_state_bytes(const ag_ctx* c, int n_keys, uint32_t max_depth) {
+    n_keys = wave_keys(c, n_keys);
     const size_t cap = c->tuning.paths_per_wave;
     const size_t per_path = 6 * sizeof(float4) + sizeof(float) + 4 * sizeof(uint32_t) + sizeof(uint64_t);
     size_t b = cap * (per_path + 4 * sizeof(float4) + sizeof(float4) + 4 * sizeof(float4) + (size_t)n_keys * 4);
@@ -1523,7 +1542,8 @@ int server_render(ag_server* s, const std::vector<uint64_t>& set, const ViewJob*
             m0 = (int)mats.size();
             mats.push_back(q.r.material0);
         }
-        make_dview(j.dv, q.r.scene, &q.r.camera, q.r.turntable_deg, m0, q.film, q.r.seed, &s->ps);
+        make_dview(j.dv, q.r.scene, &q.r.camera, q.r.turntable_deg, m0, m0 >= 0 ? (int)mats[m0].kind : -1, q.film, q.r.seed,
+                   &s->ps);
         j.scene = q.r.scene;
         jobs.push_back(j);
     }
@@ -2321,7 +2341,7 @@ int live_render(ag_live* L, ag_film* f, uint32_t count, double* ms) {
         j.film = f;
         j.count = count;
         j.first = f->samples;
-        make_dview(j.dv, L->scene, &L->cam, L->scene->turntable_deg, -1, f, LIVE_SEED, &L->p.ps);
+        make_dview(j.dv, L->scene, &L->cam, L->scene->turntable_deg, -1, -1, f, LIVE_SEED, &L->p.ps);
         int pass_rc = AG_OK;
         int rc = server_step(L->server, &j, nullptr, nullptr, &pass_rc, ms);
         return rc ? rc : pass_rc;

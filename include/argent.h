@@ -664,7 +664,9 @@ typedef struct ag_tuning {
                                   runs of one traversal wavefront's hits, each
                                   run ordered by material; 3 = misses and panel
                                   hits first, then every surface hit in arrival
-                                  order */
+                                  order; 4 = by shading family (misses and
+                                  panel hits, metals, diffuse surfaces, the
+                                  rest), each family shaded by its own kernel */
     uint32_t refill_min;       /* render passes, extension rays: idle lanes (1..64)
                                   before a wavefront takes new rays; 64 = whole
                                   coherent batches */

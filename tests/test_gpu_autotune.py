@@ -39,6 +39,7 @@ def test_autotune_render_keeps_the_film(ctx):
     assert np.array_equal(_film(ctx, sc, cam, ps), want)
     # the extreme settings the search passes through render the same film
     for kw in (dict(sort_by_material=1), dict(sort_by_material=2), dict(sort_by_material=3),
+               dict(sort_by_material=4), dict(sort_by_material=4, ordered_emission=1, shade_waves=3),
                dict(ordered_emission=1), dict(ordered_emission=1, sort_by_material=3), dict(cursor_slices=1),
                dict(paths_per_wave=1 << 22, blocks_per_cu=2),
                dict(refill_min=16, shadow_refill_min=8, spread_min_lanes=64)):
