@@ -1039,18 +1039,14 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
             org = ld3(V.pos) + ld3(V.right) * (dx * V.lens_radius) + ld3(V.up) * (dy * V.lens_radius);
             dir = normalize(pf - org);
         }
-        P.beta[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        // throughput, flags, previous pdf, generator state and guides are
+        // implicit at depth 0: the first shading sets them
         P.L[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         P.lam[path] = to4(lam);
         P.pdf[path] = to4(pdf);
-        P.prev_pdf[path] = 1.0f;
-        P.flags[path] = 1u | (255u << 8);
-        P.rng[path] = rng_init(ps, si);
         P.pix_seed[path] = ps;
         P.si[path] = si;
         P.view[path] = sg.view;
-        P.galb[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.gnrm[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         Q.ray_o[0][slot] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
         Q.ray_d[0][slot] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
     }
@@ -1354,14 +1350,20 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
             // code has the top bit), up front with the rest of the state; the
             // hero pdfs only at a secondary-wavelength termination: a surface
             // hit moves ~70 bytes less than with the whole state.
+            // Camera rays (depth 0) start from the state ray generation leaves
+            // implicit: throughput 1, a specular previous bounce with pdf 1, no
+            // medium, the generator seeded from the pixel and sample.
+            const bool d0 = depth == 0;
             const bool lit = EMITS && (code & HIT_PANEL) != 0u;
-            Spec beta = from4(P.beta[path]), lam = from4(P.lam[path]);
+            Spec beta = d0 ? splat(1.0f) : from4(P.beta[path]);
+            const Spec lam = from4(P.lam[path]);
             Spec L = lit ? from4(P.L[path]) : splat(0.0f);
-            const float prev_in = lit ? P.prev_pdf[path] : 0.0f;
+            const float prev_in = lit ? (d0 ? 1.0f : P.prev_pdf[path]) : 0.0f;
             float prev_pdf = 0.0f;
-            uint32_t flags = P.flags[path];
-            uint64_t rng = SURF ? P.rng[path] : 0u;
+            uint32_t flags = d0 ? (1u | (255u << 8)) : P.flags[path];
             const uint32_t ps = SURF ? P.pix_seed[path] : 0u, si = SURF ? P.si[path] : 0u;
+            uint64_t rng = SURF ? (d0 ? rng_init(ps, si) : P.rng[path]) : 0u;
+            float4 g_alb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g_nrm = g_alb;  // depth 0: the guides
             const DView& V = A.views[P.view[path]];  // fields read below with uload
             const DScene& S = *uload(&V.scene);
             const float turn = uload(&V.turn_deg), vts = uload(&V.ts), vtc = uload(&V.tc);
@@ -1469,11 +1471,11 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                 f3 wo = to_local(fr, wo_w);
                 const uint32_t dim = 8u + 8u * (uint32_t)depth;
                 Bsdf bs = material_bsdf<KM>(M, lam, cavity, rng_dim(sob, ps, si, dim), p_obj, nsu, fr, V);
-                if (depth == 0) {
+                if (d0) {
                     float g[3];
                     guide_rgb(M, cavity, g);
-                    P.galb[path] = make_float4(g[0], g[1], g[2], 0.0f);
-                    P.gnrm[path] = make_float4(nsu.x, nsu.y, nsu.z, 0.0f);
+                    g_alb = make_float4(g[0], g[1], g[2], 0.0f);
+                    g_nrm = make_float4(nsu.x, nsu.y, nsu.z, 0.0f);
                 }
                 // next-event estimation (integrator.rs:130-178)
                 if (!bsdf_is_delta<KM>(bs)) {
@@ -1571,6 +1573,11 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                     }
                 }
                 if (ok && (uint32_t)depth + 1 < uload(&V.max_depth)) emit_ray = true;
+            }
+            // every camera ray's guides (zero where it saw no surface)
+            if (d0) {
+                P.galb[path] = g_alb;
+                P.gnrm[path] = g_nrm;
             }
             // the state the next bounce reads (a path that ended is read
             // again only for its radiance, hero pdfs and guides)

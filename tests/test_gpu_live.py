@@ -3,7 +3,8 @@ injected clock: first look at half resolution and 2 spp, 1-spp passes while
 in motion, pass sizing toward 70 ms, publish pacing, convergence at max_spp,
 the settle test, generation restarts, look-only re-development.  Frames are
 checked against the same pipeline run by hand (render at seed 1, à-trous,
-develop), and relative_change against the oracle."""
+develop) and against the oracle's render, à-trous and develop of the same
+passes; relative_change against the oracle."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,60 @@ def _hand_frame(ctx, sc, cam, w, h, passes, tonemap="agx_punchy", ev=0.0):
     out = film.develop(tonemap, exposure_ev=ev, denoise=True)
     film.close()
     return out
+
+
+def _oracle_frame(wl, cam, turn, w, h, passes, tonemap="agx_punchy"):
+    """The live frame's pipeline on the oracle: the same passes at seed 1
+    summed pass by pass (render.rs:107-111), the film's means
+    (render.rs:35-64), à-trous with the albedo and normal guides, develop."""
+    desc = wl["piece"].scene(wl["metal"], 0.07, 0.8, wl["studio"])
+    desc.d.turntable_deg = turn
+    osc = O.OracleScene(desc)
+    sums = np.zeros((3, h, w, 3), np.float32)
+    first = 0
+    for n in passes:
+        osc.render_pass(cam, w, h, sums, first, ag.PathSettings(), n, 1)
+        first += n
+    osc.close()
+    k = np.float32(1.0) / np.float32(first)
+    color, albedo, nv = sums[0] * k, sums[1] * k, sums[2] * k
+    ln = np.sqrt(nv[..., 0] * nv[..., 0] + nv[..., 1] * nv[..., 1] + nv[..., 2] * nv[..., 2])[..., None]
+    normal = np.where(ln > 1e-6, nv / np.maximum(ln, np.float32(1e-30)), np.float32(0.0)).astype(np.float32)
+    return O.develop(O.atrous(color, albedo, normal), tonemap)
+
+
+def _byte_gap(a, b):
+    d = np.abs(a.astype(np.int32) - b.astype(np.int32))
+    return float(d.mean()), float(np.mean(d > 2)), int(d.max())
+
+
+def test_live_frames_match_oracle_pipeline(ctx, piece_scene):
+    """The live view's first look and its final frame against the oracle's
+    render, à-trous and develop of the same passes.  The device and the
+    oracle agree on the films to float rounding (about 1e-3 relative RMSE at
+    a few spp); after denoising and 8-bit development that leaves the bytes
+    equal or, where a value lands next to a rounding boundary, one step
+    apart: mean gap at most 0.05 step, no value more than 1 step apart
+    (measured: 0.002 step, 1)."""
+    wl, sc, cam = piece_scene
+    _, turn = wl["piece"].shot("hero", 100.0, 0.0, W / H)
+    lv = ag.Live(ctx)
+    lv.set(sc, cam, W, H, max_spp=16, now_ms=0.0)
+    f, img = lv.step(now_ms=0.0)
+    assert f["published"] and f["half"]
+    gap = _byte_gap(img, _oracle_frame(wl, cam, turn, W // 2, H // 2, [2]))
+    print("first look vs oracle: mean step gap, share > 2 steps, max", gap)
+    assert gap[0] <= 0.05 and gap[2] <= 1, gap
+    t, passes = 1000.0, []
+    while not f["converged"]:
+        f, img = lv.step(now_ms=t)
+        passes.append(f["pass_spp"])
+        t += 2000.0
+    assert f["spp"] == 16
+    gap = _byte_gap(img, _oracle_frame(wl, cam, turn, W, H, passes))
+    print("final frame vs oracle: mean step gap, share > 2 steps, max", gap, "passes", passes)
+    assert gap[0] <= 0.05 and gap[2] <= 1, gap
+    lv.close()
 
 
 def test_relative_change_matches_oracle(ctx):

@@ -227,10 +227,17 @@ def rel_rmse(a, b):
 
 
 def test_torus_noir_render_matches_oracle(ctx):
-    pos, tris, nrm = synth.torus_ring(seed=1, nu=256, nv=64)
-    piece = workloads.Piece(pos, tris, nrm)
-    desc = piece.scene("gold14k", 0.07, 0.8, "noir")
-    cam, turn = piece.shot("hero", 100.0, 16.0, 1.0)
+    # the CPU reference still's scene in full: the 512 x 128-quad torus ring
+    # (131,072 triangles) in 14k gold under the noir rig at f/16, focused by
+    # the device's own autofocus ray
+    wl = workloads.torus_still()
+    piece = wl["piece"]
+    assert len(piece.triangles) == 131072
+    desc = piece.scene(wl["metal"], 0.07, 0.8, wl["studio"])
+    sc = ag.Scene(ctx, desc)
+    cam, turn = piece.shot("hero", 100.0, wl["f_stop"], 1.0, intersect=sc.intersect)
+    sc.close()
+    assert wl["f_stop"] == 16.0 and cam.lens_radius > 0.0
     gpu, ref = _render_both(ctx, desc, cam, 128, 128, workloads.path_settings(8), 8, 7, turn)
     e = rel_rmse(gpu[0], ref[0])
     same = np.mean(np.all(gpu[0] == ref[0], axis=-1))
