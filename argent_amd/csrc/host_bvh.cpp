@@ -5,7 +5,9 @@
 // policy of the reference's 4-wide collapse, bvh.rs:303-378), with child
 // boxes quantized to 8 bits per plane against the node's box (conservative),
 // laid out for 16-byte vector loads.  Topology differs from the reference's
-// BVH; closest-hit results do not depend on it.
+// BVH; closest-hit results do not depend on it, except for a rare grazing ray
+// whose Moller-Trumbore hit lies outside its triangle by more than the
+// traversal's box widening.
 #include <algorithm>
 #include <cmath>
 #include <cstring>

@@ -150,7 +150,8 @@ AGD uint32_t wave_append_keyed(uint32_t* counters, int key) {
 // area.  Box tests are widened by a relative 2^-16 on both ends: enough for
 // the box test's own rounding and for the f32 Moller-Trumbore hits that land
 // slightly outside their triangle on grazing rays, so the answer does not
-// depend on how the tree groups triangles.  The triangle test is
+// depend on how the tree groups triangles (except for a rare grazing ray
+// whose hit lies outside its triangle by more than the widening).  The triangle test is
 // Moller-Trumbore evaluated exactly as the reference does (no contraction),
 // with ties on t resolved to the lower original triangle index.
 // The stack's two homes carry their address spaces in the pointer types, so
