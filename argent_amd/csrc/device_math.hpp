@@ -411,6 +411,7 @@ struct Bsdf {
     float alpha;
     float eta1;      // coated eta
     Spec a, b;       // conductor eta/k, coated base, diffuse albedo, dielectric eta, opal body / lambda
+    Spec f0;         // conductor: Fresnel at normal incidence
     f3 on;           // opal lattice normal (local)
     float spacing, strength, w;
 };
@@ -488,7 +489,7 @@ AGD Spec bsdf_eval(const Bsdf& b, const float* energy, f3 wo, f3 wi) {
             Spec f = fresnel_conductor(dot(wo, h), b.a, b.b);
             float base = ggx_d(b.alpha, h) * ggx_g2(b.alpha, wo, wi) / (4.0f * wo.z * wi.z);
             float e = ggx_e(energy, wo.z, b.alpha);
-            Spec f0 = fresnel_conductor(1.0f, b.a, b.b);
+            const Spec f0 = b.f0;
             float q = (1.0f - e) / e;
             Spec r;
             for (int i = 0; i < 4; i++) r.v[i] = (f.v[i] * base) * (1.0f + f0.v[i] * q);

@@ -905,6 +905,7 @@ AGD Bsdf material_bsdf(const DMat& M, const Spec& lam, float mask, float u, f3 p
                 b.a.v[i] = grid_eval(M.tn, lam.v[i]);
                 b.b.v[i] = grid_eval(M.tk, lam.v[i]);
             }
+            b.f0 = fresnel_conductor(1.0f, b.a, b.b);
             return b;
         }
         case AG_MAT_GEM:
