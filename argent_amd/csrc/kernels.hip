@@ -223,6 +223,36 @@ AGD void slab8_t(const float4* __restrict__ nd, f3 o, f3 inv, float tmax, float 
     float sz = __uint_as_float(((w >> 16) & 0xFFu) << 23);
     uint32_t imask = w >> 24;
     K = NodeKids{n1.x, n1.y, n1.z, n1.w, imask};
+    if (fabsf(inv.x) < INF_F && fabsf(inv.y) < INF_F && fabsf(inv.z) < INF_F) {
+        // finite 1/d: each axis's near and far planes follow the direction's
+        // sign, chosen once for the node (min/max of the two products would
+        // pick the same values)
+        const bool px = inv.x >= 0.0f, py = inv.y >= 0.0f, pz = inv.z >= 0.0f;
+        const uint32_t nx0 = px ? qa.x : qb.z, nx1 = px ? qa.y : qb.w, fx0 = px ? qb.z : qa.x, fx1 = px ? qb.w : qa.y;
+        const uint32_t ny0 = py ? qa.z : qc.x, ny1 = py ? qa.w : qc.y, fy0 = py ? qc.x : qa.z, fy1 = py ? qc.y : qa.w;
+        const uint32_t nz0 = pz ? qb.x : qc.z, nz1 = pz ? qb.y : qc.w, fz0 = pz ? qc.z : qb.x, fz1 = pz ? qc.w : qb.y;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int sh = (k & 3) * 8;
+            const bool hiw = k >= 4;
+            uint32_t m = ((hiw ? n1.w : n1.z) >> sh) & 0xFFu;
+            float tnx = (fmaf((float)(((hiw ? nx1 : nx0) >> sh) & 0xFFu), sx, n0.x) - o.x) * inv.x;
+            float tfx = (fmaf((float)(((hiw ? fx1 : fx0) >> sh) & 0xFFu), sx, n0.x) - o.x) * inv.x;
+            float tny = (fmaf((float)(((hiw ? ny1 : ny0) >> sh) & 0xFFu), sy, n0.y) - o.y) * inv.y;
+            float tfy = (fmaf((float)(((hiw ? fy1 : fy0) >> sh) & 0xFFu), sy, n0.y) - o.y) * inv.y;
+            float tnz = (fmaf((float)(((hiw ? nz1 : nz0) >> sh) & 0xFFu), sz, n0.z) - o.z) * inv.z;
+            float tfz = (fmaf((float)(((hiw ? fz1 : fz0) >> sh) & 0xFFu), sz, n0.z) - o.z) * inv.z;
+            bool inner = (imask >> k) & 1u;
+            uint32_t cnt = m >> 5;
+            float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+            float tmx = fminf(fminf(tfx, tfy), tfz);
+            tmin *= BOX_LO;
+            tmx = fminf(tmx * BOX_HI, tmax);
+            bool hit = (inner || cnt != 0u) && tmin <= tmx;
+            tm[k] = hit ? tmin : INF_F;
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int sh = (k & 3) * 8;
