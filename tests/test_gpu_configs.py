@@ -1,5 +1,7 @@
-"""The batched and served workloads at their real structure, reduced in
-resolution and samples:
+"""The still at its full shape, and the batched and served workloads at their
+real structure, reduced in resolution and samples:
+
+  * the signet still, 1080 x 1920, against the oracle on a band of rows;
 
   * the turntable batch: the full ~1.2 M-triangle signet, 36 turntable steps
     of 10 degrees x 4 alloys = 144 views with spp log-uniform (seeded), all in
@@ -35,6 +37,42 @@ def _oracle_render(desc, cam, turn, w, h, spp, seed, ps):
     osc.render_pass(cam, w, h, sums, 0, ps, spp, seed, threads=16)
     osc.close()
     return sums
+
+
+def test_signet_still_full_shape_matches_oracle_rows(ctx):
+    """The headline still at its full shape -- the 1,198,332-triangle signet
+    with its patina mask prepared on the device, 1080 x 1920, depth 12,
+    sterling in the white rig, the bench's camera and seed -- rendered whole
+    on the GPU and against the oracle on a band of 60 rows through the ring
+    (8 spp keeps the oracle's share to seconds)."""
+    wl = workloads.signet_still()
+    piece = wl["piece"]
+    w, h, seed = wl["width"], wl["height"], wl["seed"]
+    ps = workloads.path_settings(wl["depth"])
+    sc0 = ag.Scene(ctx, piece.scene(wl["metal"], 0.07, 0.8, wl["studio"]))
+    piece.prepare_patina(sc0)
+    cam, turn = piece.shot("hero", 100.0, wl["f_stop"], w / h, intersect=sc0.intersect)
+    sc0.close()
+    desc = piece.scene(wl["metal"], 0.07, 0.8, wl["studio"])
+    assert len(piece.triangles) == 1198332 and piece.cavity is not None and piece.cavity.max() > 0.0
+    desc.d.turntable_deg = turn
+    sc = ag.Scene(ctx, desc)
+    film = ag.Film(ctx, w, h)
+    ag.render_pass(ctx, sc, cam, film, ps, 8, seed)
+    gpu = film.sums()
+    film.close()
+    sc.close()
+    r0, r1 = 930, 990
+    osc = O.OracleScene(desc)
+    ref = np.zeros((3, h, w, 3), np.float32)
+    osc.render_pass(cam, w, h, ref, 0, ps, 8, seed, threads=16, rows=(r0, r1))
+    osc.close()
+    # the band crosses the ring: its guides see the metal
+    assert np.mean(np.abs(ref[1][r0:r1] - ref[1][r0:r1].mean())) > 0.0
+    e = rel_rmse(gpu[0][r0:r1], ref[0][r0:r1])
+    ea = rel_rmse(gpu[1][r0:r1], ref[1][r0:r1])
+    print(f"full-shape signet still, rows {r0}-{r1}: relative RMSE against the oracle {e:.2e} (albedo {ea:.2e})")
+    assert e < 0.01 and ea < 0.01
 
 
 def test_turntable_144_views_batched_equal_plain_passes(ctx):
