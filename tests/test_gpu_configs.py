@@ -1,7 +1,8 @@
 """The still at its full shape, and the batched and served workloads at their
 real structure, reduced in resolution and samples:
 
-  * the signet still, 1080 x 1920, against the oracle on a band of rows;
+  * the signet still and the gems still (HDRI and softboxes), 1080 x 1920,
+    against the oracle on a band of rows;
 
   * the turntable batch: the full ~1.2 M-triangle signet, 36 turntable steps
     of 10 degrees x 4 alloys = 144 views with spp log-uniform (seeded), all in
@@ -72,6 +73,39 @@ def test_signet_still_full_shape_matches_oracle_rows(ctx):
     e = rel_rmse(gpu[0][r0:r1], ref[0][r0:r1])
     ea = rel_rmse(gpu[1][r0:r1], ref[1][r0:r1])
     print(f"full-shape signet still, rows {r0}-{r1}: relative RMSE against the oracle {e:.2e} (albedo {ea:.2e})")
+    assert e < 0.01 and ea < 0.01
+
+
+def test_gems_still_full_shape_matches_oracle_rows(ctx):
+    """The gems still at its full shape -- the signet with its 12 diamond and
+    sapphire brilliants, the 2048 x 1024 HDRI importance-sampled with MIS
+    against the white rig's softboxes, 1080 x 1920, depth 24 -- rendered
+    whole on the GPU and against the oracle on a band of 60 rows through the
+    head plate and its stones (8 spp)."""
+    wl = workloads.signet_gems_still()
+    piece = wl["piece"]
+    w, h, seed = wl["width"], wl["height"], wl["seed"]
+    ps = workloads.path_settings(wl["depth"])
+    desc = piece.scene(wl["metal"], 0.07, 0.8, wl["studio"], gems=wl["gems"])
+    desc.set_env_map(wl["hdri"])
+    sc0 = ag.Scene(ctx, desc)
+    cam, turn = piece.shot("hero", 100.0, wl["f_stop"], w / h, intersect=sc0.intersect)
+    sc0.close()
+    desc.d.turntable_deg = turn
+    sc = ag.Scene(ctx, desc)
+    film = ag.Film(ctx, w, h)
+    ag.render_pass(ctx, sc, cam, film, ps, 8, seed)
+    gpu = film.sums()
+    film.close()
+    sc.close()
+    r0, r1 = 930, 990
+    osc = O.OracleScene(desc)
+    ref = np.zeros((3, h, w, 3), np.float32)
+    osc.render_pass(cam, w, h, ref, 0, ps, 8, seed, threads=16, rows=(r0, r1))
+    osc.close()
+    e = rel_rmse(gpu[0][r0:r1], ref[0][r0:r1])
+    ea = rel_rmse(gpu[1][r0:r1], ref[1][r0:r1])
+    print(f"full-shape gems still, rows {r0}-{r1}: relative RMSE against the oracle {e:.2e} (albedo {ea:.2e})")
     assert e < 0.01 and ea < 0.01
 
 
