@@ -227,9 +227,9 @@ def rel_rmse(a, b):
 
 
 def test_torus_noir_render_matches_oracle(ctx):
-    # the CPU reference still's scene in full: the 512 x 128-quad torus ring
-    # (131,072 triangles) in 14k gold under the noir rig at f/16, focused by
-    # the device's own autofocus ray
+    # the CPU reference still in full: the 512 x 128-quad torus ring (131,072
+    # triangles) in 14k gold under the noir rig at f/16, focused by the
+    # device's own autofocus ray, 512 x 512 at 64 spp
     wl = workloads.torus_still()
     piece = wl["piece"]
     assert len(piece.triangles) == 131072
@@ -238,7 +238,10 @@ def test_torus_noir_render_matches_oracle(ctx):
     cam, turn = piece.shot("hero", 100.0, wl["f_stop"], 1.0, intersect=sc.intersect)
     sc.close()
     assert wl["f_stop"] == 16.0 and cam.lens_radius > 0.0
-    gpu, ref = _render_both(ctx, desc, cam, 128, 128, workloads.path_settings(8), 8, 7, turn)
+    # the whole configuration: 512 x 512, 64 spp, depth 8, the bench's seed
+    assert (wl["width"], wl["height"], wl["spp"], wl["depth"]) == (512, 512, 64, 8)
+    gpu, ref = _render_both(ctx, desc, cam, wl["width"], wl["height"], workloads.path_settings(wl["depth"]), wl["spp"],
+                            wl["seed"], turn)
     e = rel_rmse(gpu[0], ref[0])
     same = np.mean(np.all(gpu[0] == ref[0], axis=-1))
     print(f"relative RMSE {e:.2e}, pixels bit-identical {same:.4f}")
