@@ -147,6 +147,12 @@ class Context:
             setattr(t, k, int(v))
         check(A.lib().ag_ctx_set_tuning(self.h, C.byref(t)))
 
+    def debug_fail_seed(self, seed):
+        """Failure injection for tests (ag_debug_fail_seed): render passes that
+        include a view rendered with `seed` fail before launching anything;
+        None turns it off."""
+        check(A.lib().ag_debug_fail_seed(self.h, -1 if seed is None else int(seed)))
+
     def enable_timing(self, on=True):
         check(A.lib().ag_ctx_enable_timing(self.h, 1 if on else 0))
 

@@ -260,6 +260,7 @@ _SIGS = [
     ("ag_server_release", C.c_int, [C.c_void_p, C.c_uint64]),
     ("ag_server_error", C.c_char_p, [C.c_void_p, C.c_uint64]),
     ("ag_server_budget", C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    ("ag_debug_fail_seed", C.c_int, [C.c_void_p, C.c_int64]),
     ("ag_device_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     ("ag_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("ag_memcpy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -121,6 +121,10 @@ struct ag_ctx {
     DevBuf energy_buf;  // the library's GGX energy table, staged into LDS by the shading kernel
     double last_query_ms = 0.0;
     void* noalloc_scratch = nullptr;  // this device's address of `scratch`
+    // failure injection for tests (ag_debug_fail_seed): render passes with a
+    // view whose seed hashes to this value fail before launching anything
+    bool fail_seed_on = false;
+    uint32_t fail_seed_h = 0;
     // captured render passes (launch_graph): the bytes of every launch
     // argument of a pass -> its executable graph, least recently used out
     struct Graph {
@@ -1077,6 +1081,9 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
         if (j.dv.max_depth > 64) return fail(AG_E_INVALID, "max_depth above 64");
         launch.max_depth = std::max(launch.max_depth, j.dv.max_depth);
     }
+    if (c->fail_seed_on)
+        for (auto& j : jobs)
+            if (j.dv.seed_h == c->fail_seed_h) return fail(AG_E_DEVICE, "injected render failure (ag_debug_fail_seed)");
     const ag_path_settings* ps = &launch;
     HIPCHK(hipSetDevice(c->device));
     uint32_t cap = c->tuning.paths_per_wave;
@@ -1740,7 +1747,8 @@ static int server_step(ag_server* s, const ViewJob* extra, uint32_t* n_active, u
                 if (extra_rc) *extra_rc = rc;
                 if (extra_ms) *extra_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
             }
-            if (chosen.size() == 1) {
+            if (chosen.size() == 1 && !extra) {
+                // the step rendered this request alone: the error is its own
                 server_fail(s, chosen[0], first_error);
             } else if (!chosen.empty()) {
                 for (uint64_t k : chosen) {
@@ -1762,6 +1770,14 @@ static int server_step(ag_server* s, const ViewJob* extra, uint32_t* n_active, u
     }
     if (n_active) *n_active = (uint32_t)s->active.size();
     if (n_queued) *n_queued = (uint32_t)s->queue.size();
+    return AG_OK;
+}
+
+extern "C" int ag_debug_fail_seed(ag_ctx* ctx, int64_t seed) {
+    if (!ctx) return fail(AG_E_INVALID, "null context");
+    if (seed > 0xFFFFFFFFll) return fail(AG_E_INVALID, "seed above 2^32 - 1");
+    ctx->fail_seed_on = seed >= 0;
+    ctx->fail_seed_h = seed >= 0 ? hash32_host((uint32_t)seed ^ 0xa511e9b3u) : 0u;
     return AG_OK;
 }
 

@@ -651,6 +651,13 @@ int ag_server_release(ag_server* s, uint64_t id);
 const char* ag_server_error(const ag_server* s, uint64_t id);
 /* Bytes charged against the budget now (films, scenes, wave state), and the budget. */
 int ag_server_budget(const ag_server* s, size_t* used, size_t* budget);
+/* Failure injection for tests of failure handling (no counterpart in the
+ * reference): from now on,
+ * every render pass on ctx that includes a view rendered with this seed fails
+ * with AG_E_DEVICE before launching anything; -1 turns it off.  Lets a test
+ * make one request's chunks fail from the host, with every kernel still
+ * exiting normally. */
+int ag_debug_fail_seed(ag_ctx* ctx, int64_t seed);
 
 /* ------------------------------------------------------------ tuning
  * Launch shape of the wavefront kernels (the autotuner's knobs). */

@@ -136,6 +136,77 @@ def test_failed_request_does_not_stop_the_others(ctx):
     sa.close()
 
 
+def test_render_failure_fails_only_its_request(ctx):
+    """A request whose chunks fail to render (forced from the host: every pass
+    with a view at seed 77 fails before launching) fails alone once retried by
+    itself; the requests that shared its steps finish and equal their renders
+    alone."""
+    pa, _ = _pieces()
+    sa = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
+    cam, turn = pa.shot("hero", 100.0, 0.0, 1.0)
+    ok1 = dict(scene=sa, camera=cam, width=32, height=32, spp=6, seed=3, turntable_deg=turn)
+    ok2 = dict(scene=sa, camera=cam, width=24, height=40, spp=5, seed=4, turntable_deg=turn)
+    ctx.debug_fail_seed(77)
+    try:
+        srv = Server(ctx, 64 << 30, 1 << 14, workloads.path_settings(8))
+        i1 = srv.submit(**ok1)
+        bad = srv.submit(sa, cam, 16, 16, 4, 77, turn)
+        i2 = srv.submit(**ok2)
+        for _ in range(200):
+            a, q = srv.step()
+            if a == 0 and q == 0:
+                break
+        assert srv.status(bad)["state"] == A.REQ_FAILED
+        assert "injected" in srv.error(bad)
+        imgs = []
+        for i in (i1, i2):
+            assert srv.status(i)["state"] == A.REQ_DONE and srv.error(i) == ""
+            imgs.append(srv.read(i))
+        used, _ = srv.budget()
+        srv.release(bad)
+        assert srv.budget()[0] == used  # the failed request's film was freed when it failed
+        srv.close()
+    finally:
+        ctx.debug_fail_seed(None)
+    assert np.array_equal(imgs[0], _serve(ctx, [ok1])[0][1])
+    assert np.array_equal(imgs[1], _serve(ctx, [ok2])[0][1])
+    sa.close()
+
+
+def test_live_pass_failure_does_not_fail_the_request_beside_it(ctx):
+    """A step that carries an attached live view's pass and one request: when
+    the live pass is what fails (seed 1, the live view's), the request is
+    retried alone and carries on; its image equals its render alone."""
+    pa, _ = _pieces()
+    sa = ag.Scene(ctx, pa.scene("sterling", 0.07, 0.8, "white"))
+    cam, turn = pa.shot("hero", 100.0, 0.0, 1.0)
+    req = dict(scene=sa, camera=cam, width=32, height=32, spp=6, seed=5, turntable_deg=turn)
+    srv = Server(ctx, 64 << 30, 1 << 14, workloads.path_settings(8))
+    rid = srv.submit(**req)
+    lv = ag.Live(ctx)
+    lv.attach(srv)
+    lv.set(sa, cam, 32, 32, max_spp=8, now_ms=0.0)
+    ctx.debug_fail_seed(1)
+    try:
+        with pytest.raises(ag.ArgentError):
+            lv.step(now_ms=0.0)
+    finally:
+        ctx.debug_fail_seed(None)
+    st = srv.status(rid)
+    assert st["state"] == A.REQ_ACTIVE and st["samples"] > 0, st
+    lv.attach(None)
+    for _ in range(200):
+        a, q = srv.step()
+        if a == 0 and q == 0:
+            break
+    assert srv.status(rid)["state"] == A.REQ_DONE
+    img = srv.read(rid)
+    srv.close()
+    lv.close()
+    assert np.array_equal(img, _serve(ctx, [req])[0][1])
+    sa.close()
+
+
 def test_arrivals_every_step_do_not_starve_work_in_progress(ctx):
     """A new request before every step: at most 4 preview-only steps run in a
     row, so a request in progress keeps gaining samples."""
