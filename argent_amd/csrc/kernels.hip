@@ -1393,8 +1393,11 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
             const float prev_in = lit ? (d0 ? 1.0f : P.prev_pdf[path]) : 0.0f;
             float prev_pdf = 0.0f;
             uint32_t flags = d0 ? (1u | (255u << 8)) : P.flags[path];
-            const uint32_t ps = SURF ? P.pix_seed[path] : 0u, si = SURF ? P.si[path] : 0u;
-            uint64_t rng = SURF ? (d0 ? rng_init(ps, si) : P.rng[path]) : 0u;
+            // the sampler's pixel seed, sample index and generator: only a
+            // surface or floor hit draws from them (a miss or a light ends here)
+            const bool draws = SURF && (!lit || code == HIT_FLOOR);
+            const uint32_t ps = draws ? P.pix_seed[path] : 0u, si = draws ? P.si[path] : 0u;
+            uint64_t rng = draws ? (d0 ? rng_init(ps, si) : P.rng[path]) : 0u;
             float4 g_alb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g_nrm = g_alb;  // depth 0: the guides
             const DView& V = A.views[P.view[path]];  // fields read below with uload
             const DScene& S = *uload(&V.scene);
