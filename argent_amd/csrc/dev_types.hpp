@@ -128,6 +128,8 @@ struct DSeg {
     uint32_t s0, ns;
 };
 
+// Path state, one slot per path (the view a path belongs to travels in its
+// ray and shadow records instead).
 struct DPaths {
     float4* beta;
     float4* L;
@@ -138,20 +140,19 @@ struct DPaths {
     uint64_t* rng;        // next_random state
     uint32_t* pix_seed;
     uint32_t* si;
-    uint32_t* view;
     float4* galb;         // guide albedo
     float4* gnrm;         // guide normal
 };
 
 struct DQueues {
     float4* ray_o[2];     // (o.xyz, path id bits)
-    float4* ray_d[2];     // (d.xyz, depth bits)
+    float4* ray_d[2];     // (d.xyz, view << 8 | depth bits)
     float4* hit;          // (t, u, v, code bits)
     uint32_t* bucket;     // MAX_KEYS-strided buckets of ray indices, capacity cap each
     float4* sh_o;         // (origin.xyz, path id bits)
     float4* sh_d;         // panel: (light point.xyz, ignore panel bits); sky: (dir.xyz, -1 bits)
     float4* sh_c;         // contribution spectrum
-    float4* sh_x;         // (a, p_light, clamp or -1, kind bits)
+    float4* sh_x;         // (a, p_light, clamp or -1, view << 1 | kind bits)
     uint32_t* counters;   // per depth: [ray count, shadow count, key counts...]
     uint32_t* cursors;    // per depth: extension set, shadow set (CURSOR_SET u32 each)
     uint32_t cap;

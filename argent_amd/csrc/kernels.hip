@@ -16,6 +16,13 @@
 namespace ag {
 
 // ------------------------------------------------------------------ wave helpers
+// An extension ray's w word: its view (bits 8-31) and depth (bits 0-7; depth
+// never exceeds 64).  Traversal and shading take the view from here instead
+// of a per-path array, a load scattered over the path slots; a shadow
+// record's w word carries the view the same way (bits 1-31, bit 0 its kind).
+AGD uint32_t ray_tag(uint32_t view, uint32_t depth) { return view << 8 | depth; }
+AGD uint32_t tag_view(uint32_t tag) { return tag >> 8; }
+
 AGD uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 // Runs f(u) for the active lanes once per distinct value u of `key` among
@@ -1078,9 +1085,8 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
         P.pdf[path] = to4(pdf);
         P.pix_seed[path] = ps;
         P.si[path] = si;
-        P.view[path] = sg.view;
         Q.ray_o[0][slot] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
-        Q.ray_d[0][slot] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
+        Q.ray_d[0][slot] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(ray_tag(sg.view, 0u)));
     }
 }
 
@@ -1106,13 +1112,12 @@ struct ExtendJob {
     // hit, so the piece wins ties as it does when tested first.  Until the
     // piece is hit, L.u / L.v hold the panel-or-floor code and distance.
     AGD bool begin(uint32_t i, TravLane& L) {
-        float4 a = ro[i];
+        float4 a = ro[i], b = rd[i];
         if (__float_as_uint(a.w) == DEAD_PATH) {  // a path that ended: nothing to trace or shade
             L.aux = NO_RAY;
             return false;
         }
-        float4 b = rd[i];
-        uint32_t vi = P.view[__float_as_uint(a.w)];
+        const uint32_t vi = tag_view(__float_as_uint(b.w));
         per_uniform(vi, [&](uint32_t v) {
             const DScene* Sp = uload(&views[v].scene);
             const float turn = uload(&views[v].turn_deg), tsn = uload(&views[v].tsn), tcn = uload(&views[v].tcn);
@@ -1364,7 +1369,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
         uint32_t j = base + threadIdx.x;
         bool emit_ray = false;
         f3 nro, nrd;
-        uint32_t path = 0;
+        uint32_t path = 0, vidx = 0;
         const float4 a = pa, b = pb, hh = ph;
         if (j + stride < total) {
             pa = ro[r_next];
@@ -1399,7 +1404,8 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
             const uint32_t ps = draws ? P.pix_seed[path] : 0u, si = draws ? P.si[path] : 0u;
             uint64_t rng = draws ? (d0 ? rng_init(ps, si) : P.rng[path]) : 0u;
             float4 g_alb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g_nrm = g_alb;  // depth 0: the guides
-            const DView& V = A.views[P.view[path]];  // fields read below with uload
+            vidx = tag_view(__float_as_uint(b.w));
+            const DView& V = A.views[vidx];  // fields read below with uload
             const DScene& S = *uload(&V.scene);
             const float turn = uload(&V.turn_deg), vts = uload(&V.ts), vtc = uload(&V.tc);
             const float env_pick = uload(&S.env_pick);
@@ -1534,7 +1540,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                                 Q.sh_o[ss] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
                                 Q.sh_d[ss] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0xFFFFFFFFu));
                                 Q.sh_c[ss] = to4(base);
-                                Q.sh_x[ss] = make_float4(fabsf(wi.z) * w, p_light, rough ? cm : -1.0f, __uint_as_float(1u));
+                                Q.sh_x[ss] = make_float4(fabsf(wi.z) * w, p_light, rough ? cm : -1.0f, __uint_as_float(vidx << 1 | 1u));
                             }
                         }
                     } else if (uload(&S.n_lights) > 0) {
@@ -1566,7 +1572,7 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                                 Q.sh_o[ss] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
                                 Q.sh_d[ss] = make_float4(lp.x, lp.y, lp.z, __uint_as_float((uint32_t)li));
                                 Q.sh_c[ss] = to4(c);
-                                Q.sh_x[ss] = make_float4(0.0f, 0.0f, rough ? cm : -1.0f, __uint_as_float(0u));
+                                Q.sh_x[ss] = make_float4(0.0f, 0.0f, rough ? cm : -1.0f, __uint_as_float(vidx << 1));
                             }
                         }
                     }
@@ -1628,14 +1634,14 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
             // depth's hit order whatever order the blocks run in
             if (j < total && (uint32_t)depth + 1 < A.max_depth) {
                 no[jfirst + j] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(emit_ray ? path : DEAD_PATH));
-                if (emit_ray) nd[jfirst + j] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
+                if (emit_ray) nd[jfirst + j] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float(ray_tag(vidx, (uint32_t)depth + 1)));
             }
             (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
         } else {
             uint32_t slot = wave_append(&cnt_next[CNT_RAYS], emit_ray);
             if (emit_ray) {
                 no[slot] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(path));
-                nd[slot] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float((uint32_t)depth + 1));
+                nd[slot] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float(ray_tag(vidx, (uint32_t)depth + 1)));
             }
             (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
         }
@@ -1662,9 +1668,10 @@ struct ShadowJob {
         float4 a = Q.sh_o[i], b = Q.sh_d[i];
         uint32_t path = __float_as_uint(a.w);
         f3 from{a.x, a.y, a.z};
-        uint32_t kind = __float_as_uint(Q.sh_x[i].w);
+        const uint32_t xw = __float_as_uint(Q.sh_x[i].w);
+        const uint32_t kind = xw & 1u;
         bool blocked = false;
-        per_uniform(P.view[path], [&](uint32_t v) {
+        per_uniform(xw >> 1, [&](uint32_t v) {
             const DScene* Sp = uload(&views[v].scene);
             const float turn = uload(&views[v].turn_deg), tsn = uload(&views[v].tsn), tcn = uload(&views[v].tcn);
             const int np = uload(&Sp->n_panels);
@@ -1729,7 +1736,7 @@ struct ShadowJob {
         if (!mine || L.prim != NO_RAY) return;  // blocked
         float4 x = Q.sh_x[i];
         Spec c = from4(Q.sh_c[i]);
-        if (__float_as_uint(x.w) != 0u) c = c * (x.x * L.u / x.y);
+        if (__float_as_uint(x.w) & 1u) c = c * (x.x * L.u / x.y);
         uint32_t path = L.aux;
         Spec Lr = from4(P.L[path]);
         add_clamped(Lr, c, x.z >= 0.0f, x.z);

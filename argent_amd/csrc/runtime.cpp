@@ -85,6 +85,10 @@ constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_CURSOR_BYTES + NOALLOC_SPILL_BY
 static_assert(CURSOR_SET * sizeof(uint32_t) <= NOALLOC_CURSOR_BYTES, "cursor area too small");
 static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
+// Bytes of path state per path slot (DPaths: beta, L, lam, pdf, guide albedo
+// and normal, previous pdf, flags, pixel seed, sample index, generator).
+constexpr size_t PATH_STATE_BYTES = 6 * sizeof(float4) + sizeof(float) + 3 * sizeof(uint32_t) + sizeof(uint64_t);
+
 struct ag_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -394,7 +398,7 @@ static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) 
         if (cap < c->cap) {  // give a large wave's memory back rather than keep it
             for (DevBuf* b : {&c->paths_buf, &c->rays_buf, &c->hit_buf, &c->shadow_buf, &c->bucket_buf}) b->release();
         }
-        size_t per_path = 6 * sizeof(float4) + sizeof(float) + 4 * sizeof(uint32_t) + sizeof(uint64_t);
+        const size_t per_path = PATH_STATE_BYTES;
         if ((rc = c->paths_buf.ensure((size_t)cap * per_path))) return rc;
         if ((rc = c->rays_buf.ensure((size_t)cap * 4 * sizeof(float4)))) return rc;
         if ((rc = c->hit_buf.ensure((size_t)cap * sizeof(float4)))) return rc;
@@ -416,7 +420,6 @@ static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) 
         c->P.flags = (uint32_t*)take((size_t)cap * 4);
         c->P.pix_seed = (uint32_t*)take((size_t)cap * 4);
         c->P.si = (uint32_t*)take((size_t)cap * 4);
-        c->P.view = (uint32_t*)take((size_t)cap * 4);
         float4* r = c->rays_buf.as<float4>();
         c->Q.ray_o[0] = r;
         c->Q.ray_d[0] = r + cap;
@@ -1077,6 +1080,8 @@ static int run_views(ag_ctx* c, const ag_scene* s, std::vector<ViewJob>& jobs, c
                      ag_event* ev = nullptr) {
     // every view carries its own path settings; the bounce loop runs to the deepest
     ag_path_settings launch{0, 0.0f, 0};
+    // a ray carries its view in 24 bits of its w word
+    if (jobs.size() > (1u << 24)) return fail(AG_E_INVALID, "more than 2^24 views in one render");
     for (auto& j : jobs) {
         if (j.dv.max_depth > 64) return fail(AG_E_INVALID, "max_depth above 64");
         launch.max_depth = std::max(launch.max_depth, j.dv.max_depth);
@@ -1474,7 +1479,7 @@ constexpr int SERVER_PREVIEW_RUN_MAX = 4;
 size_t wave_state_bytes(const ag_ctx* c, int n_keys, uint32_t max_depth) {
     n_keys = wave_keys(c, n_keys);
     const size_t cap = c->tuning.paths_per_wave;
-    const size_t per_path = 6 * sizeof(float4) + sizeof(float) + 4 * sizeof(uint32_t) + sizeof(uint64_t);
+    const size_t per_path = PATH_STATE_BYTES;
     size_t b = cap * (per_path + 4 * sizeof(float4) + sizeof(float4) + 4 * sizeof(float4) + (size_t)n_keys * 4);
     b += (counter_words(max_depth) + (size_t)(max_depth + 2) * CURSORS_PER_DEPTH) * sizeof(uint32_t);
     b += MAX_SEGMENTS * sizeof(DSeg);

@@ -663,7 +663,7 @@ int ag_debug_fail_seed(ag_ctx* ctx, int64_t seed);
  * Launch shape of the wavefront kernels (the autotuner's knobs). */
 typedef struct ag_tuning {
     uint32_t paths_per_wave;   /* path slots in flight (1024..AG_MAX_PATHS_PER_WAVE,
-                                  about 284 bytes each) */
+                                  about 280 bytes each) */
     uint32_t block_traverse;   /* threads per block, traversal kernels */
     uint32_t blocks_per_cu;    /* persistent traversal blocks per CU, render passes */
     uint32_t sort_by_material; /* sort granularity of shading: 1 = material-sorted
