@@ -155,7 +155,8 @@ struct DQueues {
     float4* sh_x;         // (a, p_light, clamp or -1, view << 1 | kind bits)
     uint32_t* counters;   // per depth: [ray count, shadow count, key counts...]
     uint32_t* cursors;    // per depth: extension set, shadow set (CURSOR_SET u32 each)
-    uint32_t cap;
+    uint32_t cap;         // path slots (and each bucket's capacity)
+    uint32_t qcap;        // slots of each ray, hit and shadow queue: cap and the chunk tails' room
     int32_t n_keys;
     uint32_t slices;      // render traversal: queue slices with a cursor each (1..CURSOR_SLICES)
     uint32_t bands;       // ray generation: a segment's camera rays queued in this many bands of
@@ -172,15 +173,29 @@ struct DQueues {
 // atomics on different counters of one line serialise with each other (two
 // bucket keys sharing a line made the traversal that appends to them 2.7x
 // slower than one key alone).  Word offsets within a depth slot:
-// [CNT_RAYS] extension queue length, [CNT_SHADOW] shadow rays, [CNT_KEY +
-// k * CNT_PAD] bucket k's size, [CNT_LIVE] live extension rays (the queue
-// length less its dead slots when shading emits in order).
+// [CNT_RAYS] extension queue length, [CNT_SHADOW] shadow queue length,
+// [CNT_KEY + k * CNT_PAD] bucket k's size, [CNT_LIVE] live extension rays and
+// [CNT_SHLIVE] shadow rays emitted (the queue lengths less their dead slots:
+// slots of paths that ended when shading emits in order, the unused tails of
+// reserved chunks).
 constexpr int CNT_PAD = 32;  // u32 per counter (128 B)
 constexpr int CNT_RAYS = 0;
 constexpr int CNT_SHADOW = 1 * CNT_PAD;
 constexpr int CNT_KEY = 2 * CNT_PAD;
 constexpr int CNT_LIVE = (2 + MAX_KEYS) * CNT_PAD;
-constexpr int CNT_STRIDE = (3 + MAX_KEYS) * CNT_PAD;  // u32 per depth slot
+constexpr int CNT_SHLIVE = (3 + MAX_KEYS) * CNT_PAD;
+constexpr int CNT_STRIDE = (4 + MAX_KEYS) * CNT_PAD;  // u32 per depth slot
+// One 128-byte counter takes about 88 M device-scope atomics per second on
+// the MI355X (11 ns each, returning or not): a queue that every shading
+// wavefront appends to once per hit batch would bound a 2^29-hit launch at
+// ~95 ms.  Shading therefore reserves queue slots SHADE_CHUNK at a time per
+// wavefront once a launch shades at least SHADE_CHUNK_MIN hits, leaving dead
+// slots only in the tail of each wavefront's last chunk; the queues carry
+// room for those tails (SHADE_LAUNCHES_MAX shading launches per depth, each
+// wavefront of each with one tail).
+constexpr uint32_t SHADE_CHUNK = 512;
+constexpr uint32_t SHADE_CHUNK_MIN = 1u << 26;
+constexpr uint32_t SHADE_LAUNCHES_MAX = 4;
 // Bucket keys of shading by family (sort mode 4): misses, panel hits, metal
 // surfaces, diffuse surfaces (the floor when its material is diffuse), every
 // other surface.

@@ -330,6 +330,48 @@ AGD uint32_t lane_prefix(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Queue appends of one wavefront that reserve slots `chunk` at a time (0:
+// one reservation per append, exact).  Called by every lane of the wavefront
+// at a converged point, so the chunk state stays wavefront-uniform.
+struct ChunkAppend {
+    uint32_t* counter;
+    uint32_t chunk;
+    uint32_t base, used;  // the current chunk [base, base + chunk) and the slots taken from it
+    bool have;
+    AGD uint32_t append(bool pred) {
+        const uint64_t m = __ballot(pred);
+        const uint32_t k = (uint32_t)__popcll(m), prefix = lane_prefix(m);
+        if (k == 0) return 0;
+        if (chunk == 0) {
+            uint32_t b = 0;
+            if (lane_id() == 0) b = atomicAdd(counter, k);
+            return __shfl(b, 0) + prefix;
+        }
+        const uint32_t rem = have ? chunk - used : 0u;
+        if (k <= rem) {
+            const uint32_t s = base + used + prefix;
+            used += k;
+            return s;
+        }
+        // the rest of this chunk, then a new one
+        uint32_t nb = 0;
+        if (lane_id() == 0) nb = atomicAdd(counter, chunk);
+        nb = __shfl(nb, 0);
+        const uint32_t s = prefix < rem ? base + used + prefix : nb + (prefix - rem);
+        base = nb;
+        used = k - rem;
+        have = true;
+        return s;
+    }
+    // mark the unused tail of the last chunk dead (w word = DEAD_PATH)
+    AGD void close(float4* q, uint32_t qcap) {
+        if (chunk == 0 || !have) return;
+        for (uint32_t e = used + lane_id(); e < chunk; e += 64)
+            if (base + e < qcap) q[base + e] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(DEAD_PATH));
+    }
+};
+
+
 // Wavefront-uniform state of the ray supply.
 struct RaySupply {
     uint32_t n;
@@ -1094,7 +1136,11 @@ __global__ __launch_bounds__(256) void k_raygen(const DSeg* __restrict__ segs, i
 // the piece in object space (ray turned by -turntable), then panels and floor
 // in world space.  Finished rays are staged per wavefront in LDS and appended
 // to the material buckets 64 or more at a time (one atomic per key per flush).
-constexpr int STAGE = 128;  // staging entries per wavefront: ray index, and its key beside it
+// Staging entries per wavefront (ray index, and its key beside it): flushed
+// once FLUSH_AT are staged, with one atomic per key present, so a bucket
+// counter sees one atomic per FLUSH_AT rays or so instead of one per 64.
+constexpr uint32_t FLUSH_AT = 512;
+constexpr int STAGE = FLUSH_AT + 64;
 
 struct ExtendJob {
     const DView* views;
@@ -1196,9 +1242,9 @@ struct ExtendJob {
             skey[e] = (uint8_t)key;
         }
         staged += (uint32_t)__popcll(m);
-        if (staged >= 64) flush();
+        if (staged >= FLUSH_AT) flush();
     }
-    AGD void flush() {
+    __device__ __attribute__((noinline)) void flush() {  // out of line: runs once per FLUSH_AT rays, and inlined its temporaries cost the traversal loop a wavefront per SIMD
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1226,16 +1272,32 @@ struct ExtendJob {
             staged = 0;
             return;
         }
-        for (uint32_t base = 0; base < staged; base += 64) {
-            uint32_t e = base + lane_id();
-            int key = -1;
-            uint32_t i = 0;
-            if (e < staged) {
-                i = stage[e];
-                key = (int)skey[e];
+        // the keys present (MAX_KEYS <= 64), then per key one atomic for all
+        // its staged rays and their slots in staging order
+        uint64_t keys = 0;
+        for (uint32_t c = 0; c < staged; c += 64) {
+            const uint32_t e = c + lane_id();
+            if (e < staged) keys |= 1ull << skey[e];
+        }
+        for (int o = 32; o > 0; o >>= 1) keys |= (uint64_t)__shfl_xor((unsigned long long)keys, o);
+        while (keys) {
+            const int k = __ffsll((unsigned long long)keys) - 1;
+            keys &= keys - 1ull;
+            uint32_t n = 0;
+            for (uint32_t c = 0; c < staged; c += 64) {
+                const uint32_t e = c + lane_id();
+                n += (uint32_t)__popcll(__ballot(e < staged && (int)skey[e] == k));
             }
-            uint32_t slot = wave_append_keyed(cnt + CNT_KEY, key);
-            if (key >= 0) Q.bucket[(size_t)key * Q.cap + slot] = i;
+            uint32_t base = 0;
+            if (lane_id() == 0) base = atomicAdd(cnt + CNT_KEY + k * CNT_PAD, n);
+            base = __shfl(base, 0);
+            for (uint32_t c = 0; c < staged; c += 64) {
+                const uint32_t e = c + lane_id();
+                const bool mine = e < staged && (int)skey[e] == k;
+                const uint64_t m = __ballot(mine);
+                if (mine) Q.bucket[(size_t)k * Q.cap + base + lane_prefix(m)] = stage[e];
+                base += (uint32_t)__popcll(m);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1255,7 +1317,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* _
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64],
                 s_skey[threadIdx.x / 64], 0u};
-    traverse_persistent<false>(J, cnt[CNT_RAYS], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread, Q.slices);
+    traverse_persistent<false>(J, min(cnt[CNT_RAYS], Q.qcap), Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread, Q.slices);
 }
 
 struct ShadeArgs {
@@ -1302,13 +1364,22 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
         for (int k = 0; k < Q.n_keys; k++) tot += cnt[CNT_KEY + k * CNT_PAD];
         cnt_next[CNT_RAYS] = (uint32_t)depth + 1 < A.max_depth ? tot : 0u;
     }
+    // Wavefront w shades runs of RUN batches of 64 consecutive hits: batch t
+    // is hits entry(t) .. entry(t) + 63, runs dealt to the wavefronts in turn
+    // (RUN 1: a plain grid stride).  With chunked queues RUN is a chunk's
+    // worth, so one chunk holds the continuations of consecutive hits and the
+    // next queue stays spatially ordered chunk by chunk.
+    uint32_t n_hits = 0;
+    for (int k = k0; k < k1; k++) n_hits += cnt[CNT_KEY + k * CNT_PAD];
+    const uint32_t chunk = n_hits >= SHADE_CHUNK_MIN ? SHADE_CHUNK : 0u;
+    const uint32_t RUN = chunk ? chunk / 64u : 1u;
+    const uint32_t n_wave = gridDim.x * (blockDim.x / 64u), wave_g = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u;
+    auto entry = [&](uint32_t t) -> uint64_t {
+        return ((uint64_t)(t / RUN) * n_wave + wave_g) * (uint64_t)(RUN * 64u) + (uint64_t)(t % RUN) * 64u;
+    };
     // blocks past this launch's hit count have nothing to shade (uniform per
     // block, before any barrier)
-    {
-        uint32_t n = 0;
-        for (int k = k0; k < k1; k++) n += cnt[CNT_KEY + k * CNT_PAD];
-        if (blockIdx.x * blockDim.x >= n) return;
-    }
+    if ((uint64_t)blockIdx.x * (blockDim.x / 64u) * RUN * 64u >= n_hits) return;
     __shared__ uint32_t pre[MAX_KEYS + 1];
     __shared__ uint32_t s_first;  // hits in the buckets before k0 (in-order emission's slots)
     // microfacet.rs:93-140 E(mu, sqrt(alpha)): read by every rough lobe's
@@ -1346,37 +1417,45 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
     const float4* rd = Q.ray_d[depth & 1];
     float4* no = Q.ray_o[(depth + 1) & 1];
     float4* nd = Q.ray_d[(depth + 1) & 1];
-    // Software pipeline over the grid-stride loop: each iteration issues the
-    // next iteration's ray and hit loads and the bucket entry of the one after,
+    // Software pipeline over a wavefront's batches: each iteration issues the
+    // next batch's ray and hit loads and the bucket entry of the one after,
     // so two of a hit's dependent memory round trips (bucket -> ray -> path
     // state) overlap the previous hit's shading.
-    const uint32_t stride = gridDim.x * blockDim.x;
     auto slot_of = [&](uint32_t jj) -> uint32_t {
         int key = 0;
         while (key + 1 < nk && pre[key + 1] <= jj) key++;
         return Q.bucket[(size_t)(k0 + key) * Q.cap + (jj - pre[key])];
     };
-    const uint32_t j0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint64_t j0 = entry(0) + lane, j1 = entry(1) + lane;
     float4 pa = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pb = pa, ph = pa;
     if (j0 < total) {
-        const uint32_t r0 = slot_of(j0);
+        const uint32_t r0 = slot_of((uint32_t)j0);
         pa = ro[r0];
         pb = rd[r0];
         ph = Q.hit[r0];
     }
-    uint32_t r_next = j0 + stride < total ? slot_of(j0 + stride) : 0u;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
-        uint32_t j = base + threadIdx.x;
+    uint32_t r_next = j1 < total ? slot_of((uint32_t)j1) : 0u;
+    // the next depth's rays and this depth's shadow rays: appended at the
+    // loop's converged point, in reserved chunks for large launches
+    ChunkAppend rays_q{&cnt_next[CNT_RAYS], chunk, 0u, 0u, false};
+    ChunkAppend shadow_q{&cnt[CNT_SHADOW], chunk, 0u, 0u, false};
+    uint32_t n_live = 0, n_shadow = 0;  // wavefront-uniform counts of what was emitted
+    for (uint32_t t = 0; entry(t) < total; t++) {
+        const uint64_t jw = entry(t) + lane, jn = entry(t + 1) + lane, jnn = entry(t + 2) + lane;
+        const uint32_t j = (uint32_t)min(jw, (uint64_t)total);  // >= total: no hit for this lane
         bool emit_ray = false;
         f3 nro, nrd;
         uint32_t path = 0, vidx = 0;
+        bool sh_emit = false;  // this hit's next-event record, written after the loop body
+        float4 sho, shd, shc, shx;
         const float4 a = pa, b = pb, hh = ph;
-        if (j + stride < total) {
+        if (jn < total) {
             pa = ro[r_next];
             pb = rd[r_next];
             ph = Q.hit[r_next];
         }
-        if (j + 2 * stride < total) r_next = slot_of(j + 2 * stride);
+        if (jnn < total) r_next = slot_of((uint32_t)jnn);
         if (j < total) {
             path = __float_as_uint(a.w);
             f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
@@ -1534,13 +1613,11 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                                 f3 org = offset_origin(p, ngu, dir);
                                 Spec le = env_eval(S, dir, lam, &W);
                                 Spec base = beta * f * le;
-                                // appended here (lanes in this branch only) so the
-                                // record does not stay live through the BSDF sample
-                                uint32_t ss = wave_append(&cnt[CNT_SHADOW], true);
-                                Q.sh_o[ss] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
-                                Q.sh_d[ss] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0xFFFFFFFFu));
-                                Q.sh_c[ss] = to4(base);
-                                Q.sh_x[ss] = make_float4(fabsf(wi.z) * w, p_light, rough ? cm : -1.0f, __uint_as_float(vidx << 1 | 1u));
+                                sh_emit = true;
+                                sho = make_float4(org.x, org.y, org.z, __uint_as_float(path));
+                                shd = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0xFFFFFFFFu));
+                                shc = to4(base);
+                                shx = make_float4(fabsf(wi.z) * w, p_light, rough ? cm : -1.0f, __uint_as_float(vidx << 1 | 1u));
                             }
                         }
                     } else if (uload(&S.n_lights) > 0) {
@@ -1568,11 +1645,11 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                                 f3 org = offset_origin(p, ngu, dir);
                                 Spec le = panel_le(Pn, lp, dir, lam);
                                 Spec c = beta * f * le * (fabsf(wi.z) * w / p_light);
-                                uint32_t ss = wave_append(&cnt[CNT_SHADOW], true);
-                                Q.sh_o[ss] = make_float4(org.x, org.y, org.z, __uint_as_float(path));
-                                Q.sh_d[ss] = make_float4(lp.x, lp.y, lp.z, __uint_as_float((uint32_t)li));
-                                Q.sh_c[ss] = to4(c);
-                                Q.sh_x[ss] = make_float4(0.0f, 0.0f, rough ? cm : -1.0f, __uint_as_float(vidx << 1));
+                                sh_emit = true;
+                                sho = make_float4(org.x, org.y, org.z, __uint_as_float(path));
+                                shd = make_float4(lp.x, lp.y, lp.z, __uint_as_float((uint32_t)li));
+                                shc = to4(c);
+                                shx = make_float4(0.0f, 0.0f, rough ? cm : -1.0f, __uint_as_float(vidx << 1));
                             }
                         }
                     }
@@ -1629,6 +1706,16 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                 P.rng[path] = rng;
             }
         }
+        {
+            const uint32_t ss = shadow_q.append(sh_emit);
+            if (sh_emit && ss < Q.qcap) {
+                Q.sh_o[ss] = sho;
+                Q.sh_d[ss] = shd;
+                Q.sh_c[ss] = shc;
+                Q.sh_x[ss] = shx;
+            }
+            n_shadow += (uint32_t)__popcll(__ballot(sh_emit));
+        }
         if (Q.ordered) {
             // hit j's continuation goes to slot j: the next queue keeps this
             // depth's hit order whatever order the blocks run in
@@ -1636,15 +1723,21 @@ AGD void shade_bounce(const ShadeArgs& A, const DPaths& P, const DQueues& Q, int
                 no[jfirst + j] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(emit_ray ? path : DEAD_PATH));
                 if (emit_ray) nd[jfirst + j] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float(ray_tag(vidx, (uint32_t)depth + 1)));
             }
-            (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
         } else {
-            uint32_t slot = wave_append(&cnt_next[CNT_RAYS], emit_ray);
-            if (emit_ray) {
+            const uint32_t slot = rays_q.append(emit_ray);
+            if (emit_ray && slot < Q.qcap) {
                 no[slot] = make_float4(nro.x, nro.y, nro.z, __uint_as_float(path));
                 nd[slot] = make_float4(nrd.x, nrd.y, nrd.z, __uint_as_float(ray_tag(vidx, (uint32_t)depth + 1)));
             }
-            (void)wave_append(&cnt_next[CNT_LIVE], emit_ray);
         }
+        n_live += (uint32_t)__popcll(__ballot(emit_ray));
+    }
+    // dead tails of the reserved chunks; the emitted counts, one atomic each per wavefront
+    rays_q.close(no, Q.qcap);
+    shadow_q.close(Q.sh_o, Q.qcap);
+    if (lane_id() == 0) {
+        if (n_live) atomicAdd(&cnt_next[CNT_LIVE], n_live);
+        if (n_shadow) atomicAdd(&cnt[CNT_SHLIVE], n_shadow);
     }
 }
 
@@ -1667,6 +1760,11 @@ struct ShadowJob {
     AGD bool begin(uint32_t i, TravLane& L) {
         float4 a = Q.sh_o[i], b = Q.sh_d[i];
         uint32_t path = __float_as_uint(a.w);
+        if (path == DEAD_PATH) {  // an unused slot of a reserved chunk
+            L.aux = path;
+            L.prim = 1;
+            return false;
+        }
         f3 from{a.x, a.y, a.z};
         const uint32_t xw = __float_as_uint(Q.sh_x[i].w);
         const uint32_t kind = xw & 1u;
@@ -1750,7 +1848,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow(const DView* _
     TRAV_STACK_DECL
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ShadowJob J{views, P, Q};
-    traverse_persistent<true>(J, cnt[CNT_SHADOW], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st,
+    traverse_persistent<true>(J, min(cnt[CNT_SHADOW], Q.qcap), Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET, refill, st,
                               spread, Q.slices);
 }
 
@@ -1772,7 +1870,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
     {
         uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
         ShadowJob J{views, P, Q};
-        traverse_persistent<true>(J, cnt[CNT_SHADOW], Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET,
+        traverse_persistent<true>(J, min(cnt[CNT_SHADOW], Q.qcap), Q.cursors + (size_t)depth * CURSORS_PER_DEPTH + CURSOR_SET,
                                   refill_shadow, st, spread, Q.slices);
     }
     {
@@ -1780,7 +1878,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
         uint32_t* cnt = Q.counters + nd * CNT_STRIDE;
         ExtendJob J{views, P, Q, nd, cnt, Q.ray_o[nd & 1], Q.ray_d[nd & 1], s_stage[threadIdx.x / 64],
                     s_skey[threadIdx.x / 64], 0u};
-        traverse_persistent<false>(J, cnt[CNT_RAYS], Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread, Q.slices);
+        traverse_persistent<false>(J, min(cnt[CNT_RAYS], Q.qcap), Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread, Q.slices);
     }
 }
 
@@ -1851,7 +1949,7 @@ __global__ void k_tally(const uint32_t* __restrict__ counters, uint32_t depth_sl
             // live rays: the queue length at depth 0 (camera rays), the
             // continuation count after that (an in-order queue has dead slots)
             uint32_t ed = d == 0 ? counters[CNT_RAYS] : counters[d * CNT_STRIDE + CNT_LIVE];
-            uint32_t sd = counters[d * CNT_STRIDE + CNT_SHADOW];
+            uint32_t sd = counters[d * CNT_STRIDE + CNT_SHLIVE];
             e += ed;
             s += sd;
             if (d < (uint32_t)TALLY_DEPTHS) {

@@ -391,6 +391,15 @@ static int wave_keys(const ag_ctx* c, int n_keys) {
     return c->tuning.sort_by_material == 4 ? std::max(n_keys, FAM_KEYS) : n_keys;
 }
 
+// Slots of each ray, hit and shadow queue beyond the path slots: room for
+// the dead tails of shading's reserved chunks (one per shading wavefront of
+// each of up to SHADE_LAUNCHES_MAX launches per depth; the shading grid is
+// n_cu * 8 blocks of 4 wavefronts), needed only when a launch can shade
+// SHADE_CHUNK_MIN hits.
+static size_t queue_slack(const ag_ctx* c, uint32_t cap) {
+    return cap >= SHADE_CHUNK_MIN ? (size_t)SHADE_LAUNCHES_MAX * c->n_cu * 8 * (256 / 64) * SHADE_CHUNK : 0;
+}
+
 static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) {
     int rc = ensure_spill(c);
     if (rc) return rc;
@@ -400,9 +409,10 @@ static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) 
         }
         const size_t per_path = PATH_STATE_BYTES;
         if ((rc = c->paths_buf.ensure((size_t)cap * per_path))) return rc;
-        if ((rc = c->rays_buf.ensure((size_t)cap * 4 * sizeof(float4)))) return rc;
-        if ((rc = c->hit_buf.ensure((size_t)cap * sizeof(float4)))) return rc;
-        if ((rc = c->shadow_buf.ensure((size_t)cap * 4 * sizeof(float4)))) return rc;
+        const size_t qcap = (size_t)cap + queue_slack(c, cap);
+        if ((rc = c->rays_buf.ensure(qcap * 4 * sizeof(float4)))) return rc;
+        if ((rc = c->hit_buf.ensure(qcap * sizeof(float4)))) return rc;
+        if ((rc = c->shadow_buf.ensure(qcap * 4 * sizeof(float4)))) return rc;
         char* b = c->paths_buf.as<char>();
         auto take = [&](size_t bytes) {
             char* p = b;
@@ -422,15 +432,16 @@ static int ensure_wave(ag_ctx* c, uint32_t cap, int n_keys, uint32_t max_depth) 
         c->P.si = (uint32_t*)take((size_t)cap * 4);
         float4* r = c->rays_buf.as<float4>();
         c->Q.ray_o[0] = r;
-        c->Q.ray_d[0] = r + cap;
-        c->Q.ray_o[1] = r + 2 * (size_t)cap;
-        c->Q.ray_d[1] = r + 3 * (size_t)cap;
+        c->Q.ray_d[0] = r + qcap;
+        c->Q.ray_o[1] = r + 2 * qcap;
+        c->Q.ray_d[1] = r + 3 * qcap;
         c->Q.hit = c->hit_buf.as<float4>();
         float4* sh = c->shadow_buf.as<float4>();
         c->Q.sh_o = sh;
-        c->Q.sh_d = sh + cap;
-        c->Q.sh_c = sh + 2 * (size_t)cap;
-        c->Q.sh_x = sh + 3 * (size_t)cap;
+        c->Q.sh_d = sh + qcap;
+        c->Q.sh_c = sh + 2 * qcap;
+        c->Q.sh_x = sh + 3 * qcap;
+        c->Q.qcap = (uint32_t)qcap;
         c->cap = cap;
         c->n_keys_alloc = 0;
     }
@@ -1480,7 +1491,8 @@ size_t wave_state_bytes(const ag_ctx* c, int n_keys, uint32_t max_depth) {
     n_keys = wave_keys(c, n_keys);
     const size_t cap = c->tuning.paths_per_wave;
     const size_t per_path = PATH_STATE_BYTES;
-    size_t b = cap * (per_path + 4 * sizeof(float4) + sizeof(float4) + 4 * sizeof(float4) + (size_t)n_keys * 4);
+    const size_t qcap = cap + queue_slack(c, (uint32_t)cap);
+    size_t b = cap * (per_path + (size_t)n_keys * 4) + qcap * (4 * sizeof(float4) + sizeof(float4) + 4 * sizeof(float4));
     b += (counter_words(max_depth) + (size_t)(max_depth + 2) * CURSORS_PER_DEPTH) * sizeof(uint32_t);
     b += MAX_SEGMENTS * sizeof(DSeg);
     const size_t blocks = (size_t)c->n_cu * std::max(c->tuning.blocks_per_cu, c->tuning.batch_blocks_per_cu);

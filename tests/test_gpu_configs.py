@@ -109,6 +109,41 @@ def test_gems_still_full_shape_matches_oracle_rows(ctx):
     assert e < 0.01 and ea < 0.01
 
 
+def test_chunked_queues_equal_exact_appends(ctx):
+    """A shading launch over at least 2^26 hits reserves its queue slots in
+    chunks per wavefront and deals its hits in runs (dead slots in each
+    chunk's tail); smaller launches append exactly.  The headline still at
+    40 spp (82.9 M paths) rendered as one wave of 2^27 path slots (chunked at
+    depths 0 and 1) and as waves of 2^24 (never chunked): films bit-identical,
+    and the emitted ray counts equal."""
+    wl = workloads.signet_still()
+    piece = wl["piece"]
+    w, h = wl["width"], wl["height"]
+    spp = 40
+    assert w * h * spp >= 1 << 26
+    ps = workloads.path_settings(wl["depth"])
+    sc = ag.Scene(ctx, piece.scene(wl["metal"], 0.07, 0.8, wl["studio"]))
+    cam, turn = piece.shot("hero", 100.0, 0.0, w / h)
+    sc.set_turntable(turn)
+    keep = ctx.tuning
+    films, rays = [], []
+    try:
+        for ppw in (1 << 27, 1 << 24):
+            ctx.set_tuning(paths_per_wave=ppw, sort_by_material=1, ordered_emission=0)
+            f = ag.Film(ctx, w, h)
+            ag.render_pass(ctx, sc, cam, f, ps, spp, 7)
+            films.append(f.sums())
+            f.close()
+            st = ctx.depth_stats(wl["depth"])
+            rays.append((st["extension_rays"], st["shadow_rays"]))
+    finally:
+        ctx.set_tuning(**keep)
+    sc.close()
+    print(f"extension rays per depth {rays[0][0][:4]}, shadow rays {rays[0][1][:4]}")
+    assert np.array_equal(films[0], films[1])
+    assert rays[0] == rays[1]
+
+
 def test_turntable_144_views_batched_equal_plain_passes(ctx):
     pos, tris, nrm = synth.signet(seed=2)
     piece = workloads.Piece(pos, tris, nrm)
