@@ -1152,6 +1152,8 @@ struct ExtendJob {
     const float4* rd;
     uint32_t* stage;  // this wavefront's STAGE ray indices
     uint8_t* skey;    // and their bucket keys
+    uint32_t* kcnt;   // per bucket key: staged rays, then ranks (MAX_KEYS)
+    uint32_t* kbase;  // per bucket key: the flush's first slot (MAX_KEYS)
     uint32_t staged;  // uniform
     // Panels and floor first (world space, strict <: panels in index order,
     // then the floor), then the piece with t_max just above their nearest
@@ -1244,7 +1246,7 @@ struct ExtendJob {
         staged += (uint32_t)__popcll(m);
         if (staged >= FLUSH_AT) flush();
     }
-    __device__ __attribute__((noinline)) void flush() {  // out of line: runs once per FLUSH_AT rays, and inlined its temporaries cost the traversal loop a wavefront per SIMD
+    AGD void flush() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1272,31 +1274,26 @@ struct ExtendJob {
             staged = 0;
             return;
         }
-        // the keys present (MAX_KEYS <= 64), then per key one atomic for all
-        // its staged rays and their slots in staging order
-        uint64_t keys = 0;
+        // per key: count the staged rays (LDS atomics), one global atomic for
+        // all of them, then each ray's slot from a fresh per-key rank
+        for (int k = lane_id(); k < Q.n_keys; k += 64) kcnt[k] = 0u;
+        __builtin_amdgcn_wave_barrier();
         for (uint32_t c = 0; c < staged; c += 64) {
             const uint32_t e = c + lane_id();
-            if (e < staged) keys |= 1ull << skey[e];
+            if (e < staged) atomicAdd(&kcnt[skey[e]], 1u);
         }
-        for (int o = 32; o > 0; o >>= 1) keys |= (uint64_t)__shfl_xor((unsigned long long)keys, o);
-        while (keys) {
-            const int k = __ffsll((unsigned long long)keys) - 1;
-            keys &= keys - 1ull;
-            uint32_t n = 0;
-            for (uint32_t c = 0; c < staged; c += 64) {
-                const uint32_t e = c + lane_id();
-                n += (uint32_t)__popcll(__ballot(e < staged && (int)skey[e] == k));
-            }
-            uint32_t base = 0;
-            if (lane_id() == 0) base = atomicAdd(cnt + CNT_KEY + k * CNT_PAD, n);
-            base = __shfl(base, 0);
-            for (uint32_t c = 0; c < staged; c += 64) {
-                const uint32_t e = c + lane_id();
-                const bool mine = e < staged && (int)skey[e] == k;
-                const uint64_t m = __ballot(mine);
-                if (mine) Q.bucket[(size_t)k * Q.cap + base + lane_prefix(m)] = stage[e];
-                base += (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+        for (int k = lane_id(); k < Q.n_keys; k += 64) {
+            const uint32_t n = kcnt[k];
+            kbase[k] = n ? atomicAdd(cnt + CNT_KEY + k * CNT_PAD, n) : 0u;
+            kcnt[k] = 0u;
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t c = 0; c < staged; c += 64) {
+            const uint32_t e = c + lane_id();
+            if (e < staged) {
+                const uint32_t k = skey[e];
+                Q.bucket[(size_t)k * Q.cap + kbase[k] + atomicAdd(&kcnt[k], 1u)] = stage[e];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1314,9 +1311,10 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_extend(const DView* _
     TRAV_STACK_DECL
     __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
     __shared__ uint8_t s_skey[TRAV_BLOCK / 64][STAGE];
+    __shared__ uint32_t s_kcnt[TRAV_BLOCK / 64][MAX_KEYS], s_kbase[TRAV_BLOCK / 64][MAX_KEYS];
     uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
     ExtendJob J{views, P, Q, depth, cnt, Q.ray_o[depth & 1], Q.ray_d[depth & 1], s_stage[threadIdx.x / 64],
-                s_skey[threadIdx.x / 64], 0u};
+                s_skey[threadIdx.x / 64], s_kcnt[threadIdx.x / 64], s_kbase[threadIdx.x / 64], 0u};
     traverse_persistent<false>(J, min(cnt[CNT_RAYS], Q.qcap), Q.cursors + (size_t)depth * CURSORS_PER_DEPTH, refill, st, spread, Q.slices);
 }
 
@@ -1867,6 +1865,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
     TRAV_STACK_DECL
     __shared__ uint32_t s_stage[TRAV_BLOCK / 64][STAGE];
     __shared__ uint8_t s_skey[TRAV_BLOCK / 64][STAGE];
+    __shared__ uint32_t s_kcnt[TRAV_BLOCK / 64][MAX_KEYS], s_kbase[TRAV_BLOCK / 64][MAX_KEYS];
     {
         uint32_t* cnt = Q.counters + depth * CNT_STRIDE;
         ShadowJob J{views, P, Q};
@@ -1877,7 +1876,7 @@ __global__ __launch_bounds__(TRAV_BLOCK) TRAV_WAVES void k_shadow_extend(const D
         const int nd = depth + 1;
         uint32_t* cnt = Q.counters + nd * CNT_STRIDE;
         ExtendJob J{views, P, Q, nd, cnt, Q.ray_o[nd & 1], Q.ray_d[nd & 1], s_stage[threadIdx.x / 64],
-                    s_skey[threadIdx.x / 64], 0u};
+                    s_skey[threadIdx.x / 64], s_kcnt[threadIdx.x / 64], s_kbase[threadIdx.x / 64], 0u};
         traverse_persistent<false>(J, min(cnt[CNT_RAYS], Q.qcap), Q.cursors + (size_t)nd * CURSORS_PER_DEPTH, refill_extend, st, spread, Q.slices);
     }
 }
